@@ -1,0 +1,207 @@
+"""DBS evaluated flips/s on one MI355X (gfx950).
+
+Workload (default): 21x21 tiles, disjoint layout (N = 10590), 9 frequencies
+26..34 GHz, synthetic complex128 G generated on the device, Bernoulli(0.5)
+initial pattern from std::mt19937(0).  One step = one DBS run from that
+initial state to convergence (no flip of the raster cycle improves mean |S11|^2),
+the window the reference times (pngf-opt/main.cpp:333-464).  Between steps the
+optimizer state is restored from a device checkpoint outside the timed region.
+
+value = committed evaluations (the reference's flip_count) / DBS seconds.
+Printed as one JSON line; see the README of the fields in the docstring of main().
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import pngf_amd as P  # noqa: E402
+from pngf_amd.problem import WORKLOADS, initial_pattern  # noqa: E402
+
+METRIC = "DBS evaluated flips/s, 21×21 tiles × 9 freqs, vs plain FP64 GPU at matched S11"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make(wl, method, max_deferred=0):
+    T, freqs = wl["T"], wl["freqs"]
+    opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred)
+    for f, hz in enumerate(freqs):
+        opt.set_frequency(f, hz)
+        opt.synth_g(f)
+    opt.set_tile_map(initial_pattern(T, 0))
+    mid = (T - 1) // 2
+    opt.set_source_port(mid, mid, 1, 2)
+    t0 = time.perf_counter()
+    opt.init_state()
+    return opt, time.perf_counter() - t0
+
+
+def roofline(opt):
+    """Dominant kernel category by event-timed duration -> roofline record."""
+    best = None
+    for k in range(6):
+        s = opt.kernel_stats(k)
+        if s["launches"] and (best is None or s["total_ms"] > best[1]["total_ms"]):
+            best = (k, s)
+    if best is None:
+        return None, {}
+    k, s = best
+    avg_ms = s["total_ms"] / s["launches"]
+    bytes_l = s["bytes"] / s["launches"]
+    flops_l = s["flops"] / s["launches"]
+    ai = flops_l / max(bytes_l, 1.0)
+    if ai > PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9):
+        ach = flops_l / (avg_ms * 1e-3) / 1e12
+        rec = dict(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                   frac=ach / PEAK_FP64_TFLOPS, traffic=None)
+    else:
+        ach = bytes_l / (avg_ms * 1e-3) / 1e9
+        rec = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                   traffic=None)
+    rec.update(kernel=P.K_NAMES[k], avg_ms=avg_ms, launches=s["launches"], algo_bytes=bytes_l,
+               algo_flops=flops_l)
+    allk = {P.K_NAMES[i]: opt.kernel_stats(i) for i in range(6)}
+    return rec, allk
+
+
+def cpu_baseline(wl, g0, cinv0, budget_s):
+    """Reference-structure FP64 CPU path (the oracle) on a bounded sample: one frequency of the
+    same problem, same G and initial state, DBS loop timed for about budget_s seconds."""
+    import oracle as O
+    T = wl["T"]
+    mid = (T - 1) // 2
+    o = O.Optimizer(wl["layout"], T, g0, ne=1)
+    o.set_tile_map(initial_pattern(T, 0))
+    o.set_source_port(mid, mid, 1, 2)
+    o.set_cinv(cinv0)
+    o.compute_PEinc()
+    o.compute_R_matrix(1)
+    r = O.dbs_run([o], O.OBJ_S11, [wl["freqs"][0]], DX, DY, DZ, max_seconds=budget_s)
+    nf = len(wl["freqs"])
+    per_freq = r["flip_count"] / r["loop_s"]
+    return dict(value=per_freq / nf, unit="flips/s", cores=O.num_threads(), kind="port",
+                sample="oracle (C++ restatement of pngfopt.hpp, FP64, OpenMP) on 1 of %d frequencies, "
+                       "same G and initial state, %d evaluations incl. %d accepts in %.1f s; "
+                       "value = single-frequency flips/s / %d (frequencies are serial in the reference)"
+                       % (nf, r["flip_count"], r["accepts"], r["loop_s"], nf))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="antenna21x21_9f", choices=sorted(WORKLOADS))
+    ap.add_argument("--method", default="lowrank", choices=["lowrank", "plain"])
+    ap.add_argument("--max-deferred", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    wl = dict(WORKLOADS[args.workload])
+    method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
+
+    log("[bench] init %s method=%s" % (args.workload, args.method))
+    opt, init_s = make(wl, method, args.max_deferred)
+    opt.checkpoint()
+    log("[bench] init %.2f s, N=%d" % (init_s, opt.N))
+    for w in range(args.warmup):
+        r = opt.dbs_run(max_evals=wl["max_evals"])
+        opt.restore()
+        log("[bench] warmup %d: %d evals %d accepts %.3f s" % (w, r["flip_count"], r["accepts"], r["seconds"]))
+    flips, secs, runs = 0, 0.0, []
+    for s in range(args.steps):
+        opt.restore()
+        r = opt.dbs_run(max_evals=wl["max_evals"])
+        flips += r["flip_count"]
+        secs += r["seconds"]
+        runs.append(r)
+        log("[bench] step %d: %d evals %d accepts %.3f s F %.6g -> %.6g" % (
+            s, r["flip_count"], r["accepts"], r["seconds"], r["F_start"], r["F_final"]))
+    value = flips / secs
+    last = runs[-1]
+    roof, allk = None, {}
+    if not args.no_profile:
+        opt.restore()
+        opt.profile(True)
+        rp = opt.dbs_run(max_evals=wl["max_evals"])
+        roof, allk = roofline(opt)
+        opt.profile(False)
+        log("[bench] profiled run %.3f s; dominant %s" % (rp["seconds"], roof and roof["kernel"]))
+    opt.close()
+    del opt
+
+    plain = None
+    g0 = cinv0 = None
+    if not args.no_plain or not args.no_cpu:
+        popt, pinit = make(wl, P.METHOD_PLAIN)
+        if not args.no_cpu:
+            g0 = popt.g(0)
+            cinv0 = popt.cinv(0)
+        if not args.no_plain:
+            pr = popt.dbs_run(max_evals=wl["max_evals"])
+            plain = dict(value=pr["flip_count"] / pr["seconds"], seconds=pr["seconds"],
+                         flip_count=pr["flip_count"], accepts=pr["accepts"], F_final=pr["F_final"],
+                         init_s=pinit,
+                         same_accept_sequence=bool(np.array_equal(pr["acc_tiles"], last["acc_tiles"])))
+            log("[bench] plain: %d evals %d accepts %.3f s F %.6g" % (pr["flip_count"], pr["accepts"],
+                                                                     pr["seconds"], pr["F_final"]))
+        popt.close()
+        del popt
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (%.0f s budget)" % args.cpu_seconds)
+        cpu = cpu_baseline(wl, g0, cinv0, args.cpu_seconds)
+        del g0, cinv0
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "flips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "tiles": wl["T"], "N": P.layout_size(wl["layout"], wl["T"]),
+                   "freqs": len(wl["freqs"]), "layout": "disjoint" if wl["layout"] == 1 else "shared",
+                   "method": args.method, "objective": "mean |S11|^2", "driver": "raster",
+                   "max_evals": wl["max_evals"]},
+        "flip_count": last["flip_count"],
+        "accepts": last["accepts"],
+        "F_start": last["F_start"],
+        "F_final": last["F_final"],
+        "accept_seconds_share": None,
+        "init_s": init_s,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "kernels": allk,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

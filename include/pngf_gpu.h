@@ -1,0 +1,197 @@
+/*
+ * pngf_gpu.h -- C ABI of the MI355X (gfx950) implementation of the pngf-opt
+ * direct-binary-search (DBS) Woodbury hot path.
+ *
+ * One handle holds every frequency of one design problem (the reference keeps
+ * one PNGF_Optimizer object per frequency, pngf-opt/main.cpp:166,224; here the
+ * frequency axis is a batch axis on the device).  All functions return an int
+ * status (PNGF_OK = 0, negative on error) and never exit or throw; the text of
+ * the last error is available from pngf_last_error().  Host arrays passed in are
+ * copied during the call and never retained.  A handle is used by one host
+ * thread at a time and owns one HIP stream; all its device work is ordered on
+ * that stream.
+ *
+ * Complex numbers are pairs of doubles (re, im), layout-compatible with
+ * std::complex<double> and numpy complex128.
+ */
+#ifndef PNGF_GPU_H
+#define PNGF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pngf_handle pngf_handle;
+typedef struct { double re, im; } pngf_c128;
+
+/* status codes */
+#define PNGF_OK            0
+#define PNGF_ERR_ARG      -1  /* bad argument (null pointer, index out of range, ...) */
+#define PNGF_ERR_IO       -2  /* G file missing or unreadable */
+#define PNGF_ERR_SIZE     -3  /* G file size does not match (N+ne-1)^2 complex128 */
+#define PNGF_ERR_SINGULAR -4  /* C or a capacitance matrix is singular (LU info > 0) */
+#define PNGF_ERR_HIP      -5  /* HIP / rocBLAS / rocSOLVER error, or no GPU */
+#define PNGF_ERR_NOMEM    -6  /* device allocation failed or exceeds mem_limit */
+#define PNGF_ERR_STATE    -7  /* call out of order (e.g. eval before init_state) */
+#define PNGF_ERR_NONFINITE -8 /* G contains NaN or Inf */
+
+/* component layout of the design region */
+#define PNGF_LAYOUT_SHARED   0 /* reference Yee grid, shared tile edges (pngfopt.hpp:86-94,624-664) */
+#define PNGF_LAYOUT_DISJOINT 1 /* 24 components owned by each tile + separate port components */
+
+/* objective reduced over frequencies */
+#define PNGF_OBJ_REF 0 /* sum_f 200|S11_f|^2 + (15 - D_f)^2 (main.cpp:277-289); needs num_eobj = 3 */
+#define PNGF_OBJ_S11 1 /* mean_f |S11_f|^2 */
+
+/* evaluation / update method */
+#define PNGF_METHOD_PLAIN   0 /* reference formulation: G rows and C^-1 columns per candidate
+                                 (pngfopt.hpp:379-539), rank-k update of C^-1 on accept (256-367) */
+#define PNGF_METHOD_LOWRANK 1 /* state W = (G-I) C^-1 kept as W0 - A M (deferred rank-k factors);
+                                 candidates scored from k x k blocks of W in O(k^3) */
+
+/* DBS visit order */
+#define PNGF_DRIVER_REF    0 /* MagicHat order, mirror-symmetric 2/4-tile flips (main.cpp:338-459) */
+#define PNGF_DRIVER_RASTER 1 /* raster order, single-tile flips, cyclic until one full cycle
+                                has no improving flip */
+
+typedef struct {
+    int32_t layout;       /* PNGF_LAYOUT_* */
+    int32_t tiles_x;      /* tiles in x (reference num_tiles_x) */
+    int32_t tiles_y;      /* tiles in y */
+    int32_t cells_x;      /* Yee cells per tile in x (reference num_cells_per_tile_x; 3) */
+    int32_t cells_y;      /* Yee cells per tile in y (3) */
+    int32_t num_eobj;     /* rows of Gobj: 1 (port voltage) or 3 (+ two NF2FF rows) */
+    int32_t max_flips;    /* tiles per flip: 1 for the raster driver, 4 for the reference driver */
+    int32_t num_freq;     /* frequencies held by the handle */
+    int32_t method;       /* PNGF_METHOD_* */
+    int32_t objective;    /* PNGF_OBJ_* */
+    int32_t driver;       /* PNGF_DRIVER_*: fixes the candidate groups (flip sets) */
+    int32_t max_deferred; /* LOWRANK: accepted flips kept as factors before they are folded
+                             into W0 (0 = choose from free memory) */
+    int32_t device;       /* HIP device ordinal */
+    int32_t reserved;
+    double dx, dy, dz;    /* Yee cell size used by the port impedance (main.cpp:157-160) */
+    double z0;            /* reference impedance for S11 (50 ohm, main.cpp:279) */
+    int64_t mem_limit;    /* refuse device allocations beyond this many bytes (0 = no limit) */
+} pngf_config;
+
+/* Creates a handle; selects the device and allocates nothing large yet.
+   Replaces the PNGF_Optimizer constructor / init bookkeeping (pngfopt.hpp:66-119). */
+int pngf_create(const pngf_config* cfg, pngf_handle** out);
+int pngf_destroy(pngf_handle* h);
+const char* pngf_last_error(const pngf_handle* h);
+/* Number of design components N for a layout (pngfopt.hpp:86-94 for SHARED). */
+int pngf_layout_size(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x, int32_t cells_y);
+
+/* Frequency of slot f in Hz (used by the directivity term and the synthetic G). */
+int pngf_set_frequency(pngf_handle* h, int32_t f, double hz);
+
+/* Reads acmefdfd's raw G file for frequency slot f: M x M complex128, column-major,
+   M = N + num_eobj - 1, no header.  G = file[0:N,0:N], Gobj rows 1.. = file[N.., 0:N].
+   Replaces PNGF_Optimizer::init's fread (pngfopt.hpp:121-184); adds a size check. */
+int pngf_load_g(pngf_handle* h, int32_t f, const char* path);
+/* Same contents from a host buffer in the file layout. */
+int pngf_set_g(pngf_handle* h, int32_t f, const pngf_c128* file_buf);
+/* Synthetic stand-in for acmefdfd's G (dyadic Green's function of an eps_r = 3.5
+   medium sampled at the component positions, softened self term), generated on the
+   device for slot f at its frequency.  See pngf_synth_g_host for the definition. */
+int pngf_synth_g(pngf_handle* h, int32_t f);
+/* Host version of the synthetic generator, written in the acmefdfd file layout
+   (M x M column-major).  The device version evaluates the same formula. */
+int pngf_synth_g_host(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x,
+                      int32_t cells_y, int32_t num_eobj, double hz, pngf_c128* file_buf);
+
+/* reference: PNGF_Optimizer::set_tile_map (pngfopt.hpp:669-674) */
+int pngf_set_tile_map(pngf_handle* h, const int32_t* tile_map);
+/* reference: PNGF_Optimizer::set_source_port (pngfopt.hpp:559-618); returns
+   PNGF_ERR_ARG where the reference calls exit(0). */
+int pngf_set_source_port(pngf_handle* h, int32_t tile_x, int32_t tile_y, int32_t y_offset, int32_t y_size);
+
+/* compute_C_matrix + LinAlg::invertMatrix + compute_PEinc + compute_R_matrix(1) for every
+   frequency (pngfopt.hpp:196-250, linalg.hpp:34-85, main.cpp:244-252).  LOWRANK also
+   forms W = (G-I) C^-1 and releases G and C^-1. */
+int pngf_init_state(pngf_handle* h);
+
+/* reference: compute_updated_Eobj (pngfopt.hpp:379-539) at every frequency, plus the
+   objective of main.cpp:401-423.  new_eobj: [num_freq][num_eobj] (may be null); F may be null. */
+int pngf_eval(pngf_handle* h, const int32_t* tiles, int32_t ntiles, pngf_c128* new_eobj, double* F);
+/* Scores ncand flips against the same state in one batch.  tiles: [ncand][max_flips],
+   ntiles: [ncand].  F_out: [ncand]; eobj_out: [ncand][num_freq][num_eobj] (may be null). */
+int pngf_eval_batch(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand,
+                    double* F_out, pngf_c128* eobj_out);
+/* Commits a flip: update_C_inv + compute_PEinc + compute_R_matrix(0) at every frequency
+   (pngfopt.hpp:256-367, 542-552, 237-250).  The flip is named explicitly instead of
+   relying on the last compute_updated_Eobj call. */
+int pngf_accept(pngf_handle* h, const int32_t* tiles, int32_t ntiles);
+
+/* Current state.  Any pointer may be null.  eobj: [num_freq][num_eobj]; vinc: [num_freq];
+   ln: [num_freq][2] (LN1inc, LN2inc); s11: [num_freq]; F: objective. */
+int pngf_get_state(pngf_handle* h, pngf_c128* eobj, pngf_c128* vinc, pngf_c128* ln,
+                   pngf_c128* s11, double* F);
+int pngf_get_tile_map(pngf_handle* h, int32_t* tile_map);
+/* C^-1 of slot f, column-major N x N (PLAIN only: LOWRANK does not keep it). */
+int pngf_get_cinv(pngf_handle* h, int32_t f, pngf_c128* out);
+/* Current W = (G-I) C^-1 of slot f, row-major N x N (LOWRANK only; folds pending factors). */
+int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out);
+
+/* G of slot f, column-major N x N (PLAIN only: LOWRANK releases G after init). */
+int pngf_get_g(pngf_handle* h, int32_t f, pngf_c128* out);
+
+/* Device-side snapshot of the mutable optimizer state (C^-1 or W, S or q, R, Eobj,
+   mask, tile map) and return to it.  No counterpart in the reference (which keeps no
+   checkpoint, main.cpp:469-472); used to repeat a DBS run from one start and to
+   resume.  The first call allocates one more copy of the N x N state per frequency. */
+int pngf_checkpoint(pngf_handle* h);
+int pngf_restore(pngf_handle* h);
+
+typedef struct {
+    int64_t max_evals;     /* stop after this many evaluations (0 = run to convergence) */
+    int32_t seed;          /* MagicHat seed for PNGF_DRIVER_REF (reference default -514213645) */
+    int32_t max_accepts;   /* capacity of the arrays below */
+    int32_t* acc_tiles;    /* [max_accepts] primary tile of each accepted flip (may be null) */
+    int64_t* acc_eval;     /* [max_accepts] flip_count at which it was accepted (may be null) */
+    double* acc_F;         /* [max_accepts] objective after each accept (may be null) */
+    int32_t batch_min;     /* PLAIN: first speculative batch after an accept (0 = default) */
+    int32_t batch_max;     /* PLAIN: largest speculative batch (0 = default) */
+} pngf_dbs_params;
+
+typedef struct {
+    int64_t flip_count;        /* committed evaluations (main.cpp:458) */
+    int64_t accepts;
+    int64_t speculative_evals; /* evaluations scored after the accepted index (wasted) */
+    int64_t batches;
+    double F_start, F_final;
+    double seconds;            /* wall time of the DBS loop (main.cpp:333-464 window) */
+    double accept_seconds;     /* device time spent in accepts (event-timed) */
+    int32_t flushes;           /* LOWRANK: times the deferred factors were folded into W0 */
+    int32_t converged;         /* 1 if stopped because no flip improves */
+} pngf_dbs_result;
+
+/* Runs the DBS loop of main.cpp:338-459 on the device with the driver chosen at create.
+   Every accepted flip is the first improving one in visit order, as in the sequential
+   reference loop; candidates are scored in speculative batches against frozen state. */
+int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r);
+
+/* Per-kernel timing collected with HIP events on the handle's stream (enable first). */
+#define PNGF_K_EVAL       0 /* candidate scoring (both methods) */
+#define PNGF_K_COLBLK     1 /* LOWRANK accept: W[:,t] from W0 and the factors */
+#define PNGF_K_ROWBLK     2 /* LOWRANK accept: W[t,:] from W0 and the factors */
+#define PNGF_K_ACC_SMALL  3 /* accept: O(kN) updates of q / S, R, blocks */
+#define PNGF_K_VCINV      4 /* PLAIN accept: V C^-1 */
+#define PNGF_K_RANKK      5 /* PLAIN accept: rank-k update of C^-1;  LOWRANK: fold of factors */
+#define PNGF_K_COUNT      6
+typedef struct {
+    int64_t launches;
+    double total_ms;       /* summed event-timed duration */
+    double bytes;          /* summed algorithmic bytes */
+    double flops;          /* summed algorithmic flops */
+} pngf_kstat;
+int pngf_profile(pngf_handle* h, int32_t enable);
+int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PNGF_GPU_H */

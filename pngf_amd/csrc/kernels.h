@@ -1,0 +1,91 @@
+// Launch wrappers for the gfx950 kernels in kernels.hip (host-callable).
+#pragma once
+#include "common.h"
+
+namespace pngf {
+
+// Per-frequency pointer tables live in device memory (arrays of nf pointers).
+struct DevTabs {
+    zd** W0;     // LOWRANK: N x ldN row-major
+    zd** Aall;   // LOWRANK: N x Kmax row-major
+    zd** Mall;   // LOWRANK: Kmax x ldN row-major
+    zd** Grow;   // PLAIN: G row-major
+    zd** Cinv;   // PLAIN: C^-1 column-major
+};
+
+// Contiguous per-frequency small arrays.
+struct DevVecs {
+    zd* q;       // [nf][ldN]    LOWRANK: W PEinc
+    zd* S;       // [nf][ldN]    PLAIN:   C^-1 PEinc
+    zd* R;       // [nf][ne][ldN]
+    zd* Gobj;    // [nf][ne][ldN]
+    zd* Einc;    // [nf][ldN]
+    zd* Eobj;    // [nf][ne]
+    zd* Dblk;    // [nf][ngroups][gmax][gmax]  LOWRANK blocks of W over group components
+    FreqConst* fc;  // [nf]
+};
+
+struct Dims {
+    int N, ldN, nf, ne, ngroups, gmax, Kmax;
+};
+
+// gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
+constexpr int kLdsGmax = 48;
+
+// ---- eval
+void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+                    const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
+                    int32_t* out_info, zd* gscratch);
+void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
+                       const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
+                       int nb, int nsplit, zd* partial, zd* out_eobj, int32_t* out_info,
+                       zd* store /* null or [nf][acc_stride] */, int acc_stride, zd* gscratch);
+void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, const zd* eobj, int nb,
+                      double* F_out, const double* F_best, int32_t* sel, double* F_sel);
+
+// ---- LOWRANK accept (all frequencies in grid.y)
+// acc buffer per frequency (zd): capinv [0,k^2), scratch [k^2,2k^2), y [2k^2,+k),
+// w = dPE - s o y [2k^2+k,+k), newEobj [2k^2+2k,+4), Rt = R[:,t] [2k^2+2k+4,+ne*k)
+void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
+                           const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
+                           int32_t* info, zd* gscratch);
+void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      zd* Mc, zd* Ar, int small_stride);
+void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      const zd* Mc, int small_stride);
+void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride);
+void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
+                         size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
+                         zd* out_single, size_t out_stride, int out_row0, int ld_out);
+void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
+                     int acc_stride, int k, int K);
+void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
+                      int k, int K);
+void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
+                        const int32_t* gtiles, int ntiles);
+
+// ---- PLAIN accept
+void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, zd* V,
+                    size_t v_stride);
+void launch_snapshot_cinvu(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t,
+                           const int32_t* s, int k, zd* CU, size_t cu_stride);
+void launch_plain_small(hipStream_t st, const Dims& d, const DevVecs& v, const zd* acc, int acc_stride,
+                        const zd* CU, size_t cu_stride, const zd* M, size_t m_stride, const int32_t* t,
+                        const int32_t* s, int k, zd* rut);
+
+// ---- init
+void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos /*[N][3]*/, double kr, zd scale,
+                    double soft, int row_major, zd* out);
+void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* Gcm, zd* C);
+void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v);
+void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out);
+void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
+                      zd* einc);
+void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe);
+void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const zd* A, const zd* x, zd* out);
+void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabs& tb, GroupTab gt, int g, int K,
+                             zd* Dblk);
+void launch_extract_blocks(hipStream_t st, const Dims& d, const zd* W0, GroupTab gt, zd* Dblk_f);
+
+}  // namespace pngf

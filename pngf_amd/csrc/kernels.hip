@@ -1,0 +1,1039 @@
+// gfx950 kernels of the DBS Woodbury path.
+//
+// Notation (pngf-opt/pngfopt.hpp): N design components, P the 0/1 metal mask,
+// C = I + P(G - I), S = C^-1 PEinc, R = Gobj C^-1, Eobj = R PEinc.  A flip changes
+// the k components t (ascending) with signs s = +-1.
+//   PLAIN   (reference formulation, pngfopt.hpp:379-539): cap = I + (G-I)[t,:] C^-1[:,t] diag(s)
+//   LOWRANK (W = (G-I) C^-1, q = W PEinc): cap = I + W[t,t] diag(s), rhs = q_t + W[t,t] dPE
+// Both give the same newEobj in exact arithmetic; see DESIGN notes in the runtime.
+#include "kernels.h"
+
+namespace pngf {
+
+// ------------------------------------------------------------------ helpers --
+
+// Components of flip group g whose metal state changes, ascending, with sign
+// s = new - old and the position in the group's component list.  Serial (one
+// thread); gmax <= 96.  reference: pngfopt.hpp:384-418 (dP scan / P_indices).
+__device__ int change_set(const GroupTab& gt, int g, const int32_t* tile, const int8_t* P,
+                          const int32_t* owners, int* t, int* sg, int* lp) {
+    const int nt = gt.ntiles[g];
+    const int32_t* gtl = gt.tiles + 4 * g;
+    const int nc = gt.ncomp[g];
+    const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
+    int k = 0;
+    for (int a = 0; a < nc; a++) {
+        const int i = cl[a];
+        int nv = 0;
+        for (int h = 0; h < 2; h++) {
+            const int o = owners[2 * i + h];
+            if (o < 0) continue;
+            int v = tile[o];
+            for (int q = 0; q < nt; q++) v ^= (gtl[q] == o) ? 1 : 0;
+            nv |= v;
+        }
+        const int ov = P[i];
+        if (nv != ov) { t[k] = i; sg[k] = nv - ov; lp[k] = a; k++; }
+    }
+    return k;
+}
+
+// LU factorisation with partial pivoting of the k x k leading block of the
+// row-major LDS matrix a (leading dimension lda), applying the row operations to
+// columns k..ncol-1 as well.  Pivot = first row of max |Re|+|Im| (LAPACK
+// izamax / zgetf2); rows are swapped across all ncol columns and the multipliers
+// l = a_ij * (1/a_jj) are stored below the diagonal.  piv[j] = pivot row.
+// All threads of the block call it.  Returns LAPACK-style info (0 or j+1).
+__device__ int lds_lu(zd* a, int lda, int k, int ncol, int* piv) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    __shared__ int s_info;
+    if (tid == 0) s_info = 0;
+    for (int j = 0; j < k; j++) {
+        if (tid < 64) {
+            double best = -1.0; int bi = 0x7fffffff;
+            for (int i = j + tid; i < k; i += 64) {
+                double v = zabs1(a[i * lda + j]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                double ov = __shfl_xor(best, off, 64);
+                int oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (tid == 0) piv[j] = bi;
+        }
+        __syncthreads();
+        const int p = piv[j];
+        if (p != j)
+            for (int c = tid; c < ncol; c += nth) { zd x = a[j * lda + c]; a[j * lda + c] = a[p * lda + c]; a[p * lda + c] = x; }
+        __syncthreads();
+        const zd pv = a[j * lda + j];
+        if (pv.x == 0.0 && pv.y == 0.0) {
+            if (tid == 0 && s_info == 0) s_info = j + 1;
+            __syncthreads();
+            continue;
+        }
+        const zd r = zrecip(pv);
+        for (int i = j + 1 + tid; i < k; i += nth) a[i * lda + j] = zmul(a[i * lda + j], r);
+        __syncthreads();
+        const int w = ncol - j - 1, h = k - j - 1;
+        for (int e = tid; e < w * h; e += nth) {
+            const int i = j + 1 + e / w, c = j + 1 + e % w;
+            a[i * lda + c] = zfms(a[i * lda + j], a[j * lda + c], a[i * lda + c]);
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    return s_info;
+}
+
+// Back substitution U x = c for the column `col` of the factored LDS matrix.
+__device__ void lds_backsolve(zd* a, int lda, int k, int col) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int j = k - 1; j >= 0; j--) {
+        if (tid == 0) a[j * lda + col] = zdiv(a[j * lda + col], a[j * lda + j]);
+        __syncthreads();
+        const zd xj = a[j * lda + col];
+        for (int i = tid; i < j; i += nth) a[i * lda + col] = zfms(a[i * lda + j], xj, a[i * lda + col]);
+        __syncthreads();
+    }
+}
+
+// In-place inverse from the LU factors (zgetri: inv(U), then solve X L = inv(U),
+// then undo the row swaps as column swaps).  tmp: LDS scratch of k entries.
+__device__ void lds_getri(zd* a, int lda, int k, const int* piv, zd* tmp) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int j = 0; j < k; j++) {
+        if (tid == 0) a[j * lda + j] = zrecip(a[j * lda + j]);
+        __syncthreads();
+        const zd ajj = zmk(-a[j * lda + j].x, -a[j * lda + j].y);
+        for (int i = tid; i < j; i += nth) {
+            zd s = zmk(0.0, 0.0);
+            for (int l = i; l < j; l++) s = zfma(a[i * lda + l], a[l * lda + j], s);
+            tmp[i] = s;
+        }
+        __syncthreads();
+        for (int i = tid; i < j; i += nth) a[i * lda + j] = zmul(tmp[i], ajj);
+        __syncthreads();
+    }
+    for (int j = k - 2; j >= 0; j--) {
+        for (int i = j + 1 + tid; i < k; i += nth) { tmp[i] = a[i * lda + j]; a[i * lda + j] = zmk(0.0, 0.0); }
+        __syncthreads();
+        for (int i = tid; i < k; i += nth) {
+            zd s = a[i * lda + j];
+            for (int l = j + 1; l < k; l++) s = zfms(a[i * lda + l], tmp[l], s);
+            a[i * lda + j] = s;
+        }
+        __syncthreads();
+    }
+    for (int j = k - 2; j >= 0; j--) {
+        const int p = piv[j];
+        if (p != j)
+            for (int i = tid; i < k; i += nth) { zd x = a[i * lda + j]; a[i * lda + j] = a[i * lda + p]; a[i * lda + p] = x; }
+        __syncthreads();
+    }
+}
+
+// The k x (k+1) capacitance | rhs matrix lives in LDS when gmax <= 48 (37 KB);
+// larger flip sets (mirror-symmetric 4-tile flips, k <= 96) use a per-block slot
+// of a global scratch buffer instead.
+static size_t lds_matrix_bytes(int gmax, const zd* gscratch) {
+    return gscratch ? 0 : (size_t)gmax * (gmax + 1) * sizeof(zd);
+}
+
+// --------------------------------------------------------- LOWRANK eval ------
+// One workgroup per (candidate b, frequency f): cap = I + D[t,t] diag(s),
+// rhs = q_t + D[t,t] dPE, y = cap^-1 rhs, newEobj = Eobj + R[:,t] (dPE - s y),
+// where dPE = -s Einc_t and D the maintained block of W over the group's components.
+__global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
+                               const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                               const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
+                               int32_t* __restrict__ out_info, zd* gscratch) {
+    extern __shared__ zd smz[];
+    const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nth = blockDim.x;
+    const int gmax = d.gmax;
+    zd* dpe = smz;                                // [gmax]
+    int* t = (int*)(dpe + gmax);
+    int* sg = t + gmax;
+    int* lp = sg + gmax;
+    int* piv = lp + gmax;
+    // [k][k+1] capacitance | rhs: LDS when it fits, else this block's global slot
+    zd* a = gscratch ? gscratch + ((size_t)f * gridDim.x + b) * gmax * (gmax + 1) : (zd*)(piv + gmax);
+    __shared__ int s_k;
+    const int g = batch[b];
+    if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const int k = s_k;
+    const int lda = k + 1;
+    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    const zd* q = v.q + (size_t)f * d.ldN;
+    const zd* einc = v.Einc + (size_t)f * d.ldN;
+    for (int c = tid; c < k; c += nth) {
+        zd e = einc[t[c]];
+        dpe[c] = sg[c] > 0 ? zmk(-e.x, -e.y) : e;
+    }
+    __syncthreads();
+    for (int e = tid; e < k * k; e += nth) {
+        const int r = e / k, c = e - r * k;
+        zd w = D[lp[r] * gmax + lp[c]];
+        zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
+        if (r == c) val.x += 1.0;
+        a[r * lda + c] = val;
+    }
+    for (int r = tid; r < k; r += nth) {
+        zd acc = q[t[r]];
+        for (int c = 0; c < k; c++) acc = zfma(D[lp[r] * gmax + lp[c]], dpe[c], acc);
+        a[r * lda + k] = acc;
+    }
+    __syncthreads();
+    int info = lds_lu(a, lda, k, k + 1, piv);
+    lds_backsolve(a, lda, k, k);
+    if (tid < d.ne) {
+        const int r = tid;
+        const zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
+        zd acc = v.Eobj[f * d.ne + r];
+        for (int c = 0; c < k; c++) {
+            zd y = a[c * lda + k];
+            zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+            acc = zfma(R[t[c]], w, acc);
+        }
+        out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
+    }
+    if (tid == 0 && out_info) out_info[b * d.nf + f] = info;
+}
+
+void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+                    const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
+                    int32_t* out_info, zd* gscratch) {
+    dim3 grid(nb, d.nf);
+    int nth = d.gmax <= 32 ? 64 : 256;
+    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
+    hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
+                       out_info, gscratch);
+}
+
+// ------------------------------------------------------------ PLAIN eval -----
+// Split-N streaming pass (reference pngfopt.hpp:420-480, fused): for each n in the
+// split, stage V = (G-I)[t, n-chunk] (rows of the row-major G copy) and
+// C^-1[n-chunk, t] (columns of the column-major C^-1) in LDS, form
+// Snew = S - C^-1[:,t] (s o Einc_t) on the fly and accumulate the augmented
+// product  [V ; Gobj] x [C^-1[:,t] , Snew]  whose blocks are V C^-1[:,t] (the
+// capacitance), V Snew and Gobj Snew.  Each thread owns a 4x4 block of the product
+// and a strided subset of the chunk; partials are reduced in LDS in a fixed order.
+constexpr int PL_TILE = 48;     // rows / cols of the product per workgroup tile
+
+__global__ void __launch_bounds__(256) plain_partial_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
+        const int32_t* __restrict__ tile, const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+        const int32_t* __restrict__ batch, int nsplit, int PL_NC, zd* __restrict__ partial) {
+    extern __shared__ char smem[];
+    const int item = blockIdx.x, split = blockIdx.y, ttile = blockIdx.z;
+    const int b = item / d.nf, f = item - b * d.nf;
+    const int tid = threadIdx.x;
+    const int gmax = d.gmax, ne = d.ne;
+    int* t = (int*)smem;
+    int* sg = t + 96;
+    int* lp = sg + 96;
+    __shared__ int s_k;
+    zd* es = (zd*)(lp + 96);              // s o Einc_t   [96]
+    zd* Cc = es + 96;                     // [k+1][PL_NC]   C^-1 columns + Snew
+    const int g = batch[b];
+    if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const int k = s_k;
+    const int nrow = k + ne, ncolp = k + 1;
+    const int ntr = (nrow + PL_TILE - 1) / PL_TILE, ntc = (ncolp + PL_TILE - 1) / PL_TILE;
+    if (ttile >= ntr * ntc) return;
+    const int r0 = (ttile / ntc) * PL_TILE, c0 = (ttile % ntc) * PL_TILE;
+    const int rows = min(PL_TILE, nrow - r0), cols = min(PL_TILE, ncolp - c0);
+    const int nrb = (rows + 3) >> 2, ncb = (cols + 3) >> 2, nblk = nrb * ncb;
+    const int nsub = 256 / nblk;
+    zd* Vt = Cc + (size_t)(k + 1) * PL_NC;     // [rows_pad][PL_NC]
+    const int rows_pad = nrb * 4, cols_pad = ncb * 4;
+    const zd* einc = v.Einc + (size_t)f * d.ldN;
+    for (int c = tid; c < k; c += 256) { zd e = einc[t[c]]; es[c] = sg[c] > 0 ? e : zmk(-e.x, -e.y); }
+
+    const zd* Gr = tb.Grow[f];
+    const zd* Ci = tb.Cinv[f];
+    const zd* S = v.S + (size_t)f * d.ldN;
+    const zd* Go = v.Gobj + (size_t)f * ne * d.ldN;
+
+    const int per = (d.N + nsplit - 1) / nsplit;
+    const int n_begin = split * per, n_end = min(d.N, n_begin + per);
+
+    const int blk = tid / nsub, sub = tid - blk * nsub;
+    const bool active = blk < nblk;
+    const int rb = active ? blk / ncb : 0, cb = active ? blk - rb * ncb : 0;
+    zd acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = zmk(0.0, 0.0);
+
+    __syncthreads();
+    for (int nb0 = n_begin; nb0 < n_end; nb0 += PL_NC) {
+        const int nlen = min(PL_NC, n_end - nb0);
+        // stage C^-1 columns (all k: Snew needs them) and the tile's V / Gobj rows
+        for (int e = tid; e < k * PL_NC; e += 256) {
+            const int j = e / PL_NC, nn = e - j * PL_NC;
+            Cc[j * PL_NC + nn] = nn < nlen ? Ci[(size_t)t[j] * d.ldN + nb0 + nn] : zmk(0.0, 0.0);
+        }
+        for (int e = tid; e < rows_pad * PL_NC; e += 256) {
+            const int i = e / PL_NC, nn = e - i * PL_NC;
+            const int gi = r0 + i, n = nb0 + nn;
+            zd val = zmk(0.0, 0.0);
+            if (nn < nlen && i < rows) {
+                if (gi < k) { val = Gr[(size_t)t[gi] * d.ldN + n]; if (n == t[gi]) val.x -= 1.0; }
+                else val = Go[(size_t)(gi - k) * d.ldN + n];
+            }
+            Vt[i * PL_NC + nn] = val;
+        }
+        __syncthreads();
+        for (int nn = tid; nn < PL_NC; nn += 256) {
+            zd s = nn < nlen ? S[nb0 + nn] : zmk(0.0, 0.0);
+            for (int j = 0; j < k; j++) s = zfms(Cc[j * PL_NC + nn], es[j], s);
+            Cc[k * PL_NC + nn] = nn < nlen ? s : zmk(0.0, 0.0);
+        }
+        __syncthreads();
+        if (active) {
+            for (int nn = sub; nn < nlen; nn += nsub) {
+                zd vr[4], cc[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) vr[i] = Vt[(rb * 4 + i) * PL_NC + nn];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int gc = c0 + cb * 4 + j;
+                    cc[j] = gc < ncolp ? Cc[gc * PL_NC + nn] : zmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = zfma(vr[i], cc[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    // fixed-order reduction of the nsub partial blocks through LDS
+    zd* red = Cc;  // reuse: rows_pad x cols_pad entries
+    for (int e = tid; e < rows_pad * cols_pad; e += 256) red[e] = zmk(0.0, 0.0);
+    __syncthreads();
+    for (int sround = 0; sround < nsub; sround++) {
+        if (active && sub == sround) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    zd* p = &red[(rb * 4 + i) * cols_pad + cb * 4 + j];
+                    *p = zadd(*p, acc[i][j]);
+                }
+        }
+        __syncthreads();
+    }
+    // write this split's partial: full (gmax+ne) x (gmax+1) layout per (item, split)
+    const int pstride = (gmax + ne) * (gmax + 1);
+    zd* out = partial + ((size_t)item * nsplit + split) * pstride;
+    for (int e = tid; e < rows * cols; e += 256) {
+        const int i = e / cols, j = e - i * cols;
+        out[(r0 + i) * (gmax + 1) + c0 + j] = red[i * cols_pad + j];
+    }
+}
+
+static int plain_nc(int gmax) { return gmax <= 32 ? 32 : 16; }
+
+static size_t plain_partial_smem(int gmax, int ne) {
+    const int PL_NC = plain_nc(gmax);
+    const int rp = ((gmax + ne + 3) / 4) * 4, cp = ((gmax + 1 + 3) / 4) * 4;
+    const int rows_pad = rp < PL_TILE ? rp : PL_TILE, cols_pad = cp < PL_TILE ? cp : PL_TILE;
+    size_t chunk = (size_t)(gmax + 1) * PL_NC + (size_t)rows_pad * PL_NC;
+    size_t red = (size_t)rows_pad * cols_pad;
+    return (size_t)3 * 96 * sizeof(int) + 96 * sizeof(zd) + (chunk > red ? chunk : red) * sizeof(zd);
+}
+
+// Reduce the split partials in a fixed order, then cap = I + (V C^-1[:,t]) diag(s),
+// y = cap^-1 (V Snew) (LU, pngfopt.hpp:488-511) and
+// newEobj = Gobj Snew - R[:,t] diag(s) y (pngfopt.hpp:513-537).  With `store`,
+// also keeps the factors for the accept in the acc layout (capinv, y, newEobj).
+__global__ void plain_finish_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
+                                    const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                                    const int32_t* __restrict__ batch, int nsplit, const zd* __restrict__ partial,
+                                    zd* __restrict__ out_eobj, int32_t* __restrict__ out_info, zd* store,
+                                    int acc_stride, zd* gscratch) {
+    extern __shared__ char smem[];
+    const int item = blockIdx.x, b = item / d.nf, f = item - b * d.nf;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int gmax = d.gmax, ne = d.ne;
+    int* t = (int*)smem;
+    int* sg = t + 96;
+    int* lp = sg + 96;
+    int* piv = lp + 96;
+    __shared__ int s_k;
+    zd* gs = (zd*)(piv + 96);       // [ne]
+    zd* tmp = gs + 4;               // [96]
+    zd* a = gscratch ? gscratch + (size_t)item * gmax * (gmax + 1) : tmp + 96;  // [k][k+1]
+    const int g = batch[b];
+    if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const int k = s_k, lda = k + 1;
+    const int pstride = (gmax + ne) * (gmax + 1);
+    const zd* part = partial + (size_t)item * nsplit * pstride;
+    for (int e = tid; e < (k + ne) * (k + 1); e += nth) {
+        const int i = e / (k + 1), j = e - i * (k + 1);
+        zd s = zmk(0.0, 0.0);
+        for (int sp = 0; sp < nsplit; sp++) s = zadd(s, part[(size_t)sp * pstride + i * (gmax + 1) + j]);
+        if (i < k) {
+            if (j < k) {
+                if (sg[j] < 0) s = zmk(-s.x, -s.y);
+                if (i == j) s.x += 1.0;
+            }
+            a[i * lda + j] = s;
+        } else if (j == k) {
+            gs[i - k] = s;
+        }
+    }
+    __syncthreads();
+    int info = lds_lu(a, lda, k, k + 1, piv);
+    lds_backsolve(a, lda, k, k);
+    if (tid < ne) {
+        const int r = tid;
+        const zd* R = v.R + ((size_t)f * ne + r) * d.ldN;
+        zd acc = gs[r];
+        for (int c = 0; c < k; c++) {
+            zd ru = R[t[c]];
+            if (sg[c] < 0) ru = zmk(-ru.x, -ru.y);
+            acc = zfms(ru, a[c * lda + k], acc);
+        }
+        out_eobj[((size_t)b * d.nf + f) * ne + r] = acc;
+        if (store) store[(size_t)f * acc_stride + 2 * k * k + 2 * k + r] = acc;  // newEobj slot
+    }
+    if (tid == 0 && out_info) out_info[item] = info;
+    if (store) {
+        zd* st = store + (size_t)f * acc_stride;
+        for (int c = tid; c < k; c += nth) st[2 * k * k + c] = a[c * lda + k];  // y
+        __syncthreads();
+        lds_getri(a, lda, k, piv, tmp);
+        for (int e = tid; e < k * k; e += nth) { const int r = e / k, c = e - r * k; st[e] = a[r * lda + c]; }
+    }
+}
+
+void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
+                       const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
+                       int nb, int nsplit, zd* partial, zd* out_eobj, int32_t* out_info, zd* store,
+                       int acc_stride, zd* gscratch) {
+    const int nitems = nb * d.nf;
+    const int ntiles_max = ((d.gmax + d.ne + PL_TILE - 1) / PL_TILE) * ((d.gmax + 1 + PL_TILE - 1) / PL_TILE);
+    hipLaunchKernelGGL(plain_partial_kernel, dim3(nitems, nsplit, ntiles_max), dim3(256),
+                       plain_partial_smem(d.gmax, d.ne), st, d, t, v, gt, tile, P, owners, batch, nsplit,
+                       plain_nc(d.gmax), partial);
+    size_t sm = 4 * 96 * sizeof(int) + 100 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
+    hipLaunchKernelGGL(plain_finish_kernel, dim3(nitems), dim3(d.gmax <= 32 ? 64 : 256), sm, st, d, v, gt, tile,
+                       P, owners, batch, nsplit, partial, out_eobj, out_info, store, acc_stride, gscratch);
+}
+
+// ------------------------------------------------------------- objective -----
+// One thread per candidate: objective_of (common.h); then the first candidate in
+// batch order with F < F_best.
+__global__ void objective_kernel(ObjParams op, const FreqConst* __restrict__ fc, const zd* __restrict__ eobj,
+                                 int nb, double* __restrict__ F_out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    F_out[b] = objective_of(op, fc, eobj + (size_t)b * op.nf * op.ne);
+}
+
+__global__ void select_kernel(const double* __restrict__ F, int nb, const double* __restrict__ F_best,
+                              int32_t* sel, double* F_sel) {
+    __shared__ int red[256];
+    const int tid = threadIdx.x;
+    const double fb = *F_best;
+    int best = 0x7fffffff;
+    for (int b = tid; b < nb; b += 256)
+        if (F[b] < fb) { best = b; break; }
+    red[tid] = best;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] = min(red[tid], red[tid + off]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int bi = red[0];
+        *sel = bi == 0x7fffffff ? -1 : bi;
+        *F_sel = bi == 0x7fffffff ? fb : F[bi];
+    }
+}
+
+void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, const zd* eobj, int nb,
+                      double* F_out, const double* F_best, int32_t* sel, double* F_sel) {
+    hipLaunchKernelGGL(objective_kernel, dim3((nb + 127) / 128), dim3(128), 0, st, op, fc, eobj, nb, F_out);
+    if (sel) hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, st, F_out, nb, F_best, sel, F_sel);
+}
+
+// ------------------------------------------------------- LOWRANK accept -----
+// acc layout per frequency (acc_stride zd):
+//   [0, k*k)            capinv (row-major)
+//   [k*k, 2k*k)         scratch LU
+//   [2k*k, 2k*k+k)      y
+//   [2k*k+k, 2k*k+2k)   w = dPE - s o y
+//   [2k*k+2k, +ne)      newEobj
+//   [2k*k+2k+4, +ne*k)  Rt = R[:, t] (snapshot)
+__global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restrict__ tt,
+                               const int32_t* __restrict__ ss, const int32_t* __restrict__ lpp, int k, zd* acc,
+                               int acc_stride, int32_t* info_out, zd* gscratch) {
+    extern __shared__ char smem[];
+    const int f = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
+    const int gmax = d.gmax, ne = d.ne;
+    int* piv = (int*)smem;
+    zd* dpe = (zd*)(piv + 100);
+    zd* tmp = dpe + 96;
+    zd* a = gscratch ? gscratch + (size_t)f * gmax * (gmax + 1) : tmp + 96;
+    const int lda = k + 1;
+    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    const zd* q = v.q + (size_t)f * d.ldN;
+    const zd* einc = v.Einc + (size_t)f * d.ldN;
+    zd* st = acc + (size_t)f * acc_stride;
+    for (int c = tid; c < k; c += nth) { zd e = einc[tt[c]]; dpe[c] = ss[c] > 0 ? zmk(-e.x, -e.y) : e; }
+    __syncthreads();
+    for (int e = tid; e < k * k; e += nth) {
+        const int r = e / k, c = e - r * k;
+        zd w = D[lpp[r] * gmax + lpp[c]];
+        zd val = ss[c] > 0 ? w : zmk(-w.x, -w.y);
+        if (r == c) val.x += 1.0;
+        a[r * lda + c] = val;
+    }
+    for (int r = tid; r < k; r += nth) {
+        zd s = q[tt[r]];
+        for (int c = 0; c < k; c++) s = zfma(D[lpp[r] * gmax + lpp[c]], dpe[c], s);
+        a[r * lda + k] = s;
+    }
+    __syncthreads();
+    int info = lds_lu(a, lda, k, k + 1, piv);
+    lds_backsolve(a, lda, k, k);
+    for (int c = tid; c < k; c += nth) {
+        zd y = a[c * lda + k];
+        st[2 * k * k + c] = y;
+        st[2 * k * k + k + c] = ss[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+    }
+    for (int e = tid; e < ne * k; e += nth) {
+        const int r = e / k, c = e - r * k;
+        st[2 * k * k + 2 * k + 4 + e] = v.R[((size_t)f * ne + r) * d.ldN + tt[c]];
+    }
+    __syncthreads();
+    if (tid < ne) {
+        const int r = tid;
+        zd s = v.Eobj[f * ne + r];
+        for (int c = 0; c < k; c++) s = zfma(st[2 * k * k + 2 * k + 4 + r * k + c], st[2 * k * k + k + c], s);
+        st[2 * k * k + 2 * k + r] = s;
+        v.Eobj[f * ne + r] = s;
+    }
+    lds_getri(a, lda, k, piv, tmp);
+    for (int e = tid; e < k * k; e += nth) { const int r = e / k, c = e - r * k; st[e] = a[r * lda + c]; }
+    if (tid == 0 && info_out) info_out[f] = info;
+}
+
+void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
+                           const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
+                           int32_t* info, zd* gscratch) {
+    size_t sm = 100 * sizeof(int) + 192 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
+    hipLaunchKernelGGL(lr_prep_kernel, dim3(d.nf), dim3(k <= 32 ? 64 : 256), sm, st, d, v, g, t, s, lp, k, acc,
+                       acc_stride, info, gscratch);
+}
+
+// Mc[K'][c] = M_all[K'][t_c] (K x k) and Ar[c][K'] = A_all[t_c][K'] (k x K).
+__global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, int K, zd* Mc,
+                                 zd* Ar, int small_stride) {
+    const int f = blockIdx.y;
+    const zd* M = tb.Mall[f];
+    const zd* A = tb.Aall[f];
+    zd* mc = Mc + (size_t)f * small_stride;
+    zd* ar = Ar + (size_t)f * small_stride;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * k; e += gridDim.x * blockDim.x) {
+        const int kk = e / k, c = e - kk * k;
+        mc[e] = M[(size_t)kk * d.ldN + t[c]];
+        const int c2 = e / K, k2 = e - c2 * K;
+        ar[e] = A[(size_t)t[c2] * d.Kmax + k2];
+    }
+}
+
+void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      zd* Mc, zd* Ar, int small_stride) {
+    if (K == 0) return;
+    int blocks = min(1024, (K * k + 255) / 256);
+    hipLaunchKernelGGL(lr_gather_kernel, dim3(blocks, d.nf), dim3(256), 0, st, d, tb, t, k, K, Mc, Ar,
+                       small_stride);
+}
+
+// Skinny products over the deferred factors (k <= 96 columns):
+//   colblk: A_all[n][K+c] = W0[n][t_c] - sum_K' A_all[n][K'] Mc[K'][c]
+//   rowblk: Bt[n][c]      = W0[t_c][n] - sum_K' Ar[c][K'] M_all[K'][n]
+// 64 output rows per workgroup, K' staged 32 at a time through LDS; each thread
+// owns 2 rows x ceil(k/8) columns.
+constexpr int SK_R = 64, SK_K = 32;
+
+template <int KU, bool ROW>
+__global__ void __launch_bounds__(256) lr_skinny_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
+                                                        int K, const zd* __restrict__ small, int small_stride,
+                                                        zd* Bt, size_t bt_stride) {
+    __shared__ zd As[SK_R][SK_K + 1];
+    __shared__ zd Bs[SK_K][KU * 8 + 1];
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const int n0 = blockIdx.x * SK_R;
+    const int rp = tid >> 3, cg = tid & 7;
+    const zd* A = tb.Aall[f];
+    const zd* M = tb.Mall[f];
+    const zd* W0 = tb.W0[f];
+    const zd* sm = small + (size_t)f * small_stride;
+    zd acc[2][KU];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int u = 0; u < KU; u++) acc[i][u] = zmk(0.0, 0.0);
+    for (int k0 = 0; k0 < K; k0 += SK_K) {
+        const int kl = min(SK_K, K - k0);
+        for (int e = tid; e < SK_R * SK_K; e += 256) {
+            int r, kk;
+            zd val = zmk(0.0, 0.0);
+            if (!ROW) {
+                r = e / SK_K; kk = e - r * SK_K;
+                const int n = n0 + r;
+                if (n < d.N && kk < kl) val = A[(size_t)n * d.Kmax + k0 + kk];
+            } else {
+                kk = e / SK_R; r = e - kk * SK_R;
+                const int n = n0 + r;
+                if (n < d.N && kk < kl) val = M[(size_t)(k0 + kk) * d.ldN + n];
+            }
+            As[r][kk] = val;
+        }
+        for (int e = tid; e < SK_K * KU * 8; e += 256) {
+            const int kk = e / (KU * 8), c = e - kk * (KU * 8);
+            zd val = zmk(0.0, 0.0);
+            if (kk < kl && c < k) val = ROW ? sm[(size_t)c * K + k0 + kk] : sm[(size_t)(k0 + kk) * k + c];
+            Bs[kk][c] = val;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kl; kk++) {
+            const zd a0 = As[2 * rp][kk], a1 = As[2 * rp + 1][kk];
+#pragma unroll
+            for (int u = 0; u < KU; u++) {
+                const zd m = Bs[kk][cg + 8 * u];
+                acc[0][u] = zfma(a0, m, acc[0][u]);
+                acc[1][u] = zfma(a1, m, acc[1][u]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int n = n0 + 2 * rp + i;
+        if (n >= d.N) continue;
+#pragma unroll
+        for (int u = 0; u < KU; u++) {
+            const int c = cg + 8 * u;
+            if (c >= k) continue;
+            if (!ROW) {
+                zd w = W0[(size_t)n * d.ldN + t[c]];
+                ((zd*)A)[(size_t)n * d.Kmax + K + c] = zsub(w, acc[i][u]);
+            } else {
+                zd w = W0[(size_t)t[c] * d.ldN + n];
+                Bt[(size_t)f * bt_stride + (size_t)n * d.gmax + c] = zsub(w, acc[i][u]);
+            }
+        }
+    }
+}
+
+template <bool ROW>
+static void launch_skinny(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                          const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
+    dim3 grid((d.N + SK_R - 1) / SK_R, d.nf);
+    if (k <= 24)
+        hipLaunchKernelGGL((lr_skinny_kernel<3, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+    else if (k <= 48)
+        hipLaunchKernelGGL((lr_skinny_kernel<6, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+    else
+        hipLaunchKernelGGL((lr_skinny_kernel<12, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+}
+
+void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      const zd* Mc, int small_stride) {
+    launch_skinny<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0);
+}
+void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride) {
+    launch_skinny<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride);
+}
+
+// out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb)
+// PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
+// LOWRANK: M_new = diag(s) cap^-1 W[t,:]
+__global__ void apply_capinv_kernel(Dims d, const zd* __restrict__ acc, int acc_stride, const zd* __restrict__ B,
+                                    size_t b_stride, int ldb, int k, const int32_t* __restrict__ s,
+                                    zd* const* out_tab, zd* out_single, size_t out_stride, int out_row0, int ld_out) {
+    __shared__ zd Bs[32][97];
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const int n0 = blockIdx.x * 32;
+    const zd* Bf = B + (size_t)f * b_stride;
+    const zd* ci = acc + (size_t)f * acc_stride;
+    for (int e = tid; e < 32 * k; e += 256) {
+        const int nn = e / k, l = e - nn * k;
+        const int n = n0 + nn;
+        Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+    zd* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
+    for (int e = tid; e < 32 * k; e += 256) {
+        const int c = e / 32, nn = e - c * 32;
+        const int n = n0 + nn;
+        if (n >= d.N) continue;
+        zd sacc = zmk(0.0, 0.0);
+        for (int l = 0; l < k; l++) sacc = zfma(ci[c * k + l], Bs[nn][l], sacc);
+        if (s && s[c] < 0) sacc = zmk(-sacc.x, -sacc.y);
+        out[(size_t)(out_row0 + c) * ld_out + n] = sacc;
+    }
+}
+
+void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
+                         size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
+                         zd* out_single, size_t out_stride, int out_row0, int ld_out) {
+    dim3 grid((d.N + 31) / 32, d.nf);
+    hipLaunchKernelGGL(apply_capinv_kernel, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, ldb, k, s,
+                       out_tab, out_single, out_stride, out_row0, ld_out);
+}
+
+// q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept)
+__global__ void lr_small_kernel(Dims d, DevTabs tb, DevVecs v, const zd* __restrict__ acc, int acc_stride, int k,
+                                int K) {
+    const int f = blockIdx.y;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= d.N) return;
+    const zd* st = acc + (size_t)f * acc_stride;
+    const zd* w = st + 2 * k * k + k;
+    const zd* Rt = st + 2 * k * k + 2 * k + 4;
+    const zd* A = tb.Aall[f] + (size_t)n * d.Kmax + K;
+    const zd* M = tb.Mall[f] + (size_t)K * d.ldN + n;
+    zd qa = v.q[(size_t)f * d.ldN + n];
+    for (int c = 0; c < k; c++) qa = zfma(A[c], w[c], qa);
+    v.q[(size_t)f * d.ldN + n] = qa;
+    for (int r = 0; r < d.ne; r++) {
+        zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
+        zd ra = R[n];
+        for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], M[(size_t)c * d.ldN], ra);
+        R[n] = ra;
+    }
+}
+
+void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
+                     int acc_stride, int k, int K) {
+    dim3 grid((d.N + 255) / 256, d.nf);
+    hipLaunchKernelGGL(lr_small_kernel, grid, dim3(256), 0, st, d, tb, v, acc, acc_stride, k, K);
+}
+
+// D_h -= A_new[comp_h, :] M_new[:, comp_h] for every group h.
+template <int PER>
+__global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt, int k, int K) {
+    __shared__ zd Ac[96][17];
+    __shared__ zd Mc[16][97];
+    const int h = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int nc = gt.ncomp[h];
+    if (nc == 0) return;
+    const int32_t* cl = gt.comps + (size_t)h * gt.gmax;
+    const zd* A = tb.Aall[f];
+    const zd* M = tb.Mall[f];
+    zd* D = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
+    zd acc[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) acc[u] = zmk(0.0, 0.0);
+    for (int c0 = 0; c0 < k; c0 += 16) {
+        const int cl_ = min(16, k - c0);
+        for (int e = tid; e < nc * 16; e += 256) {
+            const int a = e / 16, c = e - a * 16;
+            Ac[a][c] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zmk(0.0, 0.0);
+        }
+        for (int e = tid; e < 16 * nc; e += 256) {
+            const int c = e / nc, bb = e - c * nc;
+            Mc[c][bb] = c < cl_ ? M[(size_t)(K + c0 + c) * d.ldN + cl[bb]] : zmk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + 256 * u;
+            if (e < nc * nc) {
+                const int a = e / nc, bb = e - a * nc;
+                zd s = acc[u];
+                for (int c = 0; c < cl_; c++) s = zfma(Ac[a][c], Mc[c][bb], s);
+                acc[u] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = tid + 256 * u;
+        if (e < nc * nc) {
+            const int a = e / nc, bb = e - a * nc;
+            zd* p = &D[a * gt.gmax + bb];
+            *p = zsub(*p, acc[u]);
+        }
+    }
+}
+
+void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt, int k,
+                      int K) {
+    if (gt.gmax <= 32)
+        hipLaunchKernelGGL((lr_blocks_kernel<4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+    else
+        hipLaunchKernelGGL((lr_blocks_kernel<36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+}
+
+__global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
+                                   const int32_t* gtiles, int ntiles) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (int c = 0; c < k; c++) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
+        for (int i = 0; i < ntiles; i++) tile[gtiles[i]] ^= 1;
+    }
+}
+
+void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
+                        const int32_t* gtiles, int ntiles) {
+    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(64), 0, st, P, tile, t, s, k, gtiles, ntiles);
+}
+
+// --------------------------------------------------------- PLAIN accept -----
+// V (k x N column-major, ld k) = (G - I)[t, :]  (pngfopt.hpp:420-432)
+__global__ void build_V_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, zd* V, size_t v_stride) {
+    const int f = blockIdx.y;
+    const zd* Gr = tb.Grow[f];
+    zd* Vf = V + (size_t)f * v_stride;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)k * d.N;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int n = (int)(e / k), i = (int)(e - (size_t)n * k);
+        zd val = Gr[(size_t)t[i] * d.ldN + n];
+        if (n == t[i]) val.x -= 1.0;
+        Vf[e] = val;
+    }
+}
+
+void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, zd* V,
+                    size_t v_stride) {
+    hipLaunchKernelGGL(build_V_kernel, dim3(1024, d.nf), dim3(256), 0, st, d, tb, t, k, V, v_stride);
+}
+
+// CinvU (N x k column-major, ld ldN) = C^-1[:, t] diag(s)   (pngfopt.hpp:260-274)
+__global__ void snapshot_cinvu_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
+                                      const int32_t* __restrict__ s, int k, zd* CU, size_t cu_stride) {
+    const int f = blockIdx.y, c = blockIdx.z;
+    const zd* Ci = tb.Cinv[f] + (size_t)t[c] * d.ldN;
+    zd* out = CU + (size_t)f * cu_stride + (size_t)c * d.ldN;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < d.N; n += gridDim.x * blockDim.x) {
+        zd val = Ci[n];
+        out[n] = s[c] > 0 ? val : zmk(-val.x, -val.y);
+    }
+}
+
+void launch_snapshot_cinvu(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t,
+                           const int32_t* s, int k, zd* CU, size_t cu_stride) {
+    hipLaunchKernelGGL(snapshot_cinvu_kernel, dim3((d.N + 255) / 256, d.nf, k), dim3(256), 0, st, d, tb, t, s, k,
+                       CU, cu_stride);
+}
+
+// S' = S - CinvU (Einc_t + y)  ;  R' = R - RU M  with RU = R[:,t] diag(s);  Eobj = newEobj.
+// (exact Woodbury forms of compute_R_matrix(0), pngfopt.hpp:237-250)
+// M is row-major k x ldN (M = cap^-1 V C^-1).
+__global__ void plain_small_kernel(Dims d, DevVecs v, const zd* __restrict__ acc, int acc_stride,
+                                   const zd* __restrict__ CU, size_t cu_stride, const zd* __restrict__ M,
+                                   size_t m_stride, const int32_t* __restrict__ t, const zd* __restrict__ rut,
+                                   int k) {
+    const int f = blockIdx.y;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const zd* st = acc + (size_t)f * acc_stride;
+    const zd* y = st + 2 * k * k;
+    const zd* einc = v.Einc + (size_t)f * d.ldN;
+    const zd* ru = rut + (size_t)f * d.ne * 96;
+    if (n < d.N) {
+        const zd* cu = CU + (size_t)f * cu_stride;
+        zd sa = v.S[(size_t)f * d.ldN + n];
+        for (int c = 0; c < k; c++) sa = zfms(cu[(size_t)c * d.ldN + n], zadd(einc[t[c]], y[c]), sa);
+        v.S[(size_t)f * d.ldN + n] = sa;
+        const zd* Mf = M + (size_t)f * m_stride;
+        for (int r = 0; r < d.ne; r++) {
+            zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
+            zd ra = R[n];
+            for (int c = 0; c < k; c++) ra = zfms(ru[r * 96 + c], Mf[(size_t)c * d.ldN + n], ra);
+            R[n] = ra;
+        }
+    }
+}
+
+__global__ void plain_rut_kernel(Dims d, DevVecs v, const zd* acc, int acc_stride, const int32_t* t,
+                                 const int32_t* s, int k, zd* rut) {
+    const int f = blockIdx.x, tid = threadIdx.x;
+    for (int e = tid; e < d.ne * k; e += blockDim.x) {
+        const int r = e / k, c = e - r * k;
+        zd val = v.R[((size_t)f * d.ne + r) * d.ldN + t[c]];
+        rut[(size_t)f * d.ne * 96 + r * 96 + c] = s[c] > 0 ? val : zmk(-val.x, -val.y);
+    }
+    if (tid < d.ne) v.Eobj[f * d.ne + tid] = acc[(size_t)f * acc_stride + 2 * k * k + 2 * k + tid];
+}
+
+void launch_plain_small(hipStream_t st, const Dims& d, const DevVecs& v, const zd* acc, int acc_stride,
+                        const zd* CU, size_t cu_stride, const zd* M, size_t m_stride, const int32_t* t,
+                        const int32_t* s, int k, zd* rut) {
+    // RU snapshot must precede the R update
+    hipLaunchKernelGGL(plain_rut_kernel, dim3(d.nf), dim3(64), 0, st, d, v, acc, acc_stride, t, s, k, rut);
+    hipLaunchKernelGGL(plain_small_kernel, dim3((d.N + 255) / 256, d.nf), dim3(256), 0, st, d, v, acc, acc_stride,
+                       CU, cu_stride, M, m_stride, t, rut, k);
+}
+
+// ------------------------------------------------------------------ init -----
+// Synthetic G: dyadic Green's function of a homogeneous eps_r medium, softened
+// (R^2 -> R^2 + a^2), times -i (f / f_ref) and a complex scale.
+//   G_mn = scale * g(R) [ t1 (u_m.u_n) + t2 (u_m.R)(u_n.R) / R^2 ],
+//   g = exp(-i k R) / (4 pi R),  t1 = 1 - i/(kR) - 1/(kR)^2,  t2 = -1 + 3i/(kR) + 3/(kR)^2
+// pos[m] = (x, y, orientation) in metres (orientation 0 = x, 1 = y).
+__global__ void synth_g_kernel(int N, int ldN, const double* __restrict__ pos, double kr, zd scale, double soft2,
+                               int row_major, zd* out) {
+    __shared__ double pc[32][3];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 8;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    if (ty == 0 && c0 + tx < N) { pc[tx][0] = pos[3 * (c0 + tx)]; pc[tx][1] = pos[3 * (c0 + tx) + 1]; pc[tx][2] = pos[3 * (c0 + tx) + 2]; }
+    __syncthreads();
+    const int r = r0 + ty, c = c0 + tx;
+    if (r >= N || c >= N) return;
+    double pr[3] = {pos[3 * r], pos[3 * r + 1], pos[3 * r + 2]};
+    zd v = zmul(synth_entry(pr, pc[tx], kr, soft2), scale);
+    // element (r, c); row-major puts row r contiguous, column-major puts column c contiguous
+    if (row_major) out[(size_t)r * ldN + c] = v;
+    else out[(size_t)c * ldN + r] = v;
+}
+
+void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos, double kr, zd scale, double soft,
+                    int row_major, zd* out) {
+    dim3 grid((N + 31) / 32, (N + 7) / 8);
+    hipLaunchKernelGGL(synth_g_kernel, grid, dim3(32, 8), 0, st, N, ldN, pos, kr, scale, soft * soft, row_major, out);
+}
+
+// C = I - P + P G, column-major (pngfopt.hpp:196-233)
+__global__ void build_C_kernel(int N, int ldN, const int8_t* __restrict__ P, const zd* __restrict__ G, zd* C) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+    if (row >= N) return;
+    const size_t i = (size_t)col * ldN + row;
+    C[i] = P[row] ? G[i] : zmk(row == col ? 1.0 : 0.0, 0.0);
+}
+
+void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* Gcm, zd* C) {
+    hipLaunchKernelGGL(build_C_kernel, dim3((N + 255) / 256, N), dim3(256), 0, st, N, ldN, P, Gcm, C);
+}
+
+__global__ void add_diag_kernel(int N, int ldN, zd* A, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) A[(size_t)i * ldN + i].x += v;
+}
+
+void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v) {
+    hipLaunchKernelGGL(add_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, A, v);
+}
+
+__global__ void transpose_kernel(int N, int ldN, const zd* __restrict__ in, zd* __restrict__ out) {
+    __shared__ zd tileb[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r = bx + threadIdx.x, c = by + j;
+        if (r < N && c < N) tileb[j][threadIdx.x] = in[(size_t)c * ldN + r];
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r = bx + j, c = by + threadIdx.x;
+        if (r < N && c < N) out[(size_t)r * ldN + c] = tileb[threadIdx.x][j];
+    }
+}
+
+void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out) {
+    dim3 grid((N + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, N, ldN, in, out);
+}
+
+// Gobj row 0 = sum_p G(p, :),  Einc = sum_q G(:, q)   (pngfopt.hpp:584-595)
+__global__ void port_sums_kernel(int N, int ldN, const zd* __restrict__ G, const int32_t* __restrict__ port, int np,
+                                 zd* gobj0, zd* einc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    zd a = zmk(0.0, 0.0), b = zmk(0.0, 0.0);
+    for (int i = 0; i < np; i++) {
+        const int p = port[i];
+        a = zadd(a, G[(size_t)n * ldN + p]);
+        b = zadd(b, G[(size_t)p * ldN + n]);
+    }
+    gobj0[n] = a;
+    einc[n] = b;
+}
+
+void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
+                      zd* einc) {
+    hipLaunchKernelGGL(port_sums_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, Gcm, port, np, gobj0, einc);
+}
+
+__global__ void pe_kernel(int N, const int8_t* __restrict__ P, const zd* __restrict__ einc, zd* pe) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    zd e = einc[n];
+    pe[n] = P[n] ? zmk(-e.x, -e.y) : zmk(0.0, 0.0);
+}
+
+void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe) {
+    hipLaunchKernelGGL(pe_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, P, einc, pe);
+}
+
+// out[r] = sum_n A[r][n] x[n]   (rows of length N at stride ldN; fixed-order tree)
+__global__ void dot_rows_kernel(int N, int ldN, const zd* __restrict__ A, const zd* __restrict__ x, zd* out) {
+    __shared__ zd red[256];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    zd s = zmk(0.0, 0.0);
+    for (int n = tid; n < N; n += 256) s = zfma(A[(size_t)r * ldN + n], x[n], s);
+    red[tid] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] = zadd(red[tid], red[tid + off]);
+        __syncthreads();
+    }
+    if (tid == 0) out[r] = red[0];
+}
+
+void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const zd* A, const zd* x, zd* out) {
+    hipLaunchKernelGGL(dot_rows_kernel, dim3(rows), dim3(256), 0, st, N, ldN, A, x, out);
+}
+
+
+// Current W over an arbitrary group's components (flips outside the driver's
+// group table): D[a][b] = W0[c_a][c_b] - sum_K' A_all[c_a][K'] M_all[K'][c_b].
+__global__ void lr_block_current_kernel(Dims d, DevTabs tb, GroupTab gt, int g, int K, zd* Dblk) {
+    const int f = blockIdx.y;
+    const int nc = gt.ncomp[g];
+    const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
+    const zd* W0 = tb.W0[f];
+    const zd* A = tb.Aall[f];
+    const zd* M = tb.Mall[f];
+    zd* D = Dblk + ((size_t)f * d.ngroups + g) * gt.gmax * gt.gmax;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nc * nc; e += gridDim.x * blockDim.x) {
+        const int a = e / nc, b = e - a * nc;
+        zd s = W0[(size_t)cl[a] * d.ldN + cl[b]];
+        for (int kk = 0; kk < K; kk++) s = zfms(A[(size_t)cl[a] * d.Kmax + kk], M[(size_t)kk * d.ldN + cl[b]], s);
+        D[a * gt.gmax + b] = s;
+    }
+}
+
+void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabs& tb, GroupTab gt, int g, int K, zd* Dblk) {
+    hipLaunchKernelGGL(lr_block_current_kernel, dim3(36, d.nf), dim3(256), 0, st, d, tb, gt, g, K, Dblk);
+}
+
+// D[g][a][b] = W0[comp_g[a]][comp_g[b]]
+__global__ void extract_blocks_kernel(Dims d, const zd* __restrict__ W0, GroupTab gt, zd* Dblk) {
+    const int g = blockIdx.x;
+    const int nc = gt.ncomp[g];
+    const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
+    zd* D = Dblk + (size_t)g * gt.gmax * gt.gmax;
+    for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
+        const int a = e / nc, b = e - a * nc;
+        D[a * gt.gmax + b] = W0[(size_t)cl[a] * d.ldN + cl[b]];
+    }
+}
+
+void launch_extract_blocks(hipStream_t st, const Dims& d, const zd* W0, GroupTab gt, zd* Dblk_f) {
+    hipLaunchKernelGGL(extract_blocks_kernel, dim3(d.ngroups), dim3(256), 0, st, d, W0, gt, Dblk_f);
+}
+
+}  // namespace pngf

@@ -1,0 +1,1307 @@
+// C ABI of the gfx950 DBS Woodbury path (include/pngf_gpu.h): handle, G input,
+// one-time initial state (rocSOLVER / rocBLAS), candidate scoring, accepts and the
+// DBS driver loop.  All device work of a handle is ordered on its own stream.
+#include "kernels.h"
+#include "layout.h"
+#include "../../include/pngf_gpu.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+
+using namespace pngf;
+
+namespace {
+
+typedef rocblas_double_complex rbz;
+
+struct Timed {
+    hipEvent_t a, b;
+    int which;
+    double bytes, flops;
+};
+
+// Draw-without-replacement visit order of the reference driver
+// (pngf-opt/magichat.hpp:53-107, seed != 0 branch): std::mt19937 seeded with the
+// int seed, uniform_int_distribution<> re-parameterised on every draw.
+struct Hat {
+    std::mt19937 gen;
+    std::uniform_int_distribution<> dis;
+    int total = 0, left = 0;
+    std::vector<int> idx;
+    void init(int seed, int n) {
+        gen.seed(seed);
+        dis = std::uniform_int_distribution<>(0, n - 1);
+        total = left = n;
+        idx.resize(n);
+        for (int i = 0; i < n; i++) idx[i] = i;
+    }
+    int draw() {
+        if (left <= 0) return -1;
+        dis.param(std::uniform_int_distribution<int>::param_type(0, left - 1));
+        int i = dis(gen);
+        int v = idx[i];
+        idx[i] = idx[left - 1];
+        left--;
+        return v;
+    }
+    void reset() { for (int i = 0; i < total; i++) idx[i] = i; left = total; }
+};
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// pinned host staging (int32 slots): accept change set, selection, batch ids, LU infos
+constexpr int kPinSel = 512, kPinBatch = 1024, kPinInfo = 16384, kPinInts = 65536;
+
+}  // namespace
+
+struct pngf_handle {
+    pngf_config cfg{};
+    Layout L;
+    Groups grp;
+    int ngroups_drv = 0, adhoc = -1;
+    std::map<std::vector<int>, int> group_of;  // sorted tile set -> driver group
+    std::string err;
+    hipStream_t st = nullptr;
+    rocblas_handle rb = nullptr;
+    Dims d{};
+    ObjParams op{};
+    std::vector<double> freqs;
+    std::vector<int32_t> tile;
+    std::vector<int8_t> P;
+    std::vector<int32_t> owners;
+    bool tile_set = false, inited = false;
+    std::vector<int> g_loaded;
+    std::vector<FreqConst> fc;
+    std::vector<std::vector<zd>> gobj_extra;  // [f][(ne-1)*N]
+    std::vector<double> pos;                  // [N][3] metres, for the synthetic G
+    zd synth_scale{};                         // complex normalisation of the synthetic G
+    double synth_beta = 0.0;                  // NF2FF row scale of the synthetic G
+    // device: common
+    int32_t *d_tile = nullptr, *d_owners = nullptr, *d_gtiles = nullptr, *d_gnt = nullptr, *d_gcomps = nullptr,
+            *d_gnc = nullptr, *d_batch = nullptr, *d_small = nullptr, *d_sel = nullptr, *d_info = nullptr;
+    int8_t* d_P = nullptr;
+    double* d_pos = nullptr;
+    double *d_F = nullptr, *d_Fbest = nullptr, *d_Fsel = nullptr;
+    zd* d_eobj_out = nullptr;
+    zd* d_pe = nullptr;
+    // per frequency big buffers (host arrays of device pointers)
+    std::vector<zd*> Gcm, Grow, Cinv, W0, Aall, Mall;
+    zd** d_tabs = nullptr;  // device copy: [5][nf]
+    DevTabs tabs{};
+    DevVecs vecs{};
+    // scratch
+    zd *d_partial = nullptr, *d_acc = nullptr, *d_Mc = nullptr, *d_Ar = nullptr, *d_Bt = nullptr, *d_V = nullptr,
+       *d_VC = nullptr, *d_CU = nullptr, *d_M = nullptr, *d_rut = nullptr;
+    zd* d_lu = nullptr;  // per-block capacitance scratch when gmax > kLdsGmax
+    int acc_stride = 0;
+    size_t small_stride = 0, bt_stride = 0, v_stride = 0;
+    int nsplit_items = 2048;
+    size_t partial_elems = 0;
+    int K = 0, Kmax = 0, flushes = 0;
+    // pinned host staging
+    int32_t* h_pin_i = nullptr;   // batch ids / small ints
+    double* h_pin_d = nullptr;
+    // profiling
+    bool prof = false;
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> pool;
+    pngf_kstat kst[PNGF_K_COUNT]{};
+    double acc_seconds = 0.0;
+    // checkpoint (device copies of the mutable state)
+    bool ckpt = false;
+    std::vector<zd*> ck_big;
+    zd *ck_vec = nullptr, *ck_D = nullptr;
+    int8_t* ck_P = nullptr;
+    std::vector<int32_t> ck_tile;
+    std::vector<int8_t> ck_Ph;
+    // memory accounting
+    size_t mem_used = 0;
+    std::vector<std::pair<void*, size_t>> allocs;
+};
+
+namespace {
+
+int fail(pngf_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HC(h, x)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(h, PNGF_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define RB(h, x)                                                                                  \
+    do {                                                                                          \
+        rocblas_status s_ = (x);                                                                  \
+        if (s_ != rocblas_status_success)                                                         \
+            return fail(h, PNGF_ERR_HIP, std::string(#x) + ": rocblas status " + std::to_string((int)s_)); \
+    } while (0)
+#define CK(x)                      \
+    do {                           \
+        int r_ = (x);              \
+        if (r_ != PNGF_OK) return r_; \
+    } while (0)
+
+int dalloc(pngf_handle* h, void** p, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (h->cfg.mem_limit > 0 && h->mem_used + bytes > (size_t)h->cfg.mem_limit)
+        return fail(h, PNGF_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes exceeds mem_limit");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(h, PNGF_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    h->mem_used += bytes;
+    h->allocs.push_back({*p, bytes});
+    return PNGF_OK;
+}
+
+template <class T>
+int dalloc_t(pngf_handle* h, T** p, size_t n) { return dalloc(h, (void**)p, n * sizeof(T)); }
+
+void dfree(pngf_handle* h, void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < h->allocs.size(); i++)
+        if (h->allocs[i].first == p) {
+            h->mem_used -= h->allocs[i].second;
+            h->allocs.erase(h->allocs.begin() + i);
+            break;
+        }
+    (void)hipFree(p);
+}
+
+// ------------------------------------------------------------- profiling ---
+void prof_begin(pngf_handle* h, Timed& t) {
+    if (!h->prof) return;
+    if (h->pool.size() < 2) {
+        for (int i = 0; i < 16; i++) { hipEvent_t e; if (hipEventCreate(&e) == hipSuccess) h->pool.push_back(e); }
+    }
+    t.a = h->pool.back(); h->pool.pop_back();
+    t.b = h->pool.back(); h->pool.pop_back();
+    (void)hipEventRecord(t.a, h->st);
+}
+void prof_end(pngf_handle* h, Timed& t, int which, double bytes, double flops) {
+    if (!h->prof) return;
+    (void)hipEventRecord(t.b, h->st);
+    t.which = which; t.bytes = bytes; t.flops = flops;
+    h->pending.push_back(t);
+}
+// Resolve recorded event pairs (call after the stream has been synchronised).
+void prof_collect(pngf_handle* h) {
+    for (auto& t : h->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            auto& k = h->kst[t.which];
+            k.launches++; k.total_ms += ms; k.bytes += t.bytes; k.flops += t.flops;
+        }
+        h->pool.push_back(t.a); h->pool.push_back(t.b);
+    }
+    h->pending.clear();
+}
+
+// ----------------------------------------------------------------- groups ---
+void build_groups(pngf_handle* h) {
+    Layout& L = h->L;
+    Groups& G = h->grp;
+    G = Groups();
+    G.gmax = L.per_tile * std::max(1, h->cfg.max_flips);
+    h->group_of.clear();
+    if (h->cfg.driver == PNGF_DRIVER_RASTER) {
+        std::vector<int> c;
+        for (int t = 0; t < L.Tx * L.Ty; t++) {
+            c.clear(); L.tile_components(t, c);
+            bool touches = false;
+            for (int i : c) for (int p : L.port) if (i == p) touches = true;
+            int tt[1] = {t};
+            G.add(L, tt, touches ? 0 : 1, t);
+            if (!touches) h->group_of[{t}] = G.n - 1;
+        }
+    } else {
+        int nhat = ((L.Tx - 1) / 2 + 1) * ((L.Ty - 1) / 2 + 1) - 1;
+        for (int r = 0; r < nhat; r++) {
+            int tt[4];
+            int nt = ref_flip_set(r, L.Tx, L.Ty, tt);
+            G.add(L, tt, nt, nt ? tt[0] : -1);
+            if (nt) {
+                std::vector<int> key(tt, tt + nt);
+                std::sort(key.begin(), key.end());
+                h->group_of[key] = G.n - 1;
+            }
+        }
+    }
+    h->ngroups_drv = G.n;
+    int none[1] = {0};
+    G.add(L, none, 0, -1);  // ad-hoc slot
+    h->adhoc = G.n - 1;
+}
+
+GroupTab gtab(pngf_handle* h) {
+    GroupTab g;
+    g.tiles = h->d_gtiles; g.ntiles = h->d_gnt; g.comps = h->d_gcomps; g.ncomp = h->d_gnc; g.gmax = h->grp.gmax;
+    return g;
+}
+
+// Host change set of group g against the host mirrors (same rule as the device).
+int host_change_set(pngf_handle* h, int g, int* t, int* s, int* lp) {
+    const Groups& G = h->grp;
+    const int nt = G.ntiles[g];
+    const int32_t* gtl = &G.tiles[4 * g];
+    int k = 0;
+    for (int a = 0; a < G.ncomp[g]; a++) {
+        const int i = G.comps[(size_t)g * G.gmax + a];
+        int nv = 0;
+        for (int hh = 0; hh < 2; hh++) {
+            int o = h->owners[2 * i + hh];
+            if (o < 0) continue;
+            int v = h->tile[o];
+            for (int q = 0; q < nt; q++) v ^= (gtl[q] == o) ? 1 : 0;
+            nv |= v;
+        }
+        if (nv != h->P[i]) { t[k] = i; s[k] = nv - h->P[i]; lp[k] = a; k++; }
+    }
+    return k;
+}
+
+// Writes group `g` (the ad-hoc slot) for an arbitrary tile set and uploads it.
+int set_adhoc(pngf_handle* h, const int32_t* tiles, int nt) {
+    Groups& G = h->grp;
+    const int g = h->adhoc;
+    std::vector<int> c;
+    for (int i = 0; i < nt; i++) h->L.tile_components(tiles[i], c);
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    if ((int)c.size() > G.gmax) return fail(h, PNGF_ERR_ARG, "flip touches more components than max_flips allows");
+    for (int i = 0; i < 4; i++) G.tiles[4 * g + i] = i < nt ? tiles[i] : -1;
+    G.ntiles[g] = nt;
+    G.ncomp[g] = (int)c.size();
+    G.prim[g] = tiles[0];
+    for (int i = 0; i < G.gmax; i++) G.comps[(size_t)g * G.gmax + i] = i < (int)c.size() ? c[i] : -1;
+    HC(h, hipMemcpyAsync(h->d_gtiles + 4 * g, &G.tiles[4 * g], 4 * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    HC(h, hipMemcpyAsync(h->d_gnt + g, &G.ntiles[g], sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    HC(h, hipMemcpyAsync(h->d_gnc + g, &G.ncomp[g], sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    HC(h, hipMemcpyAsync(h->d_gcomps + (size_t)g * G.gmax, &G.comps[(size_t)g * G.gmax], G.gmax * sizeof(int32_t),
+                         hipMemcpyHostToDevice, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->inited)
+        launch_lr_block_current(h->st, h->d, h->tabs, gtab(h), g, h->K, h->vecs.Dblk);
+    return PNGF_OK;
+}
+
+// Maps a tile set to a group (driver table, or the ad-hoc slot).
+int group_for(pngf_handle* h, const int32_t* tiles, int nt, int* g) {
+    if (!tiles || nt < 1 || nt > h->cfg.max_flips) return fail(h, PNGF_ERR_ARG, "bad flip tile list");
+    std::vector<int> key(tiles, tiles + nt);
+    for (int t : key)
+        if (t < 0 || t >= h->L.Tx * h->L.Ty) return fail(h, PNGF_ERR_ARG, "tile index out of range");
+    std::sort(key.begin(), key.end());
+    for (size_t i = 1; i < key.size(); i++)
+        if (key[i] == key[i - 1]) return fail(h, PNGF_ERR_ARG, "repeated tile in flip");
+    auto it = h->group_of.find(key);
+    if (it != h->group_of.end()) { *g = it->second; return PNGF_OK; }
+    CK(set_adhoc(h, tiles, nt));
+    *g = h->adhoc;
+    return PNGF_OK;
+}
+
+// ------------------------------------------------------------ synthetic G ---
+const double kFref = 30.0e9;
+const double kEpsR = 3.5;
+const double kSoft = 1.0;  // softening radius in cells
+
+void synth_positions(const Layout& L, double dx, double dy, std::vector<double>& pos) {
+    pos.resize((size_t)L.N * 3);
+    for (int i = 0; i < L.N; i++) {
+        double x, y; int o;
+        L.position(i, x, y, o);
+        pos[3 * i] = x * dx; pos[3 * i + 1] = y * dy; pos[3 * i + 2] = o;
+    }
+}
+
+double kr_of(double hz) { return 2.0 * kPi * hz * std::sqrt(kEpsR) / kC0; }
+
+// Normalisation: at 30 GHz the port block of G sums to Vt = -(50 + 30i) / zfac,
+// i.e. the port alone sees Zin = 50 + 30i ohm; every frequency f carries an
+// extra factor f / 30 GHz.  The NF2FF rows (num_eobj = 3) are beta (f/30 GHz)
+// times the x and y orientation of each component, with beta chosen so that the
+// port alone has directivity 1.5 at 30 GHz.
+void synth_norm(const Layout& L, const std::vector<double>& pos, double dx, double dy, double dz, zd& scale,
+                double& beta) {
+    const double zfac = dx * 0.5 / (2.0 * dy * dz);
+    const double kr = kr_of(kFref), soft2 = (kSoft * dx) * (kSoft * dx);
+    zd sum = zmk(0.0, 0.0);
+    for (int p : L.port)
+        for (int q : L.port) sum = zadd(sum, synth_entry(&pos[3 * p], &pos[3 * q], kr, soft2));
+    const zd vt = zmk(-50.0 / zfac, -30.0 / zfac);
+    scale = zdiv(vt, sum);
+    const double pfac = dx * 0.5 * (2.0 * dy * dz) * 0.5;
+    const double k0 = 2.0 * kPi * kFref / kC0;
+    const double np = (double)L.port.size();
+    const double c = 4.0 * kPi * (k0 * k0 / (32 * kPi * kPi * kEta0));
+    beta = std::sqrt(1.5 * (-vt.x * pfac) / (c * np * np));
+}
+
+// ------------------------------------------------------------------ init ---
+int check_ready(pngf_handle* h) {
+    if (!h->inited) return fail(h, PNGF_ERR_STATE, "pngf_init_state has not completed");
+    return PNGF_OK;
+}
+
+int upload_tabs(pngf_handle* h) {
+    const int nf = h->d.nf;
+    std::vector<zd*> all(5 * nf, nullptr);
+    for (int f = 0; f < nf; f++) {
+        all[0 * nf + f] = h->W0[f]; all[1 * nf + f] = h->Aall[f]; all[2 * nf + f] = h->Mall[f];
+        all[3 * nf + f] = h->Grow[f]; all[4 * nf + f] = h->Cinv[f];
+    }
+    HC(h, hipMemcpy(h->d_tabs, all.data(), all.size() * sizeof(zd*), hipMemcpyHostToDevice));
+    h->tabs.W0 = h->d_tabs; h->tabs.Aall = h->d_tabs + nf; h->tabs.Mall = h->d_tabs + 2 * nf;
+    h->tabs.Grow = h->d_tabs + 3 * nf; h->tabs.Cinv = h->d_tabs + 4 * nf;
+    return PNGF_OK;
+}
+
+int alloc_gcm(pngf_handle* h, int f) {
+    if (!h->Gcm[f]) CK(dalloc_t(h, &h->Gcm[f], (size_t)h->d.ldN * h->d.N));
+    return PNGF_OK;
+}
+
+int init_freq(pngf_handle* h, int f) {
+    const int N = h->d.N, ld = h->d.ldN, ne = h->d.ne;
+    hipStream_t st = h->st;
+    const rbz one = {1.0, 0.0}, zero = {0.0, 0.0};
+    zd* G = h->Gcm[f];
+    zd* C = nullptr;
+    CK(dalloc_t(h, &C, (size_t)ld * N));
+    launch_build_C(st, N, ld, h->d_P, G, C);
+    zd* gobj = h->vecs.Gobj + (size_t)f * ne * ld;
+    zd* einc = h->vecs.Einc + (size_t)f * ld;
+    launch_port_sums(st, N, ld, G, h->d_small + 400, (int)h->L.port.size(), gobj, einc);
+    for (int r = 1; r < ne; r++)
+        HC(h, hipMemcpyAsync(gobj + (size_t)r * ld, h->gobj_extra[f].data() + (size_t)(r - 1) * N, N * sizeof(zd),
+                             hipMemcpyHostToDevice, st));
+    // C^-1 in place (linalg.hpp:49-84: zgetrf + zgetri)
+    int* ipiv = nullptr;
+    CK(dalloc_t(h, &ipiv, N));
+    RB(h, rocsolver_zgetrf(h->rb, N, N, (rbz*)C, ld, ipiv, h->d_info));
+    int info = 0;
+    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    if (info != 0) { dfree(h, ipiv); dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetrf info " + std::to_string(info) + ")"); }
+    RB(h, rocsolver_zgetri(h->rb, N, (rbz*)C, ld, ipiv, h->d_info));
+    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    dfree(h, ipiv);
+    if (info != 0) { dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetri info " + std::to_string(info) + ")"); }
+    // PEinc, R = Gobj C^-1 (row-major ne x ld: R^T = C^-T Gobj^T), Eobj = R PEinc
+    launch_pe(st, N, h->d_P, einc, h->d_pe);
+    zd* R = h->vecs.R + (size_t)f * ne * ld;
+    RB(h, rocblas_zgemm(h->rb, rocblas_operation_transpose, rocblas_operation_none, N, ne, N, &one, (rbz*)C, ld,
+                        (rbz*)gobj, ld, &zero, (rbz*)R, ld));
+    launch_dot_rows(st, N, ld, ne, R, h->d_pe, h->vecs.Eobj + (size_t)f * ne);
+    if (h->cfg.method == PNGF_METHOD_PLAIN) {
+        RB(h, rocblas_zgemv(h->rb, rocblas_operation_none, N, N, &one, (rbz*)C, ld, (rbz*)h->d_pe, 1, &zero,
+                            (rbz*)(h->vecs.S + (size_t)f * ld), 1));
+        zd* Gr = nullptr;
+        CK(dalloc_t(h, &Gr, (size_t)ld * N));
+        launch_transpose(st, N, ld, G, Gr);
+        HC(h, hipStreamSynchronize(st));
+        dfree(h, G);
+        h->Gcm[f] = nullptr;
+        h->Grow[f] = Gr;
+        h->Cinv[f] = C;
+    } else {
+        // W = (G - I) C^-1, stored row-major: W^T = C^-T (G - I)^T
+        launch_add_diag(st, N, ld, G, -1.0);
+        zd* W = nullptr;
+        CK(dalloc_t(h, &W, (size_t)ld * N));
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, N, N, N, &one, (rbz*)C,
+                            ld, (rbz*)G, ld, &zero, (rbz*)W, ld));
+        RB(h, rocblas_zgemv(h->rb, rocblas_operation_transpose, N, N, &one, (rbz*)W, ld, (rbz*)h->d_pe, 1, &zero,
+                            (rbz*)(h->vecs.q + (size_t)f * ld), 1));
+        launch_extract_blocks(st, h->d, W, gtab(h), h->vecs.Dblk + (size_t)f * h->d.ngroups * h->d.gmax * h->d.gmax);
+        HC(h, hipStreamSynchronize(st));
+        dfree(h, G);
+        dfree(h, C);
+        h->Gcm[f] = nullptr;
+        h->W0[f] = W;
+    }
+    HC(h, hipGetLastError());
+    // host constants: Vinc, LN1inc, LN2inc (pngfopt.hpp:599-617), k0
+    std::vector<zd> e(h->L.port.size());
+    for (size_t i = 0; i < h->L.port.size(); i++)
+        HC(h, hipMemcpyAsync(&e[i], einc + h->L.port[i], sizeof(zd), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    FreqConst fcv;
+    fcv.vinc = zmk(0.0, 0.0); fcv.ln1 = zmk(0.0, 0.0); fcv.ln2 = zmk(0.0, 0.0);
+    for (size_t i = 0; i < e.size(); i++) fcv.vinc = zadd(fcv.vinc, e[i]);
+    if (ne > 2)
+        for (int p : h->L.port) {
+            fcv.ln1 = zadd(fcv.ln1, h->gobj_extra[f][p]);
+            fcv.ln2 = zadd(fcv.ln2, h->gobj_extra[f][(size_t)N + p]);
+        }
+    fcv.k0 = 2.0 * kPi * h->freqs[f] / kC0;
+    h->fc[f] = fcv;
+    return PNGF_OK;
+}
+
+// ------------------------------------------------------------------ eval ---
+int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
+    hipStream_t st = h->st;
+    for (int i = 0; i < nb; i++) h->h_pin_i[kPinBatch + i] = groups[i];
+    HC(h, hipMemcpyAsync(h->d_batch, h->h_pin_i + kPinBatch, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    Timed tm{};
+    prof_begin(h, tm);
+    const Dims& d = h->d;
+    double bytes = 0, flops = 0;
+    const double kk = (double)d.gmax;
+    if (h->cfg.method == PNGF_METHOD_LOWRANK) {
+        launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb, h->d_eobj_out, h->d_info,
+                       h->d_lu);
+        bytes = (double)nb * d.nf * 16.0 * (kk * kk + kk * (d.ne + 2));
+        flops = (double)nb * d.nf * ((8.0 / 3.0) * kk * kk * kk + 16.0 * kk * kk);
+    } else {
+        const int items = nb * d.nf;
+        int nsplit = std::max(1, std::min(64, h->nsplit_items / std::max(1, items)));
+        nsplit = std::min(nsplit, std::max(1, d.N / 64));
+        launch_plain_eval(st, d, h->tabs, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb, nsplit,
+                          h->d_partial, h->d_eobj_out, h->d_info, nullptr, 0, h->d_lu);
+        bytes = (double)items * 16.0 * d.N * (2 * kk + 1 + d.ne);
+        flops = (double)items * (8.0 * d.N * (kk * kk + 2 * kk + d.ne) + (8.0 / 3.0) * kk * kk * kk);
+    }
+    prof_end(h, tm, PNGF_K_EVAL, bytes, flops);
+    launch_objective(st, h->op, h->vecs.fc, h->d_eobj_out, nb, h->d_F, h->d_Fbest, h->d_sel, h->d_Fsel);
+    HC(h, hipGetLastError());
+    return PNGF_OK;
+}
+
+// ---------------------------------------------------------------- accept ---
+int flush_factors(pngf_handle* h) {
+    if (h->K == 0) return PNGF_OK;
+    const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
+    const Dims& d = h->d;
+    Timed tm{};
+    prof_begin(h, tm);
+    for (int f = 0; f < d.nf; f++)
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
+                            (rbz*)h->Mall[f], d.ldN, (rbz*)h->Aall[f], d.Kmax, &one, (rbz*)h->W0[f], d.ldN));
+    prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (32.0 * d.N * (double)d.N + 32.0 * d.N * (double)h->K),
+             (double)d.nf * 8.0 * h->K * (double)d.N * d.N);
+    h->K = 0;
+    h->flushes++;
+    return PNGF_OK;
+}
+
+int accept_group(pngf_handle* h, int g) {
+    hipStream_t st = h->st;
+    const Dims& d = h->d;
+    int* t = h->h_pin_i;
+    int* s = t + 96;
+    int* lp = s + 96;
+    int k = host_change_set(h, g, t, s, lp);
+    int* gt = lp + 96;
+    const int nt = h->grp.ntiles[g];
+    for (int i = 0; i < 4; i++) gt[i] = h->grp.tiles[4 * g + i];
+    for (int i = 0; i < (int)h->L.port.size(); i++) gt[4 + i] = h->L.port[i];
+    HC(h, hipMemcpyAsync(h->d_small, t, 300 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    const int32_t* dt = h->d_small;
+    const int32_t* ds = h->d_small + 96;
+    const int32_t* dlp = h->d_small + 192;
+    const int32_t* dgt = h->d_small + 288;
+    auto t0 = std::chrono::high_resolution_clock::now();
+    if (k > 0 && h->cfg.method == PNGF_METHOD_LOWRANK) {
+        if (h->K + k > h->Kmax) CK(flush_factors(h));
+        const int K = h->K;
+        Timed tm{};
+        prof_begin(h, tm);
+        launch_lr_accept_prep(st, d, h->vecs, g, dt, ds, dlp, k, h->d_acc, h->acc_stride, h->d_info, h->d_lu);
+        launch_lr_gather(st, d, h->tabs, dt, k, K, h->d_Mc, h->d_Ar, (int)h->small_stride);
+        prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
+        const double N = d.N, kd = k, Kd = K;
+        Timed t1{};
+        prof_begin(h, t1);
+        launch_lr_colblk(st, d, h->tabs, dt, k, K, h->d_Mc, (int)h->small_stride);
+        prof_end(h, t1, PNGF_K_COLBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+        Timed t2{};
+        prof_begin(h, t2);
+        launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride);
+        prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+        Timed t3{};
+        prof_begin(h, t3);
+        launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds, h->tabs.Mall,
+                            nullptr, 0, K, d.ldN);
+        launch_lr_small(st, d, h->tabs, h->vecs, h->d_acc, h->acc_stride, k, K);
+        launch_lr_blocks(st, d, h->tabs, h->vecs, gtab(h), k, K);
+        launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
+        prof_end(h, t3, PNGF_K_ACC_SMALL, 0, 0);
+        h->K += k;
+    } else if (k > 0) {
+        // PLAIN: re-score the flip keeping its factors, then pngfopt.hpp:256-367
+        const rbz one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+        h->h_pin_i[kPinBatch] = g;
+        HC(h, hipMemcpyAsync(h->d_batch, h->h_pin_i + kPinBatch, sizeof(int32_t), hipMemcpyHostToDevice, st));
+        const int nsplit = std::max(1, std::min(64, h->nsplit_items / d.nf));
+        Timed tm{};
+        prof_begin(h, tm);
+        launch_plain_eval(st, d, h->tabs, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, 1, nsplit,
+                          h->d_partial, h->d_eobj_out, h->d_info, h->d_acc, h->acc_stride, h->d_lu);
+        launch_build_V(st, d, h->tabs, dt, k, h->d_V, h->v_stride);
+        prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
+        const double N = d.N, kd = k;
+        Timed t1{};
+        prof_begin(h, t1);
+        for (int f = 0; f < d.nf; f++)
+            RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, k, d.N, d.N, &one,
+                                (rbz*)(h->d_V + f * h->v_stride), k, (rbz*)h->Cinv[f], d.ldN, &zero,
+                                (rbz*)(h->d_VC + f * h->v_stride), k));
+        prof_end(h, t1, PNGF_K_VCINV, d.nf * 16.0 * (N * N + 2 * N * kd), d.nf * 8.0 * kd * N * N);
+        Timed t2{};
+        prof_begin(h, t2);
+        launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_VC, h->v_stride, k, k, nullptr, nullptr, h->d_M,
+                            h->v_stride, 0, d.ldN);
+        launch_snapshot_cinvu(st, d, h->tabs, dt, ds, k, h->d_CU, h->v_stride);
+        prof_end(h, t2, PNGF_K_ACC_SMALL, 0, 0);
+        Timed t3{};
+        prof_begin(h, t3);
+        for (int f = 0; f < d.nf; f++)
+            RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_transpose, d.N, d.N, k, &mone,
+                                (rbz*)(h->d_CU + f * h->v_stride), d.ldN, (rbz*)(h->d_M + f * h->v_stride), d.ldN,
+                                &one, (rbz*)h->Cinv[f], d.ldN));
+        prof_end(h, t3, PNGF_K_RANKK, d.nf * (32.0 * N * N + 32.0 * N * kd), d.nf * 8.0 * kd * N * N);
+        Timed t4{};
+        prof_begin(h, t4);
+        launch_plain_small(st, d, h->vecs, h->d_acc, h->acc_stride, h->d_CU, h->v_stride, h->d_M, h->v_stride, dt, ds,
+                           k, h->d_rut);
+        launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
+        prof_end(h, t4, PNGF_K_ACC_SMALL, 0, 0);
+    } else {
+        launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, nt);
+    }
+    HC(h, hipGetLastError());
+    for (int c = 0; c < k; c++) h->P[t[c]] = (int8_t)(h->P[t[c]] + s[c]);
+    for (int i = 0; i < nt; i++) h->tile[h->grp.tiles[4 * g + i]] ^= 1;
+    if (h->prof) {
+        HC(h, hipStreamSynchronize(st));
+        h->acc_seconds += std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    }
+    return PNGF_OK;
+}
+
+int sync_sel(pngf_handle* h, int nb, int* sel, double* F_sel, bool check_info) {
+    hipStream_t st = h->st;
+    HC(h, hipMemcpyAsync(h->h_pin_i + kPinSel, h->d_sel, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HC(h, hipMemcpyAsync(h->h_pin_d, h->d_Fsel, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (check_info) HC(h, hipMemcpyAsync(h->h_pin_i + kPinInfo, h->d_info, sizeof(int32_t) * nb * h->d.nf, hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    prof_collect(h);
+    *sel = h->h_pin_i[kPinSel];
+    *F_sel = h->h_pin_d[0];
+    if (check_info)
+        for (int i = 0; i < nb * h->d.nf; i++)
+            if (h->h_pin_i[kPinInfo + i] != 0) return fail(h, PNGF_ERR_SINGULAR, "capacitance matrix is singular");
+    return PNGF_OK;
+}
+
+double host_objective(pngf_handle* h, const zd* eobj) { return objective_of(h->op, h->fc.data(), eobj); }
+
+int download_eobj(pngf_handle* h, std::vector<zd>& e) {
+    e.resize((size_t)h->d.nf * h->d.ne);
+    HC(h, hipMemcpyAsync(e.data(), h->vecs.Eobj, e.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    return PNGF_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int pngf_layout_size(int32_t layout, int32_t tx, int32_t ty, int32_t cx, int32_t cy) {
+    if ((layout != 0 && layout != 1) || tx < 1 || ty < 1 || cx < 1 || cy < 1) return PNGF_ERR_ARG;
+    Layout L;
+    L.setup(layout, tx, ty, cx, cy);
+    return L.N;
+}
+
+const char* pngf_last_error(const pngf_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int pngf_create(const pngf_config* cfg, pngf_handle** out) {
+    if (!cfg || !out) return PNGF_ERR_ARG;
+    *out = nullptr;
+    const pngf_config& c = *cfg;
+    if ((c.layout != 0 && c.layout != 1) || c.tiles_x < 1 || c.tiles_y < 1 || c.cells_x < 1 || c.cells_y < 2 ||
+        c.num_eobj < 1 || c.num_eobj > 3 || c.num_freq < 1 || c.max_flips < 1 || c.max_flips > 4 ||
+        (c.method != 0 && c.method != 1) || (c.objective != 0 && c.objective != 1) || (c.driver != 0 && c.driver != 1))
+        return PNGF_ERR_ARG;
+    if (c.objective == PNGF_OBJ_REF && c.num_eobj != 3) return PNGF_ERR_ARG;
+    if (c.driver == PNGF_DRIVER_REF && (c.max_flips < 4 || c.tiles_x % 2 == 0 || c.tiles_y % 2 == 0)) return PNGF_ERR_ARG;
+    pngf_handle* h = new pngf_handle();
+    h->cfg = c;
+    if (h->cfg.z0 == 0.0) h->cfg.z0 = 50.0;
+    if (h->cfg.dx == 0.0) { h->cfg.dx = 0.01 / 60.0; h->cfg.dy = 0.01 / 60.0; h->cfg.dz = 0.01 / 180.0; }
+    h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
+    if (h->L.per_tile * c.max_flips > 96) { delete h; return PNGF_ERR_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) {
+        (void)hipGetLastError();
+        h->err = "no HIP device";
+        *out = h;
+        return PNGF_ERR_HIP;
+    }
+    *out = h;
+    HC(h, hipSetDevice(c.device));
+    HC(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    RB(h, rocblas_create_handle(&h->rb));
+    RB(h, rocblas_set_stream(h->rb, h->st));
+    RB(h, rocblas_set_pointer_mode(h->rb, rocblas_pointer_mode_host));
+    const int N = h->L.N, nf = c.num_freq, ne = c.num_eobj;
+    h->d.N = N;
+    h->d.ldN = (int)round_up(N, 8);
+    h->d.nf = nf;
+    h->d.ne = ne;
+    h->freqs.assign(nf, kFref);
+    h->g_loaded.assign(nf, 0);
+    h->fc.assign(nf, FreqConst{});
+    h->gobj_extra.assign(nf, std::vector<zd>());
+    h->Gcm.assign(nf, nullptr); h->Grow.assign(nf, nullptr); h->Cinv.assign(nf, nullptr);
+    h->W0.assign(nf, nullptr); h->Aall.assign(nf, nullptr); h->Mall.assign(nf, nullptr);
+    h->tile.assign(h->L.Tx * h->L.Ty, 0);
+    h->P.assign(N, 0);
+    h->owners.resize(2 * (size_t)N);
+    for (int i = 0; i < N; i++) h->L.owners(i, h->owners[2 * i], h->owners[2 * i + 1]);
+    h->op.mode = c.objective; h->op.nf = nf; h->op.ne = ne;
+    h->op.zfac = h->cfg.dx * 0.5 / (2.0 * h->cfg.dy * h->cfg.dz);
+    h->op.pfac = h->cfg.dx * 0.5 * (2.0 * h->cfg.dy * h->cfg.dz) * 0.5;
+    h->op.z0 = h->cfg.z0;
+    synth_positions(h->L, h->cfg.dx, h->cfg.dy, h->pos);
+    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->synth_scale, h->synth_beta);
+    // small device state
+    const size_t ld = h->d.ldN;
+    CK(dalloc_t(h, &h->d_tile, h->tile.size()));
+    CK(dalloc_t(h, &h->d_P, N));
+    CK(dalloc_t(h, &h->d_owners, 2 * (size_t)N));
+    CK(dalloc_t(h, &h->d_small, 512));
+    CK(dalloc_t(h, &h->d_sel, 4));
+    CK(dalloc_t(h, &h->d_info, 64));
+    CK(dalloc_t(h, &h->d_pos, 3 * (size_t)N));
+
+    CK(dalloc_t(h, &h->d_Fbest, 4));
+    CK(dalloc_t(h, &h->d_Fsel, 4));
+    CK(dalloc_t(h, &h->d_pe, ld));
+    CK(dalloc_t(h, &h->d_tabs, 5 * (size_t)nf));
+    CK(dalloc_t(h, &h->vecs.q, nf * ld));
+    CK(dalloc_t(h, &h->vecs.S, nf * ld));
+    CK(dalloc_t(h, &h->vecs.R, nf * ne * ld));
+    CK(dalloc_t(h, &h->vecs.Gobj, nf * ne * ld));
+    CK(dalloc_t(h, &h->vecs.Einc, nf * ld));
+    CK(dalloc_t(h, &h->vecs.Eobj, (size_t)nf * ne));
+    CK(dalloc_t(h, &h->vecs.fc, nf));
+    HC(h, hipMemset(h->vecs.Gobj, 0, nf * ne * ld * sizeof(zd)));
+    HC(h, hipMemcpy(h->d_owners, h->owners.data(), h->owners.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HC(h, hipMemcpy(h->d_pos, h->pos.data(), h->pos.size() * sizeof(double), hipMemcpyHostToDevice));
+    HC(h, hipHostMalloc((void**)&h->h_pin_i, kPinInts * sizeof(int32_t), hipHostMallocDefault));
+    HC(h, hipHostMalloc((void**)&h->h_pin_d, 64 * sizeof(double), hipHostMallocDefault));
+    return PNGF_OK;
+}
+
+int pngf_destroy(pngf_handle* h) {
+    if (!h) return PNGF_ERR_ARG;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (auto& a : h->allocs) (void)hipFree(a.first);
+    h->allocs.clear();
+    for (auto e : h->pool) (void)hipEventDestroy(e);
+    for (auto& t : h->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    if (h->h_pin_i) (void)hipHostFree(h->h_pin_i);
+    if (h->h_pin_d) (void)hipHostFree(h->h_pin_d);
+    if (h->rb) rocblas_destroy_handle(h->rb);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+    return PNGF_OK;
+}
+
+int pngf_set_frequency(pngf_handle* h, int32_t f, double hz) {
+    if (!h || f < 0 || f >= h->d.nf || !(hz > 0)) return fail(h, PNGF_ERR_ARG, "bad frequency slot or value");
+    h->freqs[f] = hz;
+    return PNGF_OK;
+}
+
+int pngf_set_g(pngf_handle* h, int32_t f, const pngf_c128* file_buf) {
+    if (!h || !file_buf || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad G slot or buffer");
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "G cannot change after pngf_init_state");
+    const int N = h->d.N, ne = h->d.ne;
+    const size_t M = (size_t)N + ne - 1;
+    CK(alloc_gcm(h, f));
+    HC(h, hipMemcpy2D(h->Gcm[f], h->d.ldN * sizeof(zd), file_buf, M * sizeof(zd), N * sizeof(zd), N,
+                      hipMemcpyHostToDevice));
+    h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
+    const zd* fb = (const zd*)file_buf;
+    for (int r = 0; r < ne - 1; r++)
+        for (int n = 0; n < N; n++) h->gobj_extra[f][(size_t)r * N + n] = fb[(N + r) + M * n];
+    h->g_loaded[f] = 1;
+    return PNGF_OK;
+}
+
+int pngf_load_g(pngf_handle* h, int32_t f, const char* path) {
+    if (!h || !path || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad G slot or path");
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "G cannot change after pngf_init_state");
+    const int N = h->d.N, ne = h->d.ne;
+    const size_t M = (size_t)N + ne - 1;
+    struct stat sb;
+    if (stat(path, &sb) != 0) return fail(h, PNGF_ERR_IO, std::string("G file not found: ") + path);
+    if ((size_t)sb.st_size != M * M * sizeof(zd))
+        return fail(h, PNGF_ERR_SIZE, std::string("G file ") + path + " has " + std::to_string(sb.st_size) +
+                                          " bytes, expected " + std::to_string(M * M * sizeof(zd)));
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return fail(h, PNGF_ERR_IO, std::string("cannot open ") + path);
+    CK(alloc_gcm(h, f));
+    h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
+    const size_t cols = std::max<size_t>(1, (64u << 20) / (M * sizeof(zd)));
+    zd* stage = nullptr;
+    if (hipHostMalloc((void**)&stage, cols * M * sizeof(zd), hipHostMallocDefault) != hipSuccess) {
+        fclose(fp);
+        return fail(h, PNGF_ERR_NOMEM, "pinned staging buffer");
+    }
+    int rc = PNGF_OK;
+    for (size_t c0 = 0; c0 < M && rc == PNGF_OK; c0 += cols) {
+        const size_t nc = std::min(cols, M - c0);
+        if (fread(stage, sizeof(zd), nc * M, fp) != nc * M) { rc = fail(h, PNGF_ERR_IO, "short read of G file"); break; }
+        for (size_t c = 0; c < nc; c++) {
+            const size_t col = c0 + c;
+            if (col >= (size_t)N) continue;
+            for (int r = 0; r < ne - 1; r++) h->gobj_extra[f][(size_t)r * N + col] = stage[c * M + N + r];
+        }
+        const size_t ncol = std::min(nc, (size_t)N > c0 ? (size_t)N - c0 : 0);
+        if (ncol) {
+            hipError_t e = hipMemcpy2D(h->Gcm[f] + c0 * h->d.ldN, h->d.ldN * sizeof(zd), stage, M * sizeof(zd),
+                                       N * sizeof(zd), ncol, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = fail(h, PNGF_ERR_HIP, std::string("hipMemcpy2D: ") + hipGetErrorString(e));
+        }
+    }
+    fclose(fp);
+    (void)hipHostFree(stage);
+    if (rc == PNGF_OK) h->g_loaded[f] = 1;
+    return rc;
+}
+
+int pngf_synth_g(pngf_handle* h, int32_t f) {
+    if (!h || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad G slot");
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "G cannot change after pngf_init_state");
+    const int N = h->d.N, ne = h->d.ne;
+    CK(alloc_gcm(h, f));
+    const double hz = h->freqs[f];
+    const zd scale = zscale(h->synth_scale, hz / kFref);
+    launch_synth_g(h->st, N, h->d.ldN, h->d_pos, kr_of(hz), scale, kSoft * h->cfg.dx, 0, h->Gcm[f]);
+    HC(h, hipGetLastError());
+    HC(h, hipStreamSynchronize(h->st));
+    h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
+    const double b = h->synth_beta * hz / kFref;
+    for (int r = 0; r < ne - 1; r++)
+        for (int n = 0; n < N; n++) {
+            const int o = (int)h->pos[3 * n + 2];
+            h->gobj_extra[f][(size_t)r * N + n] = zmk((r == 0 ? (o == 0) : (o == 1)) ? b : 0.0, 0.0);
+        }
+    h->g_loaded[f] = 1;
+    return PNGF_OK;
+}
+
+int pngf_synth_g_host(int32_t layout, int32_t tx, int32_t ty, int32_t cx, int32_t cy, int32_t ne, double hz,
+                      pngf_c128* file_buf) {
+    if ((layout != 0 && layout != 1) || tx < 1 || ty < 1 || cx < 1 || cy < 2 || ne < 1 || ne > 3 || !(hz > 0) ||
+        !file_buf)
+        return PNGF_ERR_ARG;
+    Layout L;
+    L.setup(layout, tx, ty, cx, cy);
+    const double dx = 0.01 / 60.0, dy = dx, dz = 0.01 / 180.0;
+    std::vector<double> pos;
+    synth_positions(L, dx, dy, pos);
+    zd scale; double beta;
+    synth_norm(L, pos, dx, dy, dz, scale, beta);
+    scale = zscale(scale, hz / kFref);
+    const double kr = kr_of(hz), soft2 = (kSoft * dx) * (kSoft * dx);
+    const size_t N = L.N, M = N + ne - 1;
+    zd* out = (zd*)file_buf;
+    for (size_t n = 0; n < M; n++)
+        for (size_t m = 0; m < M; m++) {
+            zd v = zmk(0.0, 0.0);
+            if (n < N && m < N) v = zmul(synth_entry(&pos[3 * m], &pos[3 * n], kr, soft2), scale);
+            else if (n < N && m >= N) {
+                const int o = (int)pos[3 * n + 2];
+                const int r = (int)(m - N);
+                v = zmk((r == 0 ? (o == 0) : (o == 1)) ? beta * hz / kFref : 0.0, 0.0);
+            }
+            out[m + M * n] = v;
+        }
+    return PNGF_OK;
+}
+
+int pngf_set_tile_map(pngf_handle* h, const int32_t* tm) {
+    if (!h || !tm) return fail(h, PNGF_ERR_ARG, "null tile map");
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "tile map is fixed after pngf_init_state (use pngf_accept)");
+    const int T2 = h->L.Tx * h->L.Ty;
+    for (int i = 0; i < T2; i++) h->tile[i] = tm[i] ? 1 : 0;
+    h->L.mask(h->tile.data(), h->P);
+    HC(h, hipMemcpy(h->d_tile, h->tile.data(), T2 * sizeof(int32_t), hipMemcpyHostToDevice));
+    HC(h, hipMemcpy(h->d_P, h->P.data(), h->P.size(), hipMemcpyHostToDevice));
+    h->tile_set = true;
+    return PNGF_OK;
+}
+
+int pngf_set_source_port(pngf_handle* h, int32_t tx, int32_t ty, int32_t yoff, int32_t ysz) {
+    if (!h) return PNGF_ERR_ARG;
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "port is fixed after pngf_init_state");
+    if (!h->L.port_ok(yoff, ysz) || tx < 0 || ty < 0 || tx >= h->L.Tx || ty >= h->L.Ty)
+        return fail(h, PNGF_ERR_ARG, "set_source_port: invalid port y position and height parameters");
+    if (h->L.kind == PNGF_LAYOUT_DISJOINT && ysz != 2)
+        return fail(h, PNGF_ERR_ARG, "the disjoint layout has exactly 2 port rows");
+    h->L.set_port(tx, ty, yoff, ysz);
+    if (h->L.kind == PNGF_LAYOUT_DISJOINT) {
+        synth_positions(h->L, h->cfg.dx, h->cfg.dy, h->pos);
+        HC(h, hipMemcpy(h->d_pos, h->pos.data(), h->pos.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->synth_scale, h->synth_beta);
+    return PNGF_OK;
+}
+
+int pngf_init_state(pngf_handle* h) {
+    if (!h) return PNGF_ERR_ARG;
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "already initialised");
+    if (!h->tile_set) return fail(h, PNGF_ERR_STATE, "set_tile_map first");
+    for (int f = 0; f < h->d.nf; f++)
+        if (!h->g_loaded[f]) return fail(h, PNGF_ERR_STATE, "G of frequency slot " + std::to_string(f) + " not set");
+    const int nf = h->d.nf, ne = h->d.ne, N = h->d.N;
+    const size_t ld = h->d.ldN;
+    build_groups(h);
+    const Groups& G = h->grp;
+    h->d.ngroups = G.n;
+    h->d.gmax = G.gmax;
+    CK(dalloc_t(h, &h->d_gtiles, G.tiles.size()));
+    CK(dalloc_t(h, &h->d_gnt, G.ntiles.size()));
+    CK(dalloc_t(h, &h->d_gcomps, G.comps.size()));
+    CK(dalloc_t(h, &h->d_gnc, G.ncomp.size()));
+    CK(dalloc_t(h, &h->d_batch, G.n + 8));
+    HC(h, hipMemcpy(h->d_gtiles, G.tiles.data(), G.tiles.size() * 4, hipMemcpyHostToDevice));
+    HC(h, hipMemcpy(h->d_gnt, G.ntiles.data(), G.ntiles.size() * 4, hipMemcpyHostToDevice));
+    HC(h, hipMemcpy(h->d_gcomps, G.comps.data(), G.comps.size() * 4, hipMemcpyHostToDevice));
+    HC(h, hipMemcpy(h->d_gnc, G.ncomp.data(), G.ncomp.size() * 4, hipMemcpyHostToDevice));
+    std::vector<int32_t> portv(h->L.port.begin(), h->L.port.end());
+    HC(h, hipMemcpy(h->d_small + 400, portv.data(), portv.size() * 4, hipMemcpyHostToDevice));
+    const int gmax = G.gmax;
+    CK(dalloc_t(h, &h->d_eobj_out, (size_t)G.n * nf * ne));
+    CK(dalloc_t(h, &h->d_F, (size_t)G.n));
+    dfree(h, h->d_info);
+    CK(dalloc_t(h, &h->d_info, (size_t)G.n * nf + 64));
+    if ((size_t)G.n * nf + 64 > (size_t)(kPinInts - kPinInfo) || G.n > kPinInfo - kPinBatch)
+        return fail(h, PNGF_ERR_ARG, "design too large for the staging buffers");
+    h->acc_stride = 2 * gmax * gmax + 2 * gmax + 4 + ne * gmax;
+    if (gmax > kLdsGmax)
+        CK(dalloc_t(h, &h->d_lu, (size_t)std::max(G.n, 1) * nf * gmax * (gmax + 1)));
+    CK(dalloc_t(h, &h->d_acc, (size_t)nf * h->acc_stride));
+    if (h->cfg.method == PNGF_METHOD_LOWRANK)
+        CK(dalloc_t(h, &h->vecs.Dblk, (size_t)nf * G.n * gmax * gmax));
+    HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
+    for (int f = 0; f < nf; f++) CK(init_freq(h, f));
+    HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
+    if (h->cfg.method == PNGF_METHOD_LOWRANK) {
+        size_t freeb = 0, totb = 0;
+        HC(h, hipMemGetInfo(&freeb, &totb));
+        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (64u << 20);
+        size_t avail = freeb > scratch ? freeb - scratch : 0;
+        if (h->cfg.mem_limit > 0) {
+            size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
+            avail = std::min(avail, lim);
+        }
+        const size_t per_col = (size_t)nf * 2 * ld * sizeof(zd);  // A_all column + M_all row
+        int maxdef = h->cfg.max_deferred > 0 ? h->cfg.max_deferred : 384;
+        size_t cols = std::min<size_t>((size_t)maxdef * gmax, (size_t)(0.85 * avail) / per_col);
+        cols = cols / gmax * gmax;
+        if (cols < (size_t)gmax) return fail(h, PNGF_ERR_NOMEM, "no device memory for the deferred factors");
+        h->Kmax = (int)cols;
+        h->d.Kmax = h->Kmax;
+        for (int f = 0; f < nf; f++) {
+            CK(dalloc_t(h, &h->Aall[f], (size_t)N * h->Kmax));
+            CK(dalloc_t(h, &h->Mall[f], (size_t)h->Kmax * ld));
+        }
+        h->small_stride = (size_t)h->Kmax * gmax;
+        CK(dalloc_t(h, &h->d_Mc, nf * h->small_stride));
+        CK(dalloc_t(h, &h->d_Ar, nf * h->small_stride));
+        h->bt_stride = (size_t)N * gmax;
+        CK(dalloc_t(h, &h->d_Bt, nf * h->bt_stride));
+    } else {
+        h->d.Kmax = 0;
+        h->v_stride = (size_t)gmax * ld;
+        CK(dalloc_t(h, &h->d_V, nf * h->v_stride));
+        CK(dalloc_t(h, &h->d_VC, nf * h->v_stride));
+        CK(dalloc_t(h, &h->d_CU, nf * h->v_stride));
+        CK(dalloc_t(h, &h->d_M, nf * h->v_stride));
+        CK(dalloc_t(h, &h->d_rut, (size_t)nf * ne * 96));
+        const size_t items = std::max<size_t>(h->nsplit_items, (size_t)G.n * nf);
+        h->partial_elems = items * (gmax + ne) * (gmax + 1);
+        CK(dalloc_t(h, &h->d_partial, h->partial_elems));
+    }
+    CK(upload_tabs(h));
+    std::vector<zd> e;
+    CK(download_eobj(h, e));
+    h->inited = true;
+    return PNGF_OK;
+}
+
+int pngf_eval(pngf_handle* h, const int32_t* tiles, int32_t nt, pngf_c128* new_eobj, double* F) {
+    if (!h) return PNGF_ERR_ARG;
+    CK(check_ready(h));
+    int g;
+    CK(group_for(h, tiles, nt, &g));
+    CK(eval_launch(h, &g, 1));
+    int sel; double fs;
+    CK(sync_sel(h, 1, &sel, &fs, true));
+    const size_t n = (size_t)h->d.nf * h->d.ne;
+    std::vector<zd> e(n);
+    HC(h, hipMemcpy(e.data(), h->d_eobj_out, n * sizeof(zd), hipMemcpyDeviceToHost));
+    if (new_eobj) std::memcpy(new_eobj, e.data(), n * sizeof(zd));
+    if (F) HC(h, hipMemcpy(F, h->d_F, sizeof(double), hipMemcpyDeviceToHost));
+    return PNGF_OK;
+}
+
+int pngf_eval_batch(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand, double* F_out,
+                    pngf_c128* eobj_out) {
+    if (!h || !tiles || !ntiles || ncand < 0) return fail(h, PNGF_ERR_ARG, "bad batch");
+    CK(check_ready(h));
+    const int mf = h->cfg.max_flips;
+    const size_t per = (size_t)h->d.nf * h->d.ne;
+    std::vector<int> gs;
+    std::vector<int> idx;
+    auto flush_batch = [&]() -> int {
+        if (gs.empty()) return PNGF_OK;
+        CK(eval_launch(h, gs.data(), (int)gs.size()));
+        int sel; double fs;
+        CK(sync_sel(h, (int)gs.size(), &sel, &fs, true));
+        std::vector<double> F(gs.size());
+        std::vector<zd> e(gs.size() * per);
+        HC(h, hipMemcpy(F.data(), h->d_F, F.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HC(h, hipMemcpy(e.data(), h->d_eobj_out, e.size() * sizeof(zd), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < gs.size(); i++) {
+            if (F_out) F_out[idx[i]] = F[i];
+            if (eobj_out) std::memcpy(eobj_out + (size_t)idx[i] * per, &e[i * per], per * sizeof(zd));
+        }
+        gs.clear(); idx.clear();
+        return PNGF_OK;
+    };
+    for (int c = 0; c < ncand; c++) {
+        int g;
+        CK(group_for(h, tiles + (size_t)c * mf, ntiles[c], &g));
+        if (g == h->adhoc) {
+            CK(flush_batch());
+            gs.push_back(g); idx.push_back(c);
+            CK(flush_batch());
+        } else {
+            gs.push_back(g); idx.push_back(c);
+            if ((int)gs.size() >= h->ngroups_drv) CK(flush_batch());
+        }
+    }
+    return flush_batch();
+}
+
+int pngf_accept(pngf_handle* h, const int32_t* tiles, int32_t nt) {
+    if (!h) return PNGF_ERR_ARG;
+    CK(check_ready(h));
+    int g;
+    CK(group_for(h, tiles, nt, &g));
+    CK(accept_group(h, g));
+    HC(h, hipMemcpyAsync(h->h_pin_i + kPinInfo, h->d_info, sizeof(int32_t) * h->d.nf, hipMemcpyDeviceToHost, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    if (h->cfg.method == PNGF_METHOD_LOWRANK)
+        for (int f = 0; f < h->d.nf; f++)
+            if (h->h_pin_i[kPinInfo + f] != 0) return fail(h, PNGF_ERR_SINGULAR, "capacitance matrix is singular");
+    HC(h, hipStreamSynchronize(h->st));
+    prof_collect(h);
+    return PNGF_OK;
+}
+
+int pngf_get_state(pngf_handle* h, pngf_c128* eobj, pngf_c128* vinc, pngf_c128* ln, pngf_c128* s11, double* F) {
+    if (!h) return PNGF_ERR_ARG;
+    CK(check_ready(h));
+    std::vector<zd> e;
+    CK(download_eobj(h, e));
+    const int nf = h->d.nf, ne = h->d.ne;
+    if (eobj) std::memcpy(eobj, e.data(), e.size() * sizeof(zd));
+    for (int f = 0; f < nf; f++) {
+        if (vinc) { vinc[f].re = h->fc[f].vinc.x; vinc[f].im = h->fc[f].vinc.y; }
+        if (ln) { ln[2 * f] = {h->fc[f].ln1.x, h->fc[f].ln1.y}; ln[2 * f + 1] = {h->fc[f].ln2.x, h->fc[f].ln2.y}; }
+        if (s11) {
+            zd u = zadd(e[(size_t)f * ne], h->fc[f].vinc);
+            zd zin = zmk(-u.x * h->op.zfac, -u.y * h->op.zfac);
+            zd s = zdiv(zmk(zin.x - h->op.z0, zin.y), zmk(zin.x + h->op.z0, zin.y));
+            s11[f] = {s.x, s.y};
+        }
+    }
+    if (F) *F = host_objective(h, e.data());
+    return PNGF_OK;
+}
+
+int pngf_get_tile_map(pngf_handle* h, int32_t* tm) {
+    if (!h || !tm) return fail(h, PNGF_ERR_ARG, "null tile map");
+    for (size_t i = 0; i < h->tile.size(); i++) tm[i] = h->tile[i];
+    return PNGF_OK;
+}
+
+int pngf_get_cinv(pngf_handle* h, int32_t f, pngf_c128* out) {
+    if (!h || !out || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad slot");
+    CK(check_ready(h));
+    if (!h->Cinv[f]) return fail(h, PNGF_ERR_STATE, "C^-1 is not kept by this method");
+    HC(h, hipMemcpy2D(out, h->d.N * sizeof(zd), h->Cinv[f], h->d.ldN * sizeof(zd), h->d.N * sizeof(zd), h->d.N,
+                      hipMemcpyDeviceToHost));
+    return PNGF_OK;
+}
+
+int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out) {
+    if (!h || !out || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad slot");
+    CK(check_ready(h));
+    if (!h->W0[f]) return fail(h, PNGF_ERR_STATE, "W is kept by the LOWRANK method only");
+    const int N = h->d.N;
+    const size_t ld = h->d.ldN;
+    zd* tmp = nullptr;
+    CK(dalloc_t(h, &tmp, ld * N));
+    HC(h, hipMemcpyAsync(tmp, h->W0[f], ld * N * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+    if (h->K > 0) {
+        const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, N, N, h->K, &mone,
+                            (rbz*)h->Mall[f], ld, (rbz*)h->Aall[f], h->Kmax, &one, (rbz*)tmp, ld));
+    }
+    HC(h, hipStreamSynchronize(h->st));
+    HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
+    dfree(h, tmp);
+    return PNGF_OK;
+}
+
+int pngf_get_g(pngf_handle* h, int32_t f, pngf_c128* out) {
+    if (!h || !out || f < 0 || f >= h->d.nf) return fail(h, PNGF_ERR_ARG, "bad slot");
+    CK(check_ready(h));
+    if (!h->Grow[f]) return fail(h, PNGF_ERR_STATE, "G is kept by the PLAIN method only");
+    // Grow is row-major G = column-major G^T; return column-major G via a device transpose
+    const int N = h->d.N;
+    const size_t ld = h->d.ldN;
+    zd* tmp = nullptr;
+    CK(dalloc_t(h, &tmp, ld * N));
+    launch_transpose(h->st, N, (int)ld, h->Grow[f], tmp);
+    HC(h, hipStreamSynchronize(h->st));
+    HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
+    dfree(h, tmp);
+    return PNGF_OK;
+}
+
+// Device-side snapshot of the whole mutable state (LOWRANK: W0 with the pending
+// factors folded in, blocks, q, R, Eobj; PLAIN: C^-1, S, R, Eobj; both: mask and
+// tile map).  pngf_restore returns to it; used to repeat a DBS run from the same
+// start and as the resume point of a long run.
+int pngf_checkpoint(pngf_handle* h) {
+    if (!h) return PNGF_ERR_ARG;
+    CK(check_ready(h));
+    const Dims& d = h->d;
+    const size_t ld = d.ldN, big = ld * d.N;
+    const bool lr = h->cfg.method == PNGF_METHOD_LOWRANK;
+    if (lr) CK(flush_factors(h));
+    if (!h->ckpt) {
+        h->ck_big.assign(d.nf, nullptr);
+        for (int f = 0; f < d.nf; f++) CK(dalloc_t(h, &h->ck_big[f], big));
+        CK(dalloc_t(h, &h->ck_vec, (size_t)d.nf * ld * (2 + d.ne) + (size_t)d.nf * d.ne));
+        if (lr) CK(dalloc_t(h, &h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax));
+        CK(dalloc_t(h, &h->ck_P, d.N));
+    }
+    hipStream_t st = h->st;
+    for (int f = 0; f < d.nf; f++)
+        HC(h, hipMemcpyAsync(h->ck_big[f], lr ? h->W0[f] : h->Cinv[f], big * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    zd* v = h->ck_vec;
+    HC(h, hipMemcpyAsync(v, lr ? h->vecs.q : h->vecs.S, d.nf * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(v + d.nf * ld, h->vecs.R, d.nf * d.ne * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(v + d.nf * ld * (1 + d.ne), h->vecs.Eobj, d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    if (lr) HC(h, hipMemcpyAsync(h->ck_D, h->vecs.Dblk, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->ck_P, h->d_P, d.N, hipMemcpyDeviceToDevice, st));
+    HC(h, hipStreamSynchronize(st));
+    h->ck_tile = h->tile;
+    h->ck_Ph = h->P;
+    h->ckpt = true;
+    return PNGF_OK;
+}
+
+int pngf_restore(pngf_handle* h) {
+    if (!h) return PNGF_ERR_ARG;
+    CK(check_ready(h));
+    if (!h->ckpt) return fail(h, PNGF_ERR_STATE, "no checkpoint");
+    const Dims& d = h->d;
+    const size_t ld = d.ldN, big = ld * d.N;
+    const bool lr = h->cfg.method == PNGF_METHOD_LOWRANK;
+    hipStream_t st = h->st;
+    for (int f = 0; f < d.nf; f++)
+        HC(h, hipMemcpyAsync(lr ? h->W0[f] : h->Cinv[f], h->ck_big[f], big * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    zd* v = h->ck_vec;
+    HC(h, hipMemcpyAsync(lr ? h->vecs.q : h->vecs.S, v, d.nf * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->vecs.R, v + d.nf * ld, d.nf * d.ne * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->vecs.Eobj, v + d.nf * ld * (1 + d.ne), d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    if (lr) HC(h, hipMemcpyAsync(h->vecs.Dblk, h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->d_P, h->ck_P, d.N, hipMemcpyDeviceToDevice, st));
+    h->tile = h->ck_tile;
+    h->P = h->ck_Ph;
+    HC(h, hipMemcpyAsync(h->d_tile, h->tile.data(), h->tile.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HC(h, hipStreamSynchronize(st));
+    h->K = 0;
+    return PNGF_OK;
+}
+
+int pngf_profile(pngf_handle* h, int32_t enable) {
+    if (!h) return PNGF_ERR_ARG;
+    h->prof = enable != 0;
+    for (auto& k : h->kst) k = pngf_kstat{};
+    h->acc_seconds = 0.0;
+    return PNGF_OK;
+}
+
+int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out) {
+    if (!h || !out || which < 0 || which >= PNGF_K_COUNT) return fail(h, PNGF_ERR_ARG, "bad kernel id");
+    *out = h->kst[which];
+    return PNGF_OK;
+}
+
+int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
+    if (!h || !p || !r) return fail(h, PNGF_ERR_ARG, "null argument");
+    CK(check_ready(h));
+    std::memset(r, 0, sizeof(*r));
+    std::vector<zd> e;
+    CK(download_eobj(h, e));
+    double F_best = host_objective(h, e.data());
+    r->F_start = F_best;
+    const int bmin = p->batch_min > 0 ? p->batch_min : 4;
+    const int bmax_cfg = p->batch_max > 0 ? p->batch_max : (1 << 30);
+    const bool plain = h->cfg.method == PNGF_METHOD_PLAIN;
+    int64_t flip_count = 0, nacc = 0, spec = 0, batches = 0;
+    const double acc0 = h->acc_seconds;
+    h->flushes = 0;
+    HC(h, hipStreamSynchronize(h->st));
+    auto t0 = std::chrono::high_resolution_clock::now();
+    auto record = [&](int g, double F) -> int {
+        if (nacc >= p->max_accepts) return fail(h, PNGF_ERR_ARG, "accepted-flip arrays are full");
+        if (p->acc_tiles) p->acc_tiles[nacc] = h->grp.prim[g];
+        if (p->acc_eval) p->acc_eval[nacc] = flip_count;
+        if (p->acc_F) p->acc_F[nacc] = F;
+        return PNGF_OK;
+    };
+    int rc = PNGF_OK;
+    std::vector<int> batch;
+    int bs = bmin;
+    if (h->cfg.driver == PNGF_DRIVER_RASTER) {
+        std::vector<int> cl;
+        for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
+        const int ncl = (int)cl.size();
+        int pos = 0;
+        int64_t nonimp = 0;
+        while (nonimp < ncl) {
+            if (p->max_evals > 0 && flip_count >= p->max_evals) break;
+            int64_t B = ncl - nonimp;
+            if (p->max_evals > 0) B = std::min<int64_t>(B, p->max_evals - flip_count);
+            if (plain) B = std::min<int64_t>(B, std::min(bs, bmax_cfg));
+            batch.resize(B);
+            for (int64_t i = 0; i < B; i++) batch[i] = cl[(pos + i) % ncl];
+            h->h_pin_d[8] = F_best;
+            HC(h, hipMemcpyAsync(h->d_Fbest, h->h_pin_d + 8, sizeof(double), hipMemcpyHostToDevice, h->st));
+            CK(eval_launch(h, batch.data(), (int)B));
+            int sel; double Fs;
+            CK(sync_sel(h, (int)B, &sel, &Fs, true));
+            batches++;
+            if (sel >= 0) {
+                flip_count += sel;
+                if ((rc = record(batch[sel], Fs)) != PNGF_OK) break;
+                CK(accept_group(h, batch[sel]));
+                flip_count += 1;
+                nacc++;
+                spec += B - sel - 1;
+                F_best = Fs;
+                pos = (pos + sel + 1) % ncl;
+                nonimp = 0;
+                bs = bmin;
+            } else {
+                flip_count += B;
+                nonimp += B;
+                pos = (int)((pos + B) % ncl);
+                bs = std::min<int64_t>((int64_t)bs * 2, (int64_t)1 << 30);
+            }
+        }
+        r->converged = nonimp >= ncl;
+    } else {
+        Hat hat;
+        const int nhat = h->ngroups_drv;
+        hat.init(p->seed, nhat);
+        int last_tried = -1;
+        bool done = false;
+        while (!done) {
+            if (p->max_evals > 0 && flip_count >= p->max_evals) break;
+            // the draws that would follow if nothing were accepted (main.cpp:343-392)
+            Hat sim = hat;
+            int lt = last_tried;
+            batch.clear();
+            int64_t cap = p->max_evals > 0 ? p->max_evals - flip_count : (int64_t)1 << 40;
+            if (plain) cap = std::min<int64_t>(cap, std::min(bs, bmax_cfg));
+            while ((int64_t)batch.size() < cap) {
+                int rr = sim.draw();
+                if (rr < 0) break;
+                if (rr == lt) continue;
+                if (h->grp.ntiles[rr] == 0) continue;
+                lt = rr;
+                batch.push_back(rr);
+            }
+            if (batch.empty()) { done = true; r->converged = 1; break; }
+            h->h_pin_d[8] = F_best;
+            HC(h, hipMemcpyAsync(h->d_Fbest, h->h_pin_d + 8, sizeof(double), hipMemcpyHostToDevice, h->st));
+            CK(eval_launch(h, batch.data(), (int)batch.size()));
+            int sel; double Fs;
+            CK(sync_sel(h, (int)batch.size(), &sel, &Fs, true));
+            batches++;
+            const int upto = sel >= 0 ? sel : (int)batch.size() - 1;
+            // replay the real hat through the evaluated draws
+            int cnt = 0;
+            while (true) {
+                int rr = hat.draw();
+                if (rr == last_tried) continue;
+                if (h->grp.ntiles[rr] == 0) continue;
+                last_tried = rr;
+                if (cnt == upto) break;
+                cnt++;
+            }
+            if (sel >= 0) {
+                flip_count += sel;
+                if ((rc = record(batch[sel], Fs)) != PNGF_OK) break;
+                CK(accept_group(h, batch[sel]));
+                flip_count += 1;
+                nacc++;
+                spec += (int64_t)batch.size() - sel - 1;
+                F_best = Fs;
+                hat.reset();
+                bs = bmin;
+            } else {
+                flip_count += (int64_t)batch.size();
+                bs = std::min<int64_t>((int64_t)bs * 2, (int64_t)1 << 30);
+            }
+        }
+    }
+    HC(h, hipStreamSynchronize(h->st));
+    prof_collect(h);
+    r->seconds = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    r->flip_count = flip_count;
+    r->accepts = nacc;
+    r->speculative_evals = spec;
+    r->batches = batches;
+    r->F_final = F_best;
+    r->accept_seconds = h->acc_seconds - acc0;
+    r->flushes = h->flushes;
+    return rc;
+}
+
+}  // extern "C"

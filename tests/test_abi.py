@@ -1,0 +1,87 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol, and the
+host-only entry points (layout sizes, synthetic G) behave; no GPU compute calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pngf_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "pngf_gpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pngf_\w+)\s*\(", txt, re.M)))
+
+
+def test_exports_match_header():
+    L = P.lib()
+    decl = header_functions()
+    assert decl, "no declarations parsed"
+    assert sorted(P.EXPORTS) == decl
+    for name in decl:
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("layout,T,N", [
+    (P.LAYOUT_SHARED, 7, 924), (P.LAYOUT_SHARED, 21, 8064), (P.LAYOUT_SHARED, 31, 17484),
+    (P.LAYOUT_DISJOINT, 7, 1182), (P.LAYOUT_DISJOINT, 11, 2910), (P.LAYOUT_DISJOINT, 15, 5406),
+    (P.LAYOUT_DISJOINT, 21, 10590), (P.LAYOUT_DISJOINT, 25, 15006), (P.LAYOUT_DISJOINT, 31, 23070),
+])
+def test_layout_size(layout, T, N):
+    assert P.layout_size(layout, T) == N
+    assert O.layout_size(layout, T) == N
+
+
+def test_bad_layout_args():
+    assert P.lib().pngf_layout_size(7, 3, 3, 3, 3) == -1
+
+
+def test_synth_g_host_structure():
+    T = 3
+    for layout in (P.LAYOUT_SHARED, P.LAYOUT_DISJOINT):
+        G = P.synth_g_host(layout, T, 3, 30e9)
+        N = P.layout_size(layout, T)
+        assert G.shape == (N + 2, N + 2)
+        assert np.isfinite(G).all()
+        g = G[:N, :N]
+        # reciprocity of the dyadic kernel: G is complex symmetric
+        assert np.array_equal(g, g.T)
+        # normalisation: port block sums to -(50+30i)/zfac at 30 GHz
+        port = O.layout_port(layout, T, 1, 1)
+        zfac = (0.01 / 60) * 0.5 / (2 * (0.01 / 60) * (0.01 / 180))
+        s = g[np.ix_(port, port)].sum()
+        assert abs(s - (-(50 + 30j) / zfac)) < 1e-12 * abs(s)
+        # NF2FF rows: x / y orientation, zero padding columns
+        assert np.all(G[:, N:] == 0)
+        assert np.all((G[N, :N] != 0) | (G[N + 1, :N] != 0))
+
+
+def test_synth_g_host_bad_args():
+    buf = np.zeros(4, dtype=np.complex128)
+    assert P.lib().pngf_synth_g_host(3, 3, 3, 3, 3, 1, 30e9, buf.ctypes.data) == -1
+    assert P.lib().pngf_synth_g_host(1, 3, 3, 3, 3, 1, -1.0, buf.ctypes.data) == -1
+
+
+def test_create_rejects_bad_config():
+    with pytest.raises(P.PNGFError) as e:
+        P.Optimizer(P.LAYOUT_DISJOINT, 7, num_eobj=1, objective=P.OBJ_REF)
+    assert e.value.code == -1
+    with pytest.raises(P.PNGFError):
+        P.Optimizer(P.LAYOUT_SHARED, 8, num_eobj=3, max_flips=4, driver=P.DRIVER_REF, objective=P.OBJ_REF)
+
+
+def test_initial_pattern_convention():
+    from pngf_amd.problem import MT19937, initial_pattern
+    g = MT19937(5489)
+    v = [g() for _ in range(10000)]
+    assert v[-1] == 4123659995  # C++ [rand.predef] known answer
+    h = O.Hat(-514213645, 120)
+    g2 = MT19937(-514213645 & 0xFFFFFFFF)
+    assert [h.rand_binary() for _ in range(64)] == [g2() & 1 for _ in range(64)]
+    tm = initial_pattern(21, 0)
+    assert tm[10 * 21 + 9] == 1 and tm[10 * 21 + 11] == 1 and tm[10 * 21 + 10] == 0
+    assert tm[9 * 21 + 10] == 0 and tm[11 * 21 + 10] == 0

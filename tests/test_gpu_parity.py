@@ -1,0 +1,162 @@
+"""GPU parity: the HIP library against the CPU oracle on the same G and seed.
+
+Bar (FP64): newEobj, Eobj and the objective within 1e-10 relative of the oracle,
+and the identical accepted-flip sequence (tile and evaluation index) in DBS.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pngf_amd as P
+from pngf_amd.problem import freq_list, initial_pattern
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def setup_pair(layout, T, freqs, method, ne=1, driver=P.DRIVER_RASTER, objective=P.OBJ_S11, max_flips=1,
+               max_deferred=0, seed=0, tm=None):
+    Gs = [P.synth_g_host(layout, T, ne, hz) for hz in freqs]
+    if tm is None:
+        tm = initial_pattern(T, seed)
+    mid = (T - 1) // 2
+    gpu = P.Optimizer(layout, T, num_freq=len(freqs), num_eobj=ne, max_flips=max_flips, method=method,
+                      objective=objective, driver=driver, max_deferred=max_deferred)
+    ors = []
+    for f, (hz, G) in enumerate(zip(freqs, Gs)):
+        gpu.set_frequency(f, hz)
+        gpu.set_g(f, G)
+        o = O.Optimizer(layout, T, G, ne=ne, max_flips=max_flips)
+        assert o.init_state(tm, (mid, mid, 1, 2)) == 0
+        ors.append(o)
+    gpu.set_tile_map(tm)
+    gpu.set_source_port(mid, mid, 1, 2)
+    gpu.init_state()
+    return gpu, ors, tm
+
+
+def oracle_state(ors):
+    E = np.array([o.scalars()["Eobj"] for o in ors])
+    V = np.array([o.scalars()["Vinc"] for o in ors])
+    return E, V
+
+
+def candidates(layout, T):
+    port = set(O.layout_port(layout, T, (T - 1) // 2, (T - 1) // 2).tolist())
+    out = []
+    for t in range(T * T):
+        tm = np.zeros(T * T, dtype=np.int32)
+        tm[t] = 1
+        comps = set(np.nonzero(O.layout_mask(layout, T, tm))[0].tolist())
+        if not comps & port:
+            out.append(t)
+    return out
+
+
+@pytest.mark.parametrize("layout", [P.LAYOUT_DISJOINT, P.LAYOUT_SHARED])
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_init_and_single_flips(layout, method):
+    T = 7
+    freqs = [30e9]
+    gpu, ors, tm = setup_pair(layout, T, freqs, method)
+    st = gpu.state()
+    E, V = oracle_state(ors)
+    assert rel(st["Eobj"], E) < RTOL
+    assert rel(st["Vinc"], V) < RTOL
+    cands = candidates(layout, T)
+    F, e = gpu.eval_batch([[t] for t in cands])
+    for i, t in enumerate(cands):
+        ref = np.array([o.compute_updated_Eobj([t]) for o in ors])
+        assert rel(e[i], ref) < RTOL, (t, e[i], ref)
+        Fo, _, _ = O.objective(O.OBJ_S11, ref, V, None, freqs, DX, DY, DZ)
+        assert abs(F[i] - Fo) <= RTOL * abs(Fo)
+
+
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_accept_sequence(method):
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    layout = P.LAYOUT_DISJOINT
+    gpu, ors, tm = setup_pair(layout, T, freqs, method)
+    cands = candidates(layout, T)
+    for t in [3, 17, 30, 17, 44, 8]:
+        for o in ors:
+            o.compute_updated_Eobj([t])
+            o.accept()
+        gpu.accept([t])
+        E, V = oracle_state(ors)
+        assert rel(gpu.state()["Eobj"], E) < RTOL
+    assert np.array_equal(gpu.tile_map(), ors[0].tile_map())
+    F, e = gpu.eval_batch([[t] for t in cands])
+    for i, t in enumerate(cands):
+        ref = np.array([o.compute_updated_Eobj([t]) for o in ors])
+        assert rel(e[i], ref) < RTOL
+
+
+def run_both(gpu, ors, freqs, objective, seed=0, max_evals=0):
+    g = gpu.dbs_run(max_evals=max_evals, seed=seed)
+    o = O.dbs_run(ors, objective, freqs, DX, DY, DZ, seed=seed, max_evals=max_evals)
+    return g, o
+
+
+@pytest.mark.parametrize("layout", [P.LAYOUT_DISJOINT, P.LAYOUT_SHARED])
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_dbs_raster_identical_sequence(layout, method):
+    T = 7
+    freqs = [30e9]
+    gpu, ors, tm = setup_pair(layout, T, freqs, method)
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert g["flip_count"] == o["flip_count"]
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    assert np.array_equal(gpu.tile_map(), o["final_map"])
+    assert g["converged"]
+
+
+def test_dbs_multifreq_and_flush():
+    """3 frequencies, deferred factors folded into W every 3 accepts."""
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3)
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["flushes"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_dbs_reference_driver(method):
+    """MagicHat order, mirror-symmetric 2/4-tile flips (k up to 96), REF objective, ne = 3."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    seed = -514213645
+    gpu, ors, tm = setup_pair(P.LAYOUT_SHARED, T, freqs, method, ne=3, driver=P.DRIVER_REF,
+                              objective=P.OBJ_REF, max_flips=4, seed=1)
+    g, o = run_both(gpu, ors, freqs, O.OBJ_REF, seed=seed)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
+def test_checkpoint_restore_repeats_run():
+    T = 7
+    freqs = [30e9]
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK)
+    gpu.checkpoint()
+    a = gpu.dbs_run()
+    gpu.restore()
+    b = gpu.dbs_run()
+    assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
+    assert a["acc_F"].tolist() == b["acc_F"].tolist()
