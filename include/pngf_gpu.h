@@ -155,6 +155,9 @@ typedef struct {
     double* acc_F;         /* [max_accepts] objective after each accept (may be null) */
     int32_t batch_min;     /* PLAIN: first speculative batch after an accept (0 = default) */
     int32_t batch_max;     /* PLAIN: largest speculative batch (0 = default) */
+    int32_t rng_discard;   /* PNGF_DRIVER_REF: outputs of the hat's mt19937 already consumed before
+                              the loop (main.cpp:195 draws (T/2+1)^2 bits for INIT_RANDOM; 0 for
+                              the default INIT_EMPTY) */
 } pngf_dbs_params;
 
 typedef struct {

@@ -53,7 +53,7 @@ class Config(ct.Structure):
 class DbsParams(ct.Structure):
     _fields_ = [("max_evals", ct.c_int64), ("seed", ct.c_int32), ("max_accepts", ct.c_int32),
                 ("acc_tiles", ct.c_void_p), ("acc_eval", ct.c_void_p), ("acc_F", ct.c_void_p),
-                ("batch_min", ct.c_int32), ("batch_max", ct.c_int32)]
+                ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32)]
 
 
 class DbsResult(ct.Structure):
@@ -257,12 +257,13 @@ class Optimizer:
         return out
 
     # ---- DBS loop (main.cpp:338-459)
-    def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0):
+    def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0,
+                rng_discard=0):
         acc_t = np.zeros(max_accepts, dtype=np.int32)
         acc_e = np.zeros(max_accepts, dtype=np.int64)
         acc_F = np.zeros(max_accepts)
         p = DbsParams(max_evals, seed, max_accepts, acc_t.ctypes.data, acc_e.ctypes.data,
-                      acc_F.ctypes.data, batch_min, batch_max)
+                      acc_F.ctypes.data, batch_min, batch_max, rng_discard)
         r = DbsResult()
         self._ck(lib().pngf_dbs_run(self.h, ct.byref(p), ct.byref(r)), "dbs_run")
         na = r.accepts

@@ -51,6 +51,14 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # main.cpp-shaped command-line driver over the C ABI (host code only)
+    cli_src = os.path.join(HERE, "cli", "pngf_opt.cpp")
+    cli = os.path.join(HERE, "cli", "pngf_opt")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), cli_src, "-o", cli,
+           "-L" + HERE, "-lpngf_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cli build failed:\n%s" % r.stderr)
     return LIB
 
 

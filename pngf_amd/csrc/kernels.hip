@@ -637,6 +637,107 @@ __global__ void __launch_bounds__(256) lr_skinny_kernel(Dims d, DevTabs tb, cons
     }
 }
 
+// FP64 MFMA form of the same products (v_mfma_f64_16x16x4f64).  The complex
+// product is run as a real one: out_re/out_im interleaved (2k real columns) =
+// A_real (n x 2K, the interleaved re/im storage of A) x B_real (2K x 2k) with
+//   B_real[2K'+0][2c+0] =  Re b   B_real[2K'+0][2c+1] = Im b
+//   B_real[2K'+1][2c+0] = -Im b   B_real[2K'+1][2c+1] = Re b      (b = B[K'][c]).
+// One wave owns 16 output rows and all CT 16-column tiles; 8 waves share the
+// B values staged in LDS.  The reduction order over the inner index is free, so
+// at sub-step s lane l feeds inner element 4*(l/16) + s: each lane then loads two
+// consecutive complex entries of A per 8-entry chunk (coalesced 128-256 B rows).
+// Fragment layouts (16x16x4 f64): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16(i/4), element i % 4.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int CT, bool ROW>
+__global__ void __launch_bounds__(512) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
+                                                             int K, const zd* __restrict__ small, int small_stride,
+                                                             zd* Bt, size_t bt_stride) {
+    constexpr int KC = CT <= 3 ? 64 : 16;  // complex inner entries per LDS stage
+    constexpr int BW = CT * 8;             // complex columns held (>= k)
+    __shared__ zd Bs[KC][BW + 1];
+    const int f = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int n0 = blockIdx.x * 128 + wave * 16;
+    const int n = n0 + lr;
+    const int nc = n < d.N ? n : d.N - 1;  // clamped row for loads
+    const zd* A = tb.Aall[f];
+    const zd* M = tb.Mall[f];
+    const zd* W0 = tb.W0[f];
+    const zd* sm = small + (size_t)f * small_stride;
+    d4v acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) acc[c] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KC * BW; e += 512) {
+            const int kk = e / BW, c = e - kk * BW;
+            zd v = zmk(0.0, 0.0);
+            if (k0 + kk < K && c < k) v = ROW ? sm[(size_t)c * K + k0 + kk] : sm[(size_t)(k0 + kk) * k + c];
+            Bs[kk][c] = v;
+        }
+        __syncthreads();
+        const int kend = min(KC, K - k0);
+        for (int kk0 = 0; kk0 < kend; kk0 += 8) {
+            zd a[2];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int Kp = k0 + kk0 + 2 * lq + j;
+                zd v = zmk(0.0, 0.0);
+                if (Kp < K && n < d.N) v = ROW ? M[(size_t)Kp * d.ldN + nc] : A[(size_t)nc * d.Kmax + Kp];
+                a[j] = v;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                zd m[CT];
+#pragma unroll
+                for (int c = 0; c < CT; c++) m[c] = Bs[kk0 + 2 * lq + j][(c * 16 + lr) >> 1];
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const double av = p ? a[j].y : a[j].x;
+#pragma unroll
+                    for (int c = 0; c < CT; c++) {
+                        const bool odd = lr & 1;
+                        const double bv = p == 0 ? (odd ? m[c].y : m[c].x) : (odd ? m[c].x : -m[c].y);
+                        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+        if (cc >= k) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = n0 + 4 * lq + v;
+            if (row >= d.N) continue;
+            if (!ROW) {
+                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
+                ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - acc[c][v];
+            } else {
+                const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
+                ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - acc[c][v];
+            }
+        }
+    }
+}
+
+template <bool ROW>
+static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                               const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
+    dim3 grid((d.N + 127) / 128, d.nf);
+    if (k <= 24)
+        hipLaunchKernelGGL((lr_skinny_mfma_kernel<3, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+    else if (k <= 48)
+        hipLaunchKernelGGL((lr_skinny_mfma_kernel<6, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+    else
+        hipLaunchKernelGGL((lr_skinny_mfma_kernel<12, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+}
+
 template <bool ROW>
 static void launch_skinny(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                           const zd* small, int small_stride, zd* Bt, size_t bt_stride) {

@@ -1239,6 +1239,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
         Hat hat;
         const int nhat = h->ngroups_drv;
         hat.init(p->seed, nhat);
+        if (p->rng_discard > 0) hat.gen.discard(p->rng_discard);
         int last_tried = -1;
         bool done = false;
         while (!done) {
