@@ -75,6 +75,8 @@ typedef struct {
     double dx, dy, dz;    /* Yee cell size used by the port impedance (main.cpp:157-160) */
     double z0;            /* reference impedance for S11 (50 ohm, main.cpp:279) */
     int64_t mem_limit;    /* refuse device allocations beyond this many bytes (0 = no limit) */
+    double synth_soft;    /* synthetic G only: softening radius in cells (0 = 1.0); smaller values
+                             give better-conditioned C */
 } pngf_config;
 
 /* Creates a handle; selects the device and allocates nothing large yet.
@@ -101,7 +103,8 @@ int pngf_synth_g(pngf_handle* h, int32_t f);
 /* Host version of the synthetic generator, written in the acmefdfd file layout
    (M x M column-major).  The device version evaluates the same formula. */
 int pngf_synth_g_host(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x,
-                      int32_t cells_y, int32_t num_eobj, double hz, pngf_c128* file_buf);
+                      int32_t cells_y, int32_t num_eobj, double hz, double soft_cells,
+                      pngf_c128* file_buf);
 
 /* reference: PNGF_Optimizer::set_tile_map (pngfopt.hpp:669-674) */
 int pngf_set_tile_map(pngf_handle* h, const int32_t* tile_map);
@@ -158,6 +161,13 @@ typedef struct {
     int32_t rng_discard;   /* PNGF_DRIVER_REF: outputs of the hat's mt19937 already consumed before
                               the loop (main.cpp:195 draws (T/2+1)^2 bits for INIT_RANDOM; 0 for
                               the default INIT_EMPTY) */
+    int32_t reserved;
+    /* Optional training log (LOWRANK): for every scored candidate, its ranker inputs
+       (pngf_ranker_dims din floats) and the exact change of |S11_f|^2 at every frequency
+       (dout floats), appended until log_cap rows. */
+    float* log_x;
+    float* log_y;
+    int64_t log_cap;
 } pngf_dbs_params;
 
 typedef struct {
@@ -170,12 +180,55 @@ typedef struct {
     double accept_seconds;     /* device time spent in accepts (event-timed) */
     int32_t flushes;           /* LOWRANK: times the deferred factors were folded into W0 */
     int32_t converged;         /* 1 if stopped because no flip improves */
+    int64_t logged;            /* rows written to log_x / log_y */
+    int64_t ranked_scored;     /* ranked driver: candidates scored by the network */
+    int64_t full_sweeps;       /* ranked driver: exact full sweeps (fallbacks and final check) */
 } pngf_dbs_result;
 
 /* Runs the DBS loop of main.cpp:338-459 on the device with the driver chosen at create.
    Every accepted flip is the first improving one in visit order, as in the sequential
    reference loop; candidates are scored in speculative batches against frozen state. */
 int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r);
+
+/* ---- learned flip ranker (BF16 MLP, hand-written HIP forward / backward / Adam) ----
+   A din -> dh -> dh -> dout network with GELU, FP32 master weights, BF16 MFMA products
+   with FP32 accumulation.  No counterpart in the reference: it ranks candidate flips so
+   that exact scoring runs on the top-K only. */
+typedef struct pngf_mlp pngf_mlp;
+int pngf_mlp_create(int32_t din, int32_t dh, int32_t dout, int32_t seed, pngf_mlp** out);
+int pngf_mlp_destroy(pngf_mlp* m);
+const char* pngf_mlp_last_error(const pngf_mlp* m);
+int pngf_mlp_num_params(const pngf_mlp* m);
+/* parameters: W1[din][dh] b1[dh] W2[dh][dh] b2[dh] W3[dh][dout] b3[dout], FP32 */
+int pngf_mlp_get_params(pngf_mlp* m, float* out);
+int pngf_mlp_set_params(pngf_mlp* m, const float* in);
+int pngf_mlp_predict(pngf_mlp* m, const float* X, int32_t n, float* out);
+/* gradient of mean((f(X) - Y)^2) for one batch, no update */
+int pngf_mlp_grad(pngf_mlp* m, const float* X, const float* Y, int32_t n, float* grads, float* loss);
+int pngf_mlp_train(pngf_mlp* m, const float* X, const float* Y, int32_t n, int32_t epochs, int32_t batch,
+                   float lr, int32_t seed, float* loss);
+
+/* Ranker inputs for the current state (LOWRANK, raster driver): din = 51 + 8 num_freq,
+   dout = num_freq.  tiles: candidate tiles (single-tile flips). */
+int pngf_ranker_dims(pngf_handle* h, int32_t* din, int32_t* dout);
+int pngf_features(pngf_handle* h, const int32_t* tiles, int32_t ncand, float* out);
+
+typedef struct {
+    int32_t top_k;          /* candidates scored exactly per step */
+    int32_t window;         /* steps over which the hit rate is measured */
+    float min_hit_rate;     /* below it, steps fall back to exact full sweeps */
+    int32_t reserved;
+    const float* x_mean;    /* [din] input standardisation */
+    const float* x_std;
+    const float* y_mean;    /* [dout] output de-standardisation */
+    const float* y_std;
+} pngf_rank_params;
+/* Ranked DBS (S11 objective, raster candidates): each step the network scores every
+   candidate, the top_k are scored exactly in FP64 and the best improving one is
+   accepted; if none improves, an exact full sweep decides (accept its best, or stop:
+   converged).  flip_count counts exact evaluations only. */
+int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp, const pngf_dbs_params* p,
+                        pngf_dbs_result* r);
 
 /* Per-kernel timing collected with HIP events on the handle's stream (enable first). */
 #define PNGF_K_EVAL       0 /* candidate scoring (both methods) */

@@ -31,6 +31,9 @@ EXPORTS = [
     "pngf_set_source_port", "pngf_init_state", "pngf_eval", "pngf_eval_batch", "pngf_accept",
     "pngf_get_state", "pngf_get_tile_map", "pngf_get_cinv", "pngf_get_w", "pngf_dbs_run",
     "pngf_profile", "pngf_kernel_stats", "pngf_get_g", "pngf_checkpoint", "pngf_restore",
+    "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
+    "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
+    "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked",
 ]
 
 
@@ -47,20 +50,29 @@ class Config(ct.Structure):
                 ("objective", ct.c_int32), ("driver", ct.c_int32), ("max_deferred", ct.c_int32),
                 ("device", ct.c_int32), ("reserved", ct.c_int32), ("dx", ct.c_double),
                 ("dy", ct.c_double), ("dz", ct.c_double), ("z0", ct.c_double),
-                ("mem_limit", ct.c_int64)]
+                ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double)]
 
 
 class DbsParams(ct.Structure):
     _fields_ = [("max_evals", ct.c_int64), ("seed", ct.c_int32), ("max_accepts", ct.c_int32),
                 ("acc_tiles", ct.c_void_p), ("acc_eval", ct.c_void_p), ("acc_F", ct.c_void_p),
-                ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32)]
+                ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32),
+                ("reserved", ct.c_int32), ("log_x", ct.c_void_p), ("log_y", ct.c_void_p),
+                ("log_cap", ct.c_int64)]
 
 
 class DbsResult(ct.Structure):
     _fields_ = [("flip_count", ct.c_int64), ("accepts", ct.c_int64), ("speculative_evals", ct.c_int64),
                 ("batches", ct.c_int64), ("F_start", ct.c_double), ("F_final", ct.c_double),
                 ("seconds", ct.c_double), ("accept_seconds", ct.c_double), ("flushes", ct.c_int32),
-                ("converged", ct.c_int32)]
+                ("converged", ct.c_int32), ("logged", ct.c_int64), ("ranked_scored", ct.c_int64),
+                ("full_sweeps", ct.c_int64)]
+
+
+class RankParams(ct.Structure):
+    _fields_ = [("top_k", ct.c_int32), ("window", ct.c_int32), ("min_hit_rate", ct.c_float),
+                ("reserved", ct.c_int32), ("x_mean", ct.c_void_p), ("x_std", ct.c_void_p),
+                ("y_mean", ct.c_void_p), ("y_std", ct.c_void_p)]
 
 
 class KStat(ct.Structure):
@@ -89,7 +101,7 @@ def lib():
         "pngf_load_g": (I, [P, I, ct.c_char_p]),
         "pngf_set_g": (I, [P, I, P]),
         "pngf_synth_g": (I, [P, I]),
-        "pngf_synth_g_host": (I, [I, I, I, I, I, I, D, P]),
+        "pngf_synth_g_host": (I, [I, I, I, I, I, I, D, D, P]),
         "pngf_set_tile_map": (I, [P, P]),
         "pngf_set_source_port": (I, [P, I, I, I, I]),
         "pngf_init_state": (I, [P]),
@@ -104,6 +116,19 @@ def lib():
         "pngf_profile": (I, [P, I]),
         "pngf_kernel_stats": (I, [P, I, ct.POINTER(KStat)]),
         "pngf_get_g": (I, [P, I, P]),
+        "pngf_mlp_create": (I, [I, I, I, I, ct.POINTER(P)]),
+        "pngf_mlp_destroy": (I, [P]),
+        "pngf_mlp_last_error": (ct.c_char_p, [P]),
+        "pngf_mlp_num_params": (I, [P]),
+        "pngf_mlp_get_params": (I, [P, P]),
+        "pngf_mlp_set_params": (I, [P, P]),
+        "pngf_mlp_predict": (I, [P, P, I, P]),
+        "pngf_mlp_grad": (I, [P, P, P, I, P, P]),
+        "pngf_mlp_train": (I, [P, P, P, I, I, I, ct.c_float, I, P]),
+        "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
+        "pngf_features": (I, [P, P, I, P]),
+        "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
+                                    ct.POINTER(DbsResult)]),
         "pngf_checkpoint": (I, [P]),
         "pngf_restore": (I, [P]),
     }
@@ -123,12 +148,13 @@ def layout_size(layout, tiles_x, tiles_y=None, cells_x=3, cells_y=3):
     return lib().pngf_layout_size(layout, tiles_x, tiles_y or tiles_x, cells_x, cells_y)
 
 
-def synth_g_host(layout, tiles, num_eobj, hz, cells=3):
-    """Synthetic G in acmefdfd's file layout: (M, M) complex128, column-major (order='F')."""
+def synth_g_host(layout, tiles, num_eobj, hz, cells=3, soft=0.0):
+    """Synthetic G in acmefdfd's file layout: (M, M) complex128, column-major (order='F').
+    soft: softening radius in cells (0 = default 1.0)."""
     N = layout_size(layout, tiles, tiles, cells, cells)
     M = N + num_eobj - 1
     buf = np.zeros((M, M), dtype=np.complex128, order="F")
-    rc = lib().pngf_synth_g_host(layout, tiles, tiles, cells, cells, num_eobj, hz, _p(buf))
+    rc = lib().pngf_synth_g_host(layout, tiles, tiles, cells, cells, num_eobj, hz, soft, _p(buf))
     if rc != 0:
         raise PNGFError(rc, "pngf_synth_g_host")
     return buf
@@ -139,9 +165,9 @@ class Optimizer:
 
     def __init__(self, layout, tiles, num_freq=1, num_eobj=1, max_flips=1, method=METHOD_LOWRANK,
                  objective=OBJ_S11, driver=DRIVER_RASTER, max_deferred=0, device=0, mem_limit=0,
-                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3):
+                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0):
         self.cfg = Config(layout, tiles, tiles, cells, cells, num_eobj, max_flips, num_freq, method,
-                          objective, driver, max_deferred, device, 0, dx, dy, dz, z0, mem_limit)
+                          objective, driver, max_deferred, device, 0, dx, dy, dz, z0, mem_limit, synth_soft)
         self.h = ct.c_void_p()
         rc = lib().pngf_create(ct.byref(self.cfg), ct.byref(self.h))
         if rc != 0:
@@ -257,20 +283,61 @@ class Optimizer:
         return out
 
     # ---- DBS loop (main.cpp:338-459)
+    def _dbs_params(self, max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap):
+        acc = (np.zeros(max_accepts, dtype=np.int32), np.zeros(max_accepts, dtype=np.int64),
+               np.zeros(max_accepts))
+        log = None
+        if log_cap:
+            din, dout = self.ranker_dims()
+            log = (np.zeros((log_cap, din), dtype=np.float32), np.zeros((log_cap, dout), dtype=np.float32))
+        p = DbsParams(max_evals, seed, max_accepts, acc[0].ctypes.data, acc[1].ctypes.data, acc[2].ctypes.data,
+                      batch_min, batch_max, rng_discard, 0, log[0].ctypes.data if log else None,
+                      log[1].ctypes.data if log else None, log_cap)
+        return p, acc, log
+
+    @staticmethod
+    def _dbs_result(r, acc, log):
+        na = r.accepts
+        out = dict(flip_count=r.flip_count, accepts=na, speculative_evals=r.speculative_evals,
+                   batches=r.batches, F_start=r.F_start, F_final=r.F_final, seconds=r.seconds,
+                   accept_seconds=r.accept_seconds, flushes=r.flushes, converged=bool(r.converged),
+                   acc_tiles=acc[0][:na].copy(), acc_eval=acc[1][:na].copy(), acc_F=acc[2][:na].copy(),
+                   ranked_scored=r.ranked_scored, full_sweeps=r.full_sweeps)
+        if log is not None:
+            out["log_x"] = log[0][:r.logged].copy()
+            out["log_y"] = log[1][:r.logged].copy()
+        return out
+
     def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0,
-                rng_discard=0):
-        acc_t = np.zeros(max_accepts, dtype=np.int32)
-        acc_e = np.zeros(max_accepts, dtype=np.int64)
-        acc_F = np.zeros(max_accepts)
-        p = DbsParams(max_evals, seed, max_accepts, acc_t.ctypes.data, acc_e.ctypes.data,
-                      acc_F.ctypes.data, batch_min, batch_max, rng_discard)
+                rng_discard=0, log_cap=0):
+        p, acc, log = self._dbs_params(max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap)
         r = DbsResult()
         self._ck(lib().pngf_dbs_run(self.h, ct.byref(p), ct.byref(r)), "dbs_run")
-        na = r.accepts
-        return dict(flip_count=r.flip_count, accepts=na, speculative_evals=r.speculative_evals,
-                    batches=r.batches, F_start=r.F_start, F_final=r.F_final, seconds=r.seconds,
-                    accept_seconds=r.accept_seconds, flushes=r.flushes, converged=bool(r.converged),
-                    acc_tiles=acc_t[:na].copy(), acc_eval=acc_e[:na].copy(), acc_F=acc_F[:na].copy())
+        return self._dbs_result(r, acc, log)
+
+    # ---- learned ranker
+    def ranker_dims(self):
+        din, dout = ct.c_int32(), ct.c_int32()
+        self._ck(lib().pngf_ranker_dims(self.h, ct.byref(din), ct.byref(dout)), "ranker_dims")
+        return din.value, dout.value
+
+    def features(self, tiles):
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        din, _ = self.ranker_dims()
+        out = np.zeros((len(t), din), dtype=np.float32)
+        self._ck(lib().pngf_features(self.h, _p(t), len(t), _p(out)), "features")
+        return out
+
+    def dbs_run_ranked(self, mlp, norm, top_k=32, window=8, min_hit_rate=0.25, max_evals=0,
+                       max_accepts=100000, log_cap=0):
+        xm, xs, ym, ys = [np.ascontiguousarray(a, dtype=np.float32) for a in norm]
+        rp = RankParams(top_k, window, min_hit_rate, 0, xm.ctypes.data, xs.ctypes.data, ym.ctypes.data,
+                        ys.ctypes.data)
+        p, acc, log = self._dbs_params(max_evals, 0, max_accepts, 0, 0, 0, log_cap)
+        r = DbsResult()
+        self._ck(lib().pngf_dbs_run_ranked(self.h, mlp.h, ct.byref(rp), ct.byref(p), ct.byref(r)),
+                 "dbs_run_ranked")
+        return self._dbs_result(r, acc, log)
 
     def profile(self, enable=True):
         self._ck(lib().pngf_profile(self.h, int(enable)), "profile")
@@ -279,3 +346,71 @@ class Optimizer:
         k = KStat()
         self._ck(lib().pngf_kernel_stats(self.h, which, ct.byref(k)), "kernel_stats")
         return dict(launches=k.launches, total_ms=k.total_ms, bytes=k.bytes, flops=k.flops)
+
+
+class MLP:
+    """The learned flip ranker (din -> dh -> dh -> dout, GELU, BF16 MFMA / FP32 accumulation)."""
+
+    def __init__(self, din, dh, dout, seed=0):
+        self.h = ct.c_void_p()
+        rc = lib().pngf_mlp_create(din, dh, dout, seed, ct.byref(self.h))
+        if rc != 0:
+            msg = lib().pngf_mlp_last_error(self.h).decode() if self.h else "bad dimensions"
+            raise PNGFError(rc, msg)
+        self.din, self.dh, self.dout = din, dh, dout
+        self.nparam = lib().pngf_mlp_num_params(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pngf_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise PNGFError(rc, "%s: %s" % (what, lib().pngf_mlp_last_error(self.h).decode()))
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        self._ck(lib().pngf_mlp_get_params(self.h, _p(out)), "get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._ck(lib().pngf_mlp_set_params(self.h, _p(w)), "set_params")
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros((len(X), self.dout), dtype=np.float32)
+        self._ck(lib().pngf_mlp_predict(self.h, _p(X), len(X), _p(out)), "predict")
+        return out
+
+    def grad(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = ct.c_float()
+        self._ck(lib().pngf_mlp_grad(self.h, _p(X), _p(Y), len(X), _p(g), ct.byref(loss)), "grad")
+        return g, loss.value
+
+    def train(self, X, Y, epochs=10, batch=512, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        loss = ct.c_float()
+        self._ck(lib().pngf_mlp_train(self.h, _p(X), _p(Y), len(X), epochs, batch, lr, seed, ct.byref(loss)),
+                 "train")
+        return loss.value
+
+    def split(self, w=None):
+        """Parameter vector -> (W1, b1, W2, b2, W3, b3) views."""
+        w = self.params() if w is None else w
+        d, h, o = self.din, self.dh, self.dout
+        sizes = [d * h, h, h * h, h, h * o, o]
+        shapes = [(d, h), (h,), (h, h), (h,), (h, o), (o,)]
+        out, off = [], 0
+        for n, s in zip(sizes, shapes):
+            out.append(w[off:off + n].reshape(s))
+            off += n
+        return out

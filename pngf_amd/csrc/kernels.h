@@ -74,6 +74,13 @@ void launch_plain_small(hipStream_t st, const Dims& d, const DevVecs& v, const z
                         const zd* CU, size_t cu_stride, const zd* M, size_t m_stride, const int32_t* t,
                         const int32_t* s, int k, zd* rut);
 
+// ---- learned ranker inputs / labels (LOWRANK state)
+void launch_features(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, int Tx, int Ty,
+                     const ObjParams& op, float* out);
+void launch_labels(hipStream_t st, const Dims& d, const DevVecs& v, const ObjParams& op, const zd* eobj_out, int nb,
+                   float* out);
+
 // ---- init
 void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos /*[N][3]*/, double kr, zd scale,
                     double soft, int row_major, zd* out);

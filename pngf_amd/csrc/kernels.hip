@@ -559,86 +559,11 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
                        small_stride);
 }
 
-// Skinny products over the deferred factors (k <= 96 columns):
+// Skinny products over the deferred factors (k <= 96 columns), on FP64 MFMA
+// (v_mfma_f64_16x16x4f64):
 //   colblk: A_all[n][K+c] = W0[n][t_c] - sum_K' A_all[n][K'] Mc[K'][c]
 //   rowblk: Bt[n][c]      = W0[t_c][n] - sum_K' Ar[c][K'] M_all[K'][n]
-// 64 output rows per workgroup, K' staged 32 at a time through LDS; each thread
-// owns 2 rows x ceil(k/8) columns.
-constexpr int SK_R = 64, SK_K = 32;
-
-template <int KU, bool ROW>
-__global__ void __launch_bounds__(256) lr_skinny_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
-                                                        int K, const zd* __restrict__ small, int small_stride,
-                                                        zd* Bt, size_t bt_stride) {
-    __shared__ zd As[SK_R][SK_K + 1];
-    __shared__ zd Bs[SK_K][KU * 8 + 1];
-    const int f = blockIdx.y, tid = threadIdx.x;
-    const int n0 = blockIdx.x * SK_R;
-    const int rp = tid >> 3, cg = tid & 7;
-    const zd* A = tb.Aall[f];
-    const zd* M = tb.Mall[f];
-    const zd* W0 = tb.W0[f];
-    const zd* sm = small + (size_t)f * small_stride;
-    zd acc[2][KU];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int u = 0; u < KU; u++) acc[i][u] = zmk(0.0, 0.0);
-    for (int k0 = 0; k0 < K; k0 += SK_K) {
-        const int kl = min(SK_K, K - k0);
-        for (int e = tid; e < SK_R * SK_K; e += 256) {
-            int r, kk;
-            zd val = zmk(0.0, 0.0);
-            if (!ROW) {
-                r = e / SK_K; kk = e - r * SK_K;
-                const int n = n0 + r;
-                if (n < d.N && kk < kl) val = A[(size_t)n * d.Kmax + k0 + kk];
-            } else {
-                kk = e / SK_R; r = e - kk * SK_R;
-                const int n = n0 + r;
-                if (n < d.N && kk < kl) val = M[(size_t)(k0 + kk) * d.ldN + n];
-            }
-            As[r][kk] = val;
-        }
-        for (int e = tid; e < SK_K * KU * 8; e += 256) {
-            const int kk = e / (KU * 8), c = e - kk * (KU * 8);
-            zd val = zmk(0.0, 0.0);
-            if (kk < kl && c < k) val = ROW ? sm[(size_t)c * K + k0 + kk] : sm[(size_t)(k0 + kk) * k + c];
-            Bs[kk][c] = val;
-        }
-        __syncthreads();
-        for (int kk = 0; kk < kl; kk++) {
-            const zd a0 = As[2 * rp][kk], a1 = As[2 * rp + 1][kk];
-#pragma unroll
-            for (int u = 0; u < KU; u++) {
-                const zd m = Bs[kk][cg + 8 * u];
-                acc[0][u] = zfma(a0, m, acc[0][u]);
-                acc[1][u] = zfma(a1, m, acc[1][u]);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-        const int n = n0 + 2 * rp + i;
-        if (n >= d.N) continue;
-#pragma unroll
-        for (int u = 0; u < KU; u++) {
-            const int c = cg + 8 * u;
-            if (c >= k) continue;
-            if (!ROW) {
-                zd w = W0[(size_t)n * d.ldN + t[c]];
-                ((zd*)A)[(size_t)n * d.Kmax + K + c] = zsub(w, acc[i][u]);
-            } else {
-                zd w = W0[(size_t)t[c] * d.ldN + n];
-                Bt[(size_t)f * bt_stride + (size_t)n * d.gmax + c] = zsub(w, acc[i][u]);
-            }
-        }
-    }
-}
-
-// FP64 MFMA form of the same products (v_mfma_f64_16x16x4f64).  The complex
-// product is run as a real one: out_re/out_im interleaved (2k real columns) =
+// The complex product is run as a real one: out_re/out_im interleaved (2k real columns) =
 // A_real (n x 2K, the interleaved re/im storage of A) x B_real (2K x 2k) with
 //   B_real[2K'+0][2c+0] =  Re b   B_real[2K'+0][2c+1] = Im b
 //   B_real[2K'+1][2c+0] = -Im b   B_real[2K'+1][2c+1] = Re b      (b = B[K'][c]).
@@ -646,8 +571,9 @@ __global__ void __launch_bounds__(256) lr_skinny_kernel(Dims d, DevTabs tb, cons
 // B values staged in LDS.  The reduction order over the inner index is free, so
 // at sub-step s lane l feeds inner element 4*(l/16) + s: each lane then loads two
 // consecutive complex entries of A per 8-entry chunk (coalesced 128-256 B rows).
-// Fragment layouts (16x16x4 f64): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// D[i][j] in lane j + 16(i/4), element i % 4.
+// Fragment layouts (16x16x4 f64, measured on gfx950 with a lane probe): A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4
+// (each lane group holds every fourth row).
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 template <int CT, bool ROW>
@@ -713,7 +639,7 @@ __global__ void __launch_bounds__(512) lr_skinny_mfma_kernel(Dims d, DevTabs tb,
         if (cc >= k) continue;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int row = n0 + 4 * lq + v;
+            const int row = n0 + lq + 4 * v;
             if (row >= d.N) continue;
             if (!ROW) {
                 const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
@@ -738,25 +664,13 @@ static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb,
         hipLaunchKernelGGL((lr_skinny_mfma_kernel<12, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
 }
 
-template <bool ROW>
-static void launch_skinny(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                          const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
-    dim3 grid((d.N + SK_R - 1) / SK_R, d.nf);
-    if (k <= 24)
-        hipLaunchKernelGGL((lr_skinny_kernel<3, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
-    else if (k <= 48)
-        hipLaunchKernelGGL((lr_skinny_kernel<6, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
-    else
-        hipLaunchKernelGGL((lr_skinny_kernel<12, ROW>), grid, dim3(256), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
-}
-
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Mc, int small_stride) {
-    launch_skinny<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0);
+    launch_skinny_mfma<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0);
 }
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Ar, int small_stride, zd* Bt, size_t bt_stride) {
-    launch_skinny<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride);
+    launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride);
 }
 
 // out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb)
@@ -978,6 +892,96 @@ void launch_plain_small(hipStream_t st, const Dims& d, const DevVecs& v, const z
     hipLaunchKernelGGL(plain_rut_kernel, dim3(d.nf), dim3(64), 0, st, d, v, acc, acc_stride, t, s, k, rut);
     hipLaunchKernelGGL(plain_small_kernel, dim3((d.N + 255) / 256, d.nf), dim3(256), 0, st, d, v, acc, acc_stride,
                        CU, cu_stride, M, m_stride, t, rut, k);
+}
+
+// ------------------------------------------------------- ranker features ---
+// Inputs of the learned ranker for candidate group g at the current state
+// (LOWRANK state).  Row layout (din = 51 + 8 nf floats):
+//   [0, 49)   metal bits of the 7x7 tile neighbourhood of the primary tile (0 outside)
+//   49, 50    primary tile x, y scaled to [-1, 1]
+//   51 + 8f + {0..7}: Re/Im of four complex terms at frequency f:
+//       c1 = mean_c cap_cc                      (capacitance diagonal, cap = I + W[t,t] diag(s))
+//       c2 = zfac sum_c R0[t_c] dPE_c           (first-order change of the port voltage)
+//       c3 = zfac sum_c R0[t_c] s_c rhs_c / cap_cc   (its Jacobi correction)
+//       c4 = Zin_f / z0                          (current input impedance)
+__global__ void features_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
+                                const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                                const int32_t* __restrict__ batch, int Tx, int Ty, ObjParams op, float* out) {
+    __shared__ int t[96], sg[96], lp[96];
+    __shared__ int s_k;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int g = batch[b];
+    const int din = 51 + 8 * d.nf;
+    float* row = out + (size_t)b * din;
+    if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const int k = s_k;
+    const int prim = gt.tiles[4 * g];
+    const int ty = prim / Tx, tx = prim - ty * Tx;
+    for (int e = tid; e < 49; e += blockDim.x) {
+        const int x = tx + e % 7 - 3, y = ty + e / 7 - 3;
+        row[e] = (x >= 0 && x < Tx && y >= 0 && y < Ty) ? (float)tile[y * Tx + x] : 0.f;
+    }
+    if (tid == 0) {
+        row[49] = Tx > 1 ? 2.f * tx / (Tx - 1) - 1.f : 0.f;
+        row[50] = Ty > 1 ? 2.f * ty / (Ty - 1) - 1.f : 0.f;
+    }
+    for (int f = tid; f < d.nf; f += blockDim.x) {
+        const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * d.gmax * d.gmax;
+        const zd* q = v.q + (size_t)f * d.ldN;
+        const zd* einc = v.Einc + (size_t)f * d.ldN;
+        const zd* R0 = v.R + (size_t)f * d.ne * d.ldN;
+        zd c1 = zmk(0.0, 0.0), c2 = zmk(0.0, 0.0), c3 = zmk(0.0, 0.0);
+        for (int c = 0; c < k; c++) {
+            const zd dc = D[lp[c] * d.gmax + lp[c]];
+            const zd cap = sg[c] > 0 ? zmk(1.0 + dc.x, dc.y) : zmk(1.0 - dc.x, -dc.y);
+            c1 = zadd(c1, cap);
+            zd rhs = q[t[c]];
+            for (int c2i = 0; c2i < k; c2i++) {
+                const zd e = einc[t[c2i]];
+                const zd dpe = sg[c2i] > 0 ? zmk(-e.x, -e.y) : e;
+                rhs = zfma(D[lp[c] * d.gmax + lp[c2i]], dpe, rhs);
+            }
+            const zd e = einc[t[c]];
+            const zd dpe = sg[c] > 0 ? zmk(-e.x, -e.y) : e;
+            c2 = zfma(R0[t[c]], dpe, c2);
+            zd y = zdiv(rhs, cap);
+            if (sg[c] < 0) y = zmk(-y.x, -y.y);
+            c3 = zfma(R0[t[c]], y, c3);
+        }
+        if (k > 0) c1 = zscale(c1, 1.0 / k);
+        c2 = zscale(c2, op.zfac);
+        c3 = zscale(c3, op.zfac);
+        const zd u = zadd(v.Eobj[f * d.ne], v.fc[f].vinc);
+        const zd c4 = zmk(-u.x * op.zfac / op.z0, -u.y * op.zfac / op.z0);
+        float* o = row + 51 + 8 * f;
+        o[0] = (float)c1.x; o[1] = (float)c1.y; o[2] = (float)c2.x; o[3] = (float)c2.y;
+        o[4] = (float)c3.x; o[5] = (float)c3.y; o[6] = (float)c4.x; o[7] = (float)c4.y;
+    }
+}
+
+void launch_features(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, int Tx, int Ty,
+                     const ObjParams& op, float* out) {
+    hipLaunchKernelGGL(features_kernel, dim3(nb), dim3(64), 0, st, d, v, gt, tile, P, owners, batch, Tx, Ty, op, out);
+}
+
+// Labels: |S11_f(newEobj)|^2 - |S11_f(Eobj)|^2 for every (candidate, frequency).
+__global__ void labels_kernel(Dims d, DevVecs v, ObjParams op, const zd* __restrict__ eobj_out, int nb, float* out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * d.nf) return;
+    const int b = e / d.nf, f = e - b * d.nf;
+    auto s11sq = [&](zd E0) {
+        const zd u = zadd(E0, v.fc[f].vinc);
+        const zd zin = zmk(-u.x * op.zfac, -u.y * op.zfac);
+        return zabs2(zdiv(zmk(zin.x - op.z0, zin.y), zmk(zin.x + op.z0, zin.y)));
+    };
+    out[e] = (float)(s11sq(eobj_out[((size_t)b * d.nf + f) * d.ne]) - s11sq(v.Eobj[f * d.ne]));
+}
+
+void launch_labels(hipStream_t st, const Dims& d, const DevVecs& v, const ObjParams& op, const zd* eobj_out, int nb,
+                   float* out) {
+    hipLaunchKernelGGL(labels_kernel, dim3((nb * d.nf + 255) / 256), dim3(256), 0, st, d, v, op, eobj_out, nb, out);
 }
 
 // ------------------------------------------------------------------ init -----

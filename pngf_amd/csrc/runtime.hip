@@ -104,6 +104,8 @@ struct pngf_handle {
     zd *d_partial = nullptr, *d_acc = nullptr, *d_Mc = nullptr, *d_Ar = nullptr, *d_Bt = nullptr, *d_V = nullptr,
        *d_VC = nullptr, *d_CU = nullptr, *d_M = nullptr, *d_rut = nullptr;
     zd* d_lu = nullptr;  // per-block capacitance scratch when gmax > kLdsGmax
+    float* d_feat = nullptr;  // ranker inputs [ngroups][din]
+    float* d_lab = nullptr;   // ranker labels [ngroups][nf]
     int acc_stride = 0;
     size_t small_stride = 0, bt_stride = 0, v_stride = 0;
     int nsplit_items = 2048;
@@ -319,7 +321,7 @@ int group_for(pngf_handle* h, const int32_t* tiles, int nt, int* g) {
 // ------------------------------------------------------------ synthetic G ---
 const double kFref = 30.0e9;
 const double kEpsR = 3.5;
-const double kSoft = 1.0;  // softening radius in cells
+const double kSoftDefault = 1.0;  // softening radius in cells
 
 void synth_positions(const Layout& L, double dx, double dy, std::vector<double>& pos) {
     pos.resize((size_t)L.N * 3);
@@ -337,10 +339,10 @@ double kr_of(double hz) { return 2.0 * kPi * hz * std::sqrt(kEpsR) / kC0; }
 // extra factor f / 30 GHz.  The NF2FF rows (num_eobj = 3) are beta (f/30 GHz)
 // times the x and y orientation of each component, with beta chosen so that the
 // port alone has directivity 1.5 at 30 GHz.
-void synth_norm(const Layout& L, const std::vector<double>& pos, double dx, double dy, double dz, zd& scale,
-                double& beta) {
+void synth_norm(const Layout& L, const std::vector<double>& pos, double dx, double dy, double dz, double soft,
+                zd& scale, double& beta) {
     const double zfac = dx * 0.5 / (2.0 * dy * dz);
-    const double kr = kr_of(kFref), soft2 = (kSoft * dx) * (kSoft * dx);
+    const double kr = kr_of(kFref), soft2 = (soft * dx) * (soft * dx);
     zd sum = zmk(0.0, 0.0);
     for (int p : L.port)
         for (int q : L.port) sum = zadd(sum, synth_entry(&pos[3 * p], &pos[3 * q], kr, soft2));
@@ -615,6 +617,42 @@ int sync_sel(pngf_handle* h, int nb, int* sel, double* F_sel, bool check_info) {
 
 double host_objective(pngf_handle* h, const zd* eobj) { return objective_of(h->op, h->fc.data(), eobj); }
 
+int ranker_din(const pngf_handle* h) { return 51 + 8 * h->d.nf; }
+
+int ensure_feat(pngf_handle* h) {
+    if (h->cfg.method != PNGF_METHOD_LOWRANK) return fail(h, PNGF_ERR_STATE, "ranker inputs need the LOWRANK method");
+    if (!h->d_feat) {
+        CK(dalloc_t(h, &h->d_feat, (size_t)h->d.ngroups * ranker_din(h)));
+        CK(dalloc_t(h, &h->d_lab, (size_t)h->d.ngroups * h->d.nf));
+    }
+    return PNGF_OK;
+}
+
+// Ranker inputs of the groups already uploaded to d_batch (nb of them) into host rows.
+int features_batch(pngf_handle* h, int nb, float* host) {
+    CK(ensure_feat(h));
+    launch_features(h->st, h->d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb, h->L.Tx, h->L.Ty,
+                    h->op, h->d_feat);
+    HC(h, hipGetLastError());
+    HC(h, hipMemcpyAsync(host, h->d_feat, sizeof(float) * (size_t)nb * ranker_din(h), hipMemcpyDeviceToHost, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    return PNGF_OK;
+}
+
+// Appends (inputs, exact |S11_f|^2 changes) of the last scored batch to the training log.
+int log_batch(pngf_handle* h, const pngf_dbs_params* p, int nb, int64_t* logged) {
+    if (!p->log_x || !p->log_y || *logged >= p->log_cap) return PNGF_OK;
+    const int n = (int)std::min<int64_t>(nb, p->log_cap - *logged);
+    const int din = ranker_din(h), nf = h->d.nf;
+    CK(ensure_feat(h));
+    launch_labels(h->st, h->d, h->vecs, h->op, h->d_eobj_out, n, h->d_lab);
+    HC(h, hipMemcpyAsync(p->log_y + (size_t)*logged * nf, h->d_lab, sizeof(float) * (size_t)n * nf,
+                         hipMemcpyDeviceToHost, h->st));
+    CK(features_batch(h, n, p->log_x + (size_t)*logged * din));
+    *logged += n;
+    return PNGF_OK;
+}
+
 int download_eobj(pngf_handle* h, std::vector<zd>& e) {
     e.resize((size_t)h->d.nf * h->d.ne);
     HC(h, hipMemcpyAsync(e.data(), h->vecs.Eobj, e.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
@@ -650,6 +688,7 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     h->cfg = c;
     if (h->cfg.z0 == 0.0) h->cfg.z0 = 50.0;
     if (h->cfg.dx == 0.0) { h->cfg.dx = 0.01 / 60.0; h->cfg.dy = 0.01 / 60.0; h->cfg.dz = 0.01 / 180.0; }
+    if (!(h->cfg.synth_soft > 0.0)) h->cfg.synth_soft = kSoftDefault;
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
     if (h->L.per_tile * c.max_flips > 96) { delete h; return PNGF_ERR_ARG; }
     int ndev = 0;
@@ -685,7 +724,7 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     h->op.pfac = h->cfg.dx * 0.5 * (2.0 * h->cfg.dy * h->cfg.dz) * 0.5;
     h->op.z0 = h->cfg.z0;
     synth_positions(h->L, h->cfg.dx, h->cfg.dy, h->pos);
-    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->synth_scale, h->synth_beta);
+    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->cfg.synth_soft, h->synth_scale, h->synth_beta);
     // small device state
     const size_t ld = h->d.ldN;
     CK(dalloc_t(h, &h->d_tile, h->tile.size()));
@@ -801,7 +840,7 @@ int pngf_synth_g(pngf_handle* h, int32_t f) {
     CK(alloc_gcm(h, f));
     const double hz = h->freqs[f];
     const zd scale = zscale(h->synth_scale, hz / kFref);
-    launch_synth_g(h->st, N, h->d.ldN, h->d_pos, kr_of(hz), scale, kSoft * h->cfg.dx, 0, h->Gcm[f]);
+    launch_synth_g(h->st, N, h->d.ldN, h->d_pos, kr_of(hz), scale, h->cfg.synth_soft * h->cfg.dx, 0, h->Gcm[f]);
     HC(h, hipGetLastError());
     HC(h, hipStreamSynchronize(h->st));
     h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
@@ -816,7 +855,7 @@ int pngf_synth_g(pngf_handle* h, int32_t f) {
 }
 
 int pngf_synth_g_host(int32_t layout, int32_t tx, int32_t ty, int32_t cx, int32_t cy, int32_t ne, double hz,
-                      pngf_c128* file_buf) {
+                      double soft, pngf_c128* file_buf) {
     if ((layout != 0 && layout != 1) || tx < 1 || ty < 1 || cx < 1 || cy < 2 || ne < 1 || ne > 3 || !(hz > 0) ||
         !file_buf)
         return PNGF_ERR_ARG;
@@ -825,10 +864,11 @@ int pngf_synth_g_host(int32_t layout, int32_t tx, int32_t ty, int32_t cx, int32_
     const double dx = 0.01 / 60.0, dy = dx, dz = 0.01 / 180.0;
     std::vector<double> pos;
     synth_positions(L, dx, dy, pos);
+    if (!(soft > 0.0)) soft = kSoftDefault;
     zd scale; double beta;
-    synth_norm(L, pos, dx, dy, dz, scale, beta);
+    synth_norm(L, pos, dx, dy, dz, soft, scale, beta);
     scale = zscale(scale, hz / kFref);
-    const double kr = kr_of(hz), soft2 = (kSoft * dx) * (kSoft * dx);
+    const double kr = kr_of(hz), soft2 = (soft * dx) * (soft * dx);
     const size_t N = L.N, M = N + ne - 1;
     zd* out = (zd*)file_buf;
     for (size_t n = 0; n < M; n++)
@@ -869,7 +909,7 @@ int pngf_set_source_port(pngf_handle* h, int32_t tx, int32_t ty, int32_t yoff, i
         synth_positions(h->L, h->cfg.dx, h->cfg.dy, h->pos);
         HC(h, hipMemcpy(h->d_pos, h->pos.data(), h->pos.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->synth_scale, h->synth_beta);
+    synth_norm(h->L, h->pos, h->cfg.dx, h->cfg.dy, h->cfg.dz, h->cfg.synth_soft, h->synth_scale, h->synth_beta);
     return PNGF_OK;
 }
 
@@ -1175,6 +1215,8 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     if (!h || !p || !r) return fail(h, PNGF_ERR_ARG, "null argument");
     CK(check_ready(h));
     std::memset(r, 0, sizeof(*r));
+    if (p->log_x && h->cfg.method != PNGF_METHOD_LOWRANK)
+        return fail(h, PNGF_ERR_STATE, "the training log needs the LOWRANK method");
     std::vector<zd> e;
     CK(download_eobj(h, e));
     double F_best = host_objective(h, e.data());
@@ -1216,6 +1258,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             int sel; double Fs;
             CK(sync_sel(h, (int)B, &sel, &Fs, true));
             batches++;
+            CK(log_batch(h, p, (int)B, &r->logged));
             if (sel >= 0) {
                 flip_count += sel;
                 if ((rc = record(batch[sel], Fs)) != PNGF_OK) break;
@@ -1265,6 +1308,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             int sel; double Fs;
             CK(sync_sel(h, (int)batch.size(), &sel, &Fs, true));
             batches++;
+            CK(log_batch(h, p, (int)batch.size(), &r->logged));
             const int upto = sel >= 0 ? sel : (int)batch.size() - 1;
             // replay the real hat through the evaluated draws
             int cnt = 0;
@@ -1303,6 +1347,130 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     r->accept_seconds = h->acc_seconds - acc0;
     r->flushes = h->flushes;
     return rc;
+}
+
+int pngf_ranker_dims(pngf_handle* h, int32_t* din, int32_t* dout) {
+    if (!h || !din || !dout) return PNGF_ERR_ARG;
+    *din = ranker_din(h);
+    *dout = h->d.nf;
+    return PNGF_OK;
+}
+
+int pngf_features(pngf_handle* h, const int32_t* tiles, int32_t ncand, float* out) {
+    if (!h || !tiles || !out || ncand < 0) return fail(h, PNGF_ERR_ARG, "bad feature request");
+    CK(check_ready(h));
+    if (h->cfg.driver != PNGF_DRIVER_RASTER) return fail(h, PNGF_ERR_STATE, "ranker inputs need the raster driver");
+    if (ncand > h->ngroups_drv) return fail(h, PNGF_ERR_ARG, "too many candidates");
+    for (int i = 0; i < ncand; i++) {
+        if (tiles[i] < 0 || tiles[i] >= h->ngroups_drv || h->grp.ntiles[tiles[i]] == 0)
+            return fail(h, PNGF_ERR_ARG, "candidate is not a flippable tile");
+        h->h_pin_i[kPinBatch + i] = tiles[i];
+    }
+    if (ncand == 0) return PNGF_OK;
+    HC(h, hipMemcpyAsync(h->d_batch, h->h_pin_i + kPinBatch, ncand * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    return features_batch(h, ncand, out);
+}
+
+int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp, const pngf_dbs_params* p,
+                        pngf_dbs_result* r) {
+    if (!h || !m || !rp || !p || !r || rp->top_k < 1) return fail(h, PNGF_ERR_ARG, "null argument");
+    CK(check_ready(h));
+    if (h->cfg.driver != PNGF_DRIVER_RASTER || h->cfg.objective != PNGF_OBJ_S11 ||
+        h->cfg.method != PNGF_METHOD_LOWRANK)
+        return fail(h, PNGF_ERR_STATE, "ranked DBS needs the LOWRANK method, raster driver and S11 objective");
+    std::memset(r, 0, sizeof(*r));
+    const int din = ranker_din(h), nf = h->d.nf;
+    std::vector<zd> e;
+    CK(download_eobj(h, e));
+    double F_best = host_objective(h, e.data());
+    r->F_start = F_best;
+    std::vector<int> cl;
+    for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
+    const int ncl = (int)cl.size();
+    const int K = std::min(rp->top_k, ncl);
+    std::vector<float> X((size_t)ncl * din), Y((size_t)ncl * nf);
+    std::vector<double> F(ncl), score(ncl);
+    std::vector<int> order(ncl), hits;
+    int64_t flip_count = 0, nacc = 0, scored = 0, sweeps = 0;
+    const int window = std::max(1, rp->window);
+    h->flushes = 0;
+    HC(h, hipStreamSynchronize(h->st));
+    auto t0 = std::chrono::high_resolution_clock::now();
+    // exact scores of `groups` -> F (host); returns index of the best improving or -1
+    auto exact = [&](const std::vector<int>& groups, double* Fout, int* best) -> int {
+        h->h_pin_d[8] = F_best;
+        HC(h, hipMemcpyAsync(h->d_Fbest, h->h_pin_d + 8, sizeof(double), hipMemcpyHostToDevice, h->st));
+        CK(eval_launch(h, groups.data(), (int)groups.size()));
+        int sel; double Fs;
+        CK(sync_sel(h, (int)groups.size(), &sel, &Fs, true));
+        HC(h, hipMemcpy(Fout, h->d_F, sizeof(double) * groups.size(), hipMemcpyDeviceToHost));
+        CK(log_batch(h, p, (int)groups.size(), &r->logged));
+        *best = -1;
+        for (size_t i = 0; i < groups.size(); i++)
+            if (Fout[i] < F_best && (*best < 0 || Fout[i] < Fout[*best])) *best = (int)i;
+        flip_count += (int64_t)groups.size();
+        return PNGF_OK;
+    };
+    auto commit = [&](int g, double Fnew) -> int {
+        if (nacc >= p->max_accepts) return fail(h, PNGF_ERR_ARG, "accepted-flip arrays are full");
+        if (p->acc_tiles) p->acc_tiles[nacc] = h->grp.prim[g];
+        if (p->acc_eval) p->acc_eval[nacc] = flip_count - 1;
+        if (p->acc_F) p->acc_F[nacc] = Fnew;
+        CK(accept_group(h, g));
+        nacc++;
+        F_best = Fnew;
+        return PNGF_OK;
+    };
+    while (true) {
+        if (p->max_evals > 0 && flip_count >= p->max_evals) break;
+        int hsum = 0;
+        const int hn = std::min<int>((int)hits.size(), window);
+        for (int i = 0; i < hn; i++) hsum += hits[hits.size() - 1 - i];
+        const bool fallback = hn == window && hsum < rp->min_hit_rate * window;
+        if (!fallback) {
+            for (int i = 0; i < ncl; i++) h->h_pin_i[kPinBatch + i] = cl[i];
+            HC(h, hipMemcpyAsync(h->d_batch, h->h_pin_i + kPinBatch, ncl * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+            CK(features_batch(h, ncl, X.data()));
+            for (int i = 0; i < ncl; i++)
+                for (int c = 0; c < din; c++) {
+                    float& x = X[(size_t)i * din + c];
+                    x = (x - rp->x_mean[c]) / rp->x_std[c];
+                }
+            int rc = pngf_mlp_predict(m, X.data(), ncl, Y.data());
+            if (rc != PNGF_OK) return fail(h, rc, std::string("ranker: ") + pngf_mlp_last_error(m));
+            for (int i = 0; i < ncl; i++) {
+                double s = 0;
+                for (int f = 0; f < nf; f++) s += Y[(size_t)i * nf + f] * rp->y_std[f] + rp->y_mean[f];
+                score[i] = s / nf;
+                order[i] = i;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return score[a] < score[b]; });
+            scored += ncl;
+            std::vector<int> top(K);
+            for (int i = 0; i < K; i++) top[i] = cl[order[i]];
+            int best;
+            CK(exact(top, F.data(), &best));
+            hits.push_back(best >= 0 ? 1 : 0);
+            if (best >= 0) { CK(commit(top[best], F[best])); continue; }
+        } else {
+            hits.push_back(1);  // a fallback step re-arms the window
+        }
+        int best;
+        CK(exact(cl, F.data(), &best));
+        sweeps++;
+        if (best < 0) { r->converged = 1; break; }
+        CK(commit(cl[best], F[best]));
+    }
+    HC(h, hipStreamSynchronize(h->st));
+    prof_collect(h);
+    r->seconds = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    r->flip_count = flip_count;
+    r->accepts = nacc;
+    r->F_final = F_best;
+    r->ranked_scored = scored;
+    r->full_sweeps = sweeps;
+    r->flushes = h->flushes;
+    return PNGF_OK;
 }
 
 }  // extern "C"

@@ -62,8 +62,8 @@ def test_synth_g_host_structure():
 
 def test_synth_g_host_bad_args():
     buf = np.zeros(4, dtype=np.complex128)
-    assert P.lib().pngf_synth_g_host(3, 3, 3, 3, 3, 1, 30e9, buf.ctypes.data) == -1
-    assert P.lib().pngf_synth_g_host(1, 3, 3, 3, 3, 1, -1.0, buf.ctypes.data) == -1
+    assert P.lib().pngf_synth_g_host(3, 3, 3, 3, 3, 1, 30e9, 0.0, buf.ctypes.data) == -1
+    assert P.lib().pngf_synth_g_host(1, 3, 3, 3, 3, 1, -1.0, 0.0, buf.ctypes.data) == -1
 
 
 def test_create_rejects_bad_config():
