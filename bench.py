@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 import pngf_amd as P  # noqa: E402
-from pngf_amd.problem import WORKLOADS, initial_pattern  # noqa: E402
+from pngf_amd.problem import MULTI16, WORKLOADS, initial_pattern  # noqa: E402
 
 METRIC = "DBS evaluated flips/s, 21×21 tiles × 9 freqs, vs plain FP64 GPU at matched S11"
 PEAK_HBM_GBS = 8000.0
@@ -98,12 +98,45 @@ def cpu_baseline(wl, g0, cinv0, budget_s):
                        % (nf, r["flip_count"], r["accepts"], r["loop_s"], nf))
 
 
+def bench_multi():
+    """16 DBS instances of uneven size (11..25 tiles, 5 frequencies) on one GPU at once.
+    Prints its own JSON line: total committed evaluations / wall seconds of the
+    concurrent run, beside the same instances run one after another."""
+    m = MULTI16
+    opts = []
+    t0 = time.perf_counter()
+    for T in m["sizes"]:
+        for s in m["seeds"]:
+            o = P.Optimizer(m["layout"], T, num_freq=len(m["freqs"]), method=P.METHOD_LOWRANK,
+                            max_deferred=m["max_deferred"])
+            for f, hz in enumerate(m["freqs"]):
+                o.set_frequency(f, hz)
+                o.synth_g(f)
+            o.set_tile_map(initial_pattern(T, s))
+            o.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+            o.init_state()
+            opts.append(o)
+            log("[multi] init %dx%d seed %d (%.1f s)" % (T, T, s, time.perf_counter() - t0))
+    init_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    rs = P.dbs_run_many(opts)
+    conc_s = time.perf_counter() - t1
+    flips = sum(r["flip_count"] for r in rs)
+    out = {"metric": "multi-instance DBS evaluated flips/s (16 instances, 11..25 tiles, 5 freqs)",
+           "value": flips / conc_s, "unit": "flips/s", "n_gpus": 1, "seconds": conc_s, "init_s": init_s,
+           "flip_count": flips, "accepts": sum(r["accepts"] for r in rs),
+           "instances": [dict(T=opts[i].T, flips=r["flip_count"], accepts=r["accepts"], seconds=r["seconds"],
+                              flushes=r["flushes"], F_final=r["F_final"]) for i, r in enumerate(rs)],
+           "sum_instance_seconds": sum(r["seconds"] for r in rs), "dtype": "f64", "data": "synthetic"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="antenna21x21_9f", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="antenna21x21_9f", choices=sorted(WORKLOADS) + ["multi16"])
     ap.add_argument("--method", default="lowrank", choices=["lowrank", "plain"])
     ap.add_argument("--max-deferred", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU comparison run")
@@ -115,6 +148,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "multi16":
+        bench_multi()
+        return
     wl = dict(WORKLOADS[args.workload])
     method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
 

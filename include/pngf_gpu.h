@@ -190,6 +190,12 @@ typedef struct {
    reference loop; candidates are scored in speculative batches against frozen state. */
 int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r);
 
+/* Runs pngf_dbs_run on n independent handles at once (one host thread per handle; each
+   handle's work stays on its own stream, so the device overlaps the instances).  For
+   many DBS instances of uneven size on one GPU (reference: separate processes per run,
+   main.cpp:173-175).  Returns the first non-zero status. */
+int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs);
+
 /* ---- learned flip ranker (BF16 MLP, hand-written HIP forward / backward / Adam) ----
    A din -> dh -> dh -> dout network with GELU, FP32 master weights, BF16 MFMA products
    with FP32 accumulation.  No counterpart in the reference: it ranks candidate flips so

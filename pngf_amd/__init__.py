@@ -33,7 +33,7 @@ EXPORTS = [
     "pngf_profile", "pngf_kernel_stats", "pngf_get_g", "pngf_checkpoint", "pngf_restore",
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
-    "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked",
+    "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
         "pngf_mlp_grad": (I, [P, P, P, I, P, P]),
         "pngf_mlp_train": (I, [P, P, P, I, I, I, ct.c_float, I, P]),
         "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
+        "pngf_dbs_run_many": (I, [P, I, P, P]),
         "pngf_features": (I, [P, P, I, P]),
         "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
                                     ct.POINTER(DbsResult)]),
@@ -414,3 +415,21 @@ class MLP:
             out.append(w[off:off + n].reshape(s))
             off += n
         return out
+
+
+def dbs_run_many(opts, max_evals=0, max_accepts=100000):
+    """DBS on several independent optimizers at once (one host thread and stream each)."""
+    n = len(opts)
+    hs = (ct.c_void_p * n)(*[o.h for o in opts])
+    ps = (DbsParams * n)()
+    rs = (DbsResult * n)()
+    keep = []
+    for i, o in enumerate(opts):
+        p, acc, log = o._dbs_params(max_evals, 0, max_accepts, 0, 0, 0, 0)
+        ps[i] = p
+        keep.append(acc)
+    rc = lib().pngf_dbs_run_many(hs, n, ps, rs)
+    if rc != 0:
+        msgs = [lib().pngf_last_error(o.h).decode() for o in opts]
+        raise PNGFError(rc, "dbs_run_many: " + "; ".join(m for m in msgs if m))
+    return [Optimizer._dbs_result(rs[i], keep[i], None) for i in range(n)]

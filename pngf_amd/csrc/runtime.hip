@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 #include <sys/stat.h>
+#include <thread>
 
 using namespace pngf;
 
@@ -1470,6 +1471,25 @@ int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp,
     r->ranked_scored = scored;
     r->full_sweeps = sweeps;
     r->flushes = h->flushes;
+    return PNGF_OK;
+}
+
+int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs) {
+    if (!hs || !ps || !rs || n < 1) return PNGF_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++)
+            if (!hs[i] || hs[i] == hs[j]) return PNGF_ERR_ARG;  // one thread per handle
+    std::vector<int> rc(n, PNGF_OK);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; i++)
+        th.emplace_back([&, i]() {
+            (void)hipSetDevice(hs[i]->cfg.device);
+            rc[i] = pngf_dbs_run(hs[i], &ps[i], &rs[i]);
+        });
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; i++)
+        if (rc[i] != PNGF_OK) return rc[i];
     return PNGF_OK;
 }
 

@@ -72,3 +72,7 @@ WORKLOADS = {
     # 31x31 tiles, 7 frequencies 27..33 GHz
     "antenna31x31_7f": dict(T=31, freqs=freq_list(7, 27e9, 33e9), layout=1, max_evals=0),
 }
+
+# 16 concurrent DBS instances: 4 seeds at each of 11, 15, 21 and 25 tiles, 5 frequencies
+MULTI16 = dict(sizes=[11, 15, 21, 25], seeds=[0, 1, 2, 3], freqs=freq_list(5, 28e9, 32e9), layout=1,
+               max_deferred=64)

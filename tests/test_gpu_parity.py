@@ -181,3 +181,29 @@ def test_checkpoint_restore_repeats_run():
     b = gpu.dbs_run()
     assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
     assert a["acc_F"].tolist() == b["acc_F"].tolist()
+
+
+def test_dbs_run_many_matches_sequential():
+    """Instances of uneven size run concurrently give the same results as one at a time."""
+    freqs = freq_list(2, 29e9, 31e9)
+    sizes = [5, 7, 9]
+    opts = []
+    for i, T in enumerate(sizes):
+        o = P.Optimizer(P.LAYOUT_DISJOINT, T, num_freq=len(freqs), method=P.METHOD_LOWRANK, synth_soft=SOFT_WELL)
+        for f, hz in enumerate(freqs):
+            o.set_frequency(f, hz)
+            o.synth_g(f)
+        o.set_tile_map(initial_pattern(T, i))
+        o.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+        o.init_state()
+        o.checkpoint()
+        opts.append(o)
+    seq = []
+    for o in opts:
+        seq.append(o.dbs_run())
+        o.restore()
+    conc = P.dbs_run_many(opts)
+    for a, b in zip(seq, conc):
+        assert a["accepts"] > 0
+        assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
+        assert a["acc_F"].tolist() == b["acc_F"].tolist()
