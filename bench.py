@@ -157,7 +157,8 @@ def main():
     log("[bench] init %s method=%s" % (args.workload, args.method))
     opt, init_s = make(wl, method, args.max_deferred)
     opt.checkpoint()
-    log("[bench] init %.2f s, N=%d" % (init_s, opt.N))
+    tune = opt.autotune_result() if method == P.METHOD_LOWRANK else None
+    log("[bench] init %.2f s, N=%d, tile variant %s" % (init_s, opt.N, tune))
     for w in range(args.warmup):
         r = opt.dbs_run(max_evals=wl["max_evals"])
         opt.restore()
@@ -230,6 +231,7 @@ def main():
         "F_final": last["F_final"],
         "accept_seconds_share": None,
         "init_s": init_s,
+        "autotune": tune,
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,

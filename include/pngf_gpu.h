@@ -71,7 +71,7 @@ typedef struct {
     int32_t max_deferred; /* LOWRANK: accepted flips kept as factors before they are folded
                              into W0 (0 = choose from free memory) */
     int32_t device;       /* HIP device ordinal */
-    int32_t reserved;
+    int32_t tile_variant; /* LOWRANK skinny-product tile shape: 0 = autotune at init, v+1 = use v */
     double dx, dy, dz;    /* Yee cell size used by the port impedance (main.cpp:157-160) */
     double z0;            /* reference impedance for S11 (50 ohm, main.cpp:279) */
     int64_t mem_limit;    /* refuse device allocations beyond this many bytes (0 = no limit) */
@@ -251,6 +251,9 @@ typedef struct {
     double flops;          /* summed algorithmic flops */
 } pngf_kstat;
 int pngf_profile(pngf_handle* h, int32_t enable);
+/* Tile shape chosen for the LOWRANK skinny products by the autotuner that pngf_init_state
+   runs on the handle's own buffers; ms[i] = measured time of variant i (may be null). */
+int pngf_autotune_result(pngf_handle* h, int32_t* chosen, int32_t* nvariants, double* ms);
 int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out);
 
 #ifdef __cplusplus

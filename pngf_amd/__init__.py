@@ -34,6 +34,7 @@ EXPORTS = [
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many",
+    "pngf_autotune_result",
 ]
 
 
@@ -48,7 +49,7 @@ class Config(ct.Structure):
                 ("cells_x", ct.c_int32), ("cells_y", ct.c_int32), ("num_eobj", ct.c_int32),
                 ("max_flips", ct.c_int32), ("num_freq", ct.c_int32), ("method", ct.c_int32),
                 ("objective", ct.c_int32), ("driver", ct.c_int32), ("max_deferred", ct.c_int32),
-                ("device", ct.c_int32), ("reserved", ct.c_int32), ("dx", ct.c_double),
+                ("device", ct.c_int32), ("tile_variant", ct.c_int32), ("dx", ct.c_double),
                 ("dy", ct.c_double), ("dz", ct.c_double), ("z0", ct.c_double),
                 ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double)]
 
@@ -127,6 +128,7 @@ def lib():
         "pngf_mlp_train": (I, [P, P, P, I, I, I, ct.c_float, I, P]),
         "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
         "pngf_dbs_run_many": (I, [P, I, P, P]),
+        "pngf_autotune_result": (I, [P, ct.POINTER(I), ct.POINTER(I), P]),
         "pngf_features": (I, [P, P, I, P]),
         "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
                                     ct.POINTER(DbsResult)]),
@@ -166,9 +168,10 @@ class Optimizer:
 
     def __init__(self, layout, tiles, num_freq=1, num_eobj=1, max_flips=1, method=METHOD_LOWRANK,
                  objective=OBJ_S11, driver=DRIVER_RASTER, max_deferred=0, device=0, mem_limit=0,
-                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0):
+                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0):
         self.cfg = Config(layout, tiles, tiles, cells, cells, num_eobj, max_flips, num_freq, method,
-                          objective, driver, max_deferred, device, 0, dx, dy, dz, z0, mem_limit, synth_soft)
+                          objective, driver, max_deferred, device, tile_variant, dx, dy, dz, z0, mem_limit,
+                          synth_soft)
         self.h = ct.c_void_p()
         rc = lib().pngf_create(ct.byref(self.cfg), ct.byref(self.h))
         if rc != 0:
@@ -339,6 +342,12 @@ class Optimizer:
         self._ck(lib().pngf_dbs_run_ranked(self.h, mlp.h, ct.byref(rp), ct.byref(p), ct.byref(r)),
                  "dbs_run_ranked")
         return self._dbs_result(r, acc, log)
+
+    def autotune_result(self):
+        ch, nv = ct.c_int32(), ct.c_int32()
+        ms = np.zeros(16)
+        self._ck(lib().pngf_autotune_result(self.h, ct.byref(ch), ct.byref(nv), _p(ms)), "autotune_result")
+        return dict(chosen=ch.value, ms=ms[:nv.value].tolist())
 
     def profile(self, enable=True):
         self._ck(lib().pngf_profile(self.h, int(enable)), "profile")

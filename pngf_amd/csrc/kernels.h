@@ -30,7 +30,7 @@ struct Dims {
 };
 
 // gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
-constexpr int kLdsGmax = 48;
+constexpr int kLdsGmax = 32;
 
 // ---- eval
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
@@ -52,9 +52,11 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int 
 void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       zd* Mc, zd* Ar, int small_stride);
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Mc, int small_stride);
+                      const zd* Mc, int small_stride, int variant);
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride);
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant);
+// number of tile-shape variants of the skinny products for flips of k components
+int skinny_variants(int k);
 void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
                          size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
                          zd* out_single, size_t out_stride, int out_row0, int ld_out);

@@ -99,46 +99,35 @@ __device__ void lds_backsolve(zd* a, int lda, int k, int col) {
     }
 }
 
-// In-place inverse from the LU factors (zgetri: inv(U), then solve X L = inv(U),
-// then undo the row swaps as column swaps).  tmp: LDS scratch of k entries.
-__device__ void lds_getri(zd* a, int lda, int k, const int* piv, zd* tmp) {
-    const int tid = threadIdx.x, nth = blockDim.x;
-    for (int j = 0; j < k; j++) {
-        if (tid == 0) a[j * lda + j] = zrecip(a[j * lda + j]);
-        __syncthreads();
-        const zd ajj = zmk(-a[j * lda + j].x, -a[j * lda + j].y);
-        for (int i = tid; i < j; i += nth) {
-            zd s = zmk(0.0, 0.0);
-            for (int l = i; l < j; l++) s = zfma(a[i * lda + l], a[l * lda + j], s);
-            tmp[i] = s;
+// Inverse from the LU factors of lds_lu: column c of cap^-1 solves cap x = e_c
+// (row interchanges, unit-lower forward and upper back substitution).  One thread per
+// column, each writing only its own column of X (row-major, leading dimension ldx),
+// so the columns need no synchronisation between them.
+__device__ void lu_inverse_cols(const zd* a, int lda, int k, const int* piv, zd* X, int ldx) {
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        for (int i = 0; i < k; i++) X[i * ldx + c] = zmk(i == c ? 1.0 : 0.0, 0.0);
+        for (int j = 0; j < k; j++) {
+            const int p = piv[j];
+            if (p != j) { zd x = X[j * ldx + c]; X[j * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = x; }
         }
-        __syncthreads();
-        for (int i = tid; i < j; i += nth) a[i * lda + j] = zmul(tmp[i], ajj);
-        __syncthreads();
-    }
-    for (int j = k - 2; j >= 0; j--) {
-        for (int i = j + 1 + tid; i < k; i += nth) { tmp[i] = a[i * lda + j]; a[i * lda + j] = zmk(0.0, 0.0); }
-        __syncthreads();
-        for (int i = tid; i < k; i += nth) {
-            zd s = a[i * lda + j];
-            for (int l = j + 1; l < k; l++) s = zfms(a[i * lda + l], tmp[l], s);
-            a[i * lda + j] = s;
+        for (int j = 0; j < k; j++) {
+            const zd xj = X[j * ldx + c];
+            for (int i = j + 1; i < k; i++) X[i * ldx + c] = zfms(a[i * lda + j], xj, X[i * ldx + c]);
         }
-        __syncthreads();
+        for (int j = k - 1; j >= 0; j--) {
+            const zd xj = zdiv(X[j * ldx + c], a[j * lda + j]);
+            X[j * ldx + c] = xj;
+            for (int i = 0; i < j; i++) X[i * ldx + c] = zfms(a[i * lda + j], xj, X[i * ldx + c]);
+        }
     }
-    for (int j = k - 2; j >= 0; j--) {
-        const int p = piv[j];
-        if (p != j)
-            for (int i = tid; i < k; i += nth) { zd x = a[i * lda + j]; a[i * lda + j] = a[i * lda + p]; a[i * lda + p] = x; }
-        __syncthreads();
-    }
+    __syncthreads();
 }
 
-// The k x (k+1) capacitance | rhs matrix lives in LDS when gmax <= 48 (37 KB);
-// larger flip sets (mirror-symmetric 4-tile flips, k <= 96) use a per-block slot
-// of a global scratch buffer instead.
+// The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
+// gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
+// per-block slot of a global scratch buffer instead.
 static size_t lds_matrix_bytes(int gmax, const zd* gscratch) {
-    return gscratch ? 0 : (size_t)gmax * (gmax + 1) * sizeof(zd);
+    return gscratch ? 0 : (size_t)gmax * (2 * gmax + 1) * sizeof(zd);
 }
 
 // --------------------------------------------------------- LOWRANK eval ------
@@ -158,7 +147,7 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
     int* lp = sg + gmax;
     int* piv = lp + gmax;
     // [k][k+1] capacitance | rhs: LDS when it fits, else this block's global slot
-    zd* a = gscratch ? gscratch + ((size_t)f * gridDim.x + b) * gmax * (gmax + 1) : (zd*)(piv + gmax);
+    zd* a = gscratch ? gscratch + ((size_t)f * gridDim.x + b) * gmax * (2 * gmax + 1) : (zd*)(piv + gmax);
     __shared__ int s_k;
     const int g = batch[b];
     if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
@@ -368,7 +357,7 @@ __global__ void plain_finish_kernel(Dims d, DevVecs v, GroupTab gt, const int32_
     __shared__ int s_k;
     zd* gs = (zd*)(piv + 96);       // [ne]
     zd* tmp = gs + 4;               // [96]
-    zd* a = gscratch ? gscratch + (size_t)item * gmax * (gmax + 1) : tmp + 96;  // [k][k+1]
+    zd* a = gscratch ? gscratch + (size_t)item * gmax * (2 * gmax + 1) : tmp + 96;  // [k][k+1], then [k][k]
     const int g = batch[b];
     if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
     __syncthreads();
@@ -408,9 +397,9 @@ __global__ void plain_finish_kernel(Dims d, DevVecs v, GroupTab gt, const int32_
     if (store) {
         zd* st = store + (size_t)f * acc_stride;
         for (int c = tid; c < k; c += nth) st[2 * k * k + c] = a[c * lda + k];  // y
-        __syncthreads();
-        lds_getri(a, lda, k, piv, tmp);
-        for (int e = tid; e < k * k; e += nth) { const int r = e / k, c = e - r * k; st[e] = a[r * lda + c]; }
+        zd* X = a + k * lda;
+        lu_inverse_cols(a, lda, k, piv, X, k);
+        for (int e = tid; e < k * k; e += nth) st[e] = X[e];
     }
 }
 
@@ -482,7 +471,7 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     int* piv = (int*)smem;
     zd* dpe = (zd*)(piv + 100);
     zd* tmp = dpe + 96;
-    zd* a = gscratch ? gscratch + (size_t)f * gmax * (gmax + 1) : tmp + 96;
+    zd* a = gscratch ? gscratch + (size_t)f * gmax * (2 * gmax + 1) : tmp + 96;
     const int lda = k + 1;
     const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
     const zd* q = v.q + (size_t)f * d.ldN;
@@ -522,8 +511,9 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         st[2 * k * k + 2 * k + r] = s;
         v.Eobj[f * ne + r] = s;
     }
-    lds_getri(a, lda, k, piv, tmp);
-    for (int e = tid; e < k * k; e += nth) { const int r = e / k, c = e - r * k; st[e] = a[r * lda + c]; }
+    zd* X = a + k * lda;
+    lu_inverse_cols(a, lda, k, piv, X, k);
+    for (int e = tid; e < k * k; e += nth) st[e] = X[e];
     if (tid == 0 && info_out) info_out[f] = info;
 }
 
@@ -576,29 +566,51 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
 // (each lane group holds every fourth row).
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <int CT, bool ROW>
-__global__ void __launch_bounds__(512) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
-                                                             int K, const zd* __restrict__ small, int small_stride,
-                                                             zd* Bt, size_t bt_stride) {
-    constexpr int KC = CT <= 3 ? 64 : 16;  // complex inner entries per LDS stage
-    constexpr int BW = CT * 8;             // complex columns held (>= k)
+// Tile shape: RT 16-row tiles per wave (B fragments reused RT times), KC complex inner
+// entries staged in LDS per barrier pair, WAVES waves per workgroup sharing them.  The A
+// entries of the next 8-entry sub-chunk are loaded while the current one is multiplied.
+template <int CT, bool ROW, int RT, int KC, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
+                                                                    int k, int K, const zd* __restrict__ small,
+                                                                    int small_stride, zd* Bt, size_t bt_stride) {
+    constexpr int BW = CT * 8;  // complex columns held (>= k)
+    constexpr int NT = WAVES * 64;
     __shared__ zd Bs[KC][BW + 1];
     const int f = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    const int n0 = blockIdx.x * 128 + wave * 16;
-    const int n = n0 + lr;
-    const int nc = n < d.N ? n : d.N - 1;  // clamped row for loads
+    const int n0 = (blockIdx.x * WAVES + wave) * 16 * RT;
+    int nc[RT];
+    bool nv[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+        const int n = n0 + 16 * r + lr;
+        nv[r] = n < d.N;
+        nc[r] = nv[r] ? n : d.N - 1;  // clamped row for loads
+    }
     const zd* A = tb.Aall[f];
     const zd* M = tb.Mall[f];
     const zd* W0 = tb.W0[f];
     const zd* sm = small + (size_t)f * small_stride;
-    d4v acc[CT];
+    d4v acc[RT][CT];
 #pragma unroll
-    for (int c = 0; c < CT; c++) acc[c] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[r][c] = d4v{0.0, 0.0, 0.0, 0.0};
+    auto loadA = [&](int kbase, zd (&a)[RT][2]) {
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int Kp = kbase + 2 * lq + j;
+                zd v = zmk(0.0, 0.0);
+                if (Kp < K && nv[r]) v = ROW ? M[(size_t)Kp * d.ldN + nc[r]] : A[(size_t)nc[r] * d.Kmax + Kp];
+                a[r][j] = v;
+            }
+    };
     for (int k0 = 0; k0 < K; k0 += KC) {
         __syncthreads();
-        for (int e = threadIdx.x; e < KC * BW; e += 512) {
+        for (int e = threadIdx.x; e < KC * BW; e += NT) {
             const int kk = e / BW, c = e - kk * BW;
             zd v = zmk(0.0, 0.0);
             if (k0 + kk < K && c < k) v = ROW ? sm[(size_t)c * K + k0 + kk] : sm[(size_t)(k0 + kk) * k + c];
@@ -606,15 +618,10 @@ __global__ void __launch_bounds__(512) lr_skinny_mfma_kernel(Dims d, DevTabs tb,
         }
         __syncthreads();
         const int kend = min(KC, K - k0);
+        zd a[RT][2], an[RT][2];
+        loadA(k0, a);
         for (int kk0 = 0; kk0 < kend; kk0 += 8) {
-            zd a[2];
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int Kp = k0 + kk0 + 2 * lq + j;
-                zd v = zmk(0.0, 0.0);
-                if (Kp < K && n < d.N) v = ROW ? M[(size_t)Kp * d.ldN + nc] : A[(size_t)nc * d.Kmax + Kp];
-                a[j] = v;
-            }
+            if (kk0 + 8 < kend) loadA(k0 + kk0 + 8, an);
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 zd m[CT];
@@ -622,55 +629,85 @@ __global__ void __launch_bounds__(512) lr_skinny_mfma_kernel(Dims d, DevTabs tb,
                 for (int c = 0; c < CT; c++) m[c] = Bs[kk0 + 2 * lq + j][(c * 16 + lr) >> 1];
 #pragma unroll
                 for (int p = 0; p < 2; p++) {
-                    const double av = p ? a[j].y : a[j].x;
+                    const bool odd = lr & 1;
 #pragma unroll
                     for (int c = 0; c < CT; c++) {
-                        const bool odd = lr & 1;
                         const double bv = p == 0 ? (odd ? m[c].y : m[c].x) : (odd ? m[c].x : -m[c].y);
-                        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
+#pragma unroll
+                        for (int r = 0; r < RT; r++) {
+                            const double av = p ? a[r][j].y : a[r][j].x;
+                            acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[r][c], 0, 0, 0);
+                        }
                     }
                 }
             }
+#pragma unroll
+            for (int r = 0; r < RT; r++) { a[r][0] = an[r][0]; a[r][1] = an[r][1]; }
         }
     }
 #pragma unroll
-    for (int c = 0; c < CT; c++) {
-        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
-        if (cc >= k) continue;
+    for (int r = 0; r < RT; r++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = n0 + lq + 4 * v;
-            if (row >= d.N) continue;
-            if (!ROW) {
-                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
-                ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - acc[c][v];
-            } else {
-                const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
-                ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - acc[c][v];
+        for (int c = 0; c < CT; c++) {
+            const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+            if (cc >= k) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = n0 + 16 * r + lq + 4 * v;
+                if (row >= d.N) continue;
+                if (!ROW) {
+                    const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
+                    ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - acc[r][c][v];
+                } else {
+                    const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
+                    ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - acc[r][c][v];
+                }
             }
         }
-    }
+}
+
+// Tile-shape variants offered to the autotuner, per column-tile count CT.
+template <int CT, bool ROW, int RT, int KC, int WAVES>
+static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
+                           const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
+    const int rows = 16 * RT * WAVES;
+    dim3 grid((d.N + rows - 1) / rows, d.nf);
+    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, RT, KC, WAVES>), grid, dim3(WAVES * 64), 0, st, d, tb, t, k, K,
+                       small, small_stride, Bt, bt_stride);
 }
 
 template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                               const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
-    dim3 grid((d.N + 127) / 128, d.nf);
-    if (k <= 24)
-        hipLaunchKernelGGL((lr_skinny_mfma_kernel<3, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
-    else if (k <= 48)
-        hipLaunchKernelGGL((lr_skinny_mfma_kernel<6, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
-    else
-        hipLaunchKernelGGL((lr_skinny_mfma_kernel<12, ROW>), grid, dim3(512), 0, st, d, tb, t, k, K, small, small_stride, Bt, bt_stride);
+                               const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant) {
+#define PNGF_SKV(CT, RT, KC, W) launch_variant<CT, ROW, RT, KC, W>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride)
+    if (k <= 24) {
+        switch (variant) {
+            case 1: PNGF_SKV(3, 2, 64, 8); break;
+            case 2: PNGF_SKV(3, 2, 64, 4); break;
+            case 3: PNGF_SKV(3, 1, 128, 8); break;
+            case 4: PNGF_SKV(3, 4, 32, 4); break;
+            case 5: PNGF_SKV(3, 2, 128, 4); break;
+            default: PNGF_SKV(3, 1, 64, 8); break;
+        }
+    } else if (k <= 48) {
+        if (variant % 2) PNGF_SKV(6, 2, 32, 4);
+        else PNGF_SKV(6, 1, 32, 8);
+    } else {
+        if (variant % 2) PNGF_SKV(12, 2, 16, 4);
+        else PNGF_SKV(12, 1, 16, 8);
+    }
+#undef PNGF_SKV
 }
 
+int skinny_variants(int k) { return k <= 24 ? 6 : 2; }
+
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Mc, int small_stride) {
-    launch_skinny_mfma<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0);
+                      const zd* Mc, int small_stride, int variant) {
+    launch_skinny_mfma<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0, variant);
 }
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride) {
-    launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride);
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant) {
+    launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant);
 }
 
 // out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb)

@@ -112,6 +112,8 @@ struct pngf_handle {
     int nsplit_items = 2048;
     size_t partial_elems = 0;
     int K = 0, Kmax = 0, flushes = 0;
+    int skv = 0;                     // tile-shape variant of the skinny products (autotuned)
+    std::vector<double> tune_ms;     // measured time per variant (colblk + rowblk), ms
     // pinned host staging
     int32_t* h_pin_i = nullptr;   // batch ids / small ints
     double* h_pin_d = nullptr;
@@ -459,6 +461,47 @@ int init_freq(pngf_handle* h, int f) {
     return PNGF_OK;
 }
 
+// ------------------------------------------------------------ autotuner ---
+// Times every tile-shape variant of the two skinny products on this handle's own
+// buffers (N, frequencies, a representative factor depth K) and keeps the fastest.
+// The factor buffers are zero at this point, so the timed products read finite data;
+// the columns they write are rewritten by the first accepts that reach them.
+int autotune_skinny(pngf_handle* h) {
+    const Dims& d = h->d;
+    int g0 = -1;
+    for (int g = 0; g < h->ngroups_drv; g++)
+        if (h->grp.ntiles[g] > 0) { g0 = g; break; }
+    if (g0 < 0) return PNGF_OK;
+    const int k = h->grp.ncomp[g0];
+    const int K = std::max(0, std::min(h->Kmax - k, 64 * k));
+    for (int i = 0; i < k; i++) h->h_pin_i[i] = h->grp.comps[(size_t)g0 * d.gmax + i];
+    HC(h, hipMemcpy(h->d_small, h->h_pin_i, k * sizeof(int32_t), hipMemcpyHostToDevice));
+    const int nv = skinny_variants(k);
+    hipEvent_t e0, e1;
+    HC(h, hipEventCreate(&e0));
+    HC(h, hipEventCreate(&e1));
+    h->tune_ms.assign(nv, 0.0);
+    double best = 1e30;
+    for (int v = 0; v < nv; v++) {
+        for (int rep = 0; rep < 3; rep++) {
+            if (rep == 1) HC(h, hipEventRecord(e0, h->st));
+            launch_lr_colblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Mc, (int)h->small_stride, v);
+            launch_lr_rowblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt,
+                             h->bt_stride, v);
+        }
+        HC(h, hipEventRecord(e1, h->st));
+        HC(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HC(h, hipEventElapsedTime(&ms, e0, e1));
+        h->tune_ms[v] = ms / 2.0;
+        if (ms < best) { best = ms; h->skv = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HC(h, hipGetLastError());
+    return PNGF_OK;
+}
+
 // ------------------------------------------------------------------ eval ---
 int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
     hipStream_t st = h->st;
@@ -534,11 +577,11 @@ int accept_group(pngf_handle* h, int g) {
         const double N = d.N, kd = k, Kd = K;
         Timed t1{};
         prof_begin(h, t1);
-        launch_lr_colblk(st, d, h->tabs, dt, k, K, h->d_Mc, (int)h->small_stride);
+        launch_lr_colblk(st, d, h->tabs, dt, k, K, h->d_Mc, (int)h->small_stride, h->skv);
         prof_end(h, t1, PNGF_K_COLBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
         Timed t2{};
         prof_begin(h, t2);
-        launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride);
+        launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv);
         prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
         Timed t3{};
         prof_begin(h, t3);
@@ -691,7 +734,7 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     if (h->cfg.dx == 0.0) { h->cfg.dx = 0.01 / 60.0; h->cfg.dy = 0.01 / 60.0; h->cfg.dz = 0.01 / 180.0; }
     if (!(h->cfg.synth_soft > 0.0)) h->cfg.synth_soft = kSoftDefault;
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
-    if (h->L.per_tile * c.max_flips > 96) { delete h; return PNGF_ERR_ARG; }
+    if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6) { delete h; return PNGF_ERR_ARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) {
         (void)hipGetLastError();
@@ -946,7 +989,7 @@ int pngf_init_state(pngf_handle* h) {
         return fail(h, PNGF_ERR_ARG, "design too large for the staging buffers");
     h->acc_stride = 2 * gmax * gmax + 2 * gmax + 4 + ne * gmax;
     if (gmax > kLdsGmax)
-        CK(dalloc_t(h, &h->d_lu, (size_t)std::max(G.n, 1) * nf * gmax * (gmax + 1)));
+        CK(dalloc_t(h, &h->d_lu, (size_t)std::max(G.n, 1) * nf * gmax * (2 * gmax + 1)));
     CK(dalloc_t(h, &h->d_acc, (size_t)nf * h->acc_stride));
     if (h->cfg.method == PNGF_METHOD_LOWRANK)
         CK(dalloc_t(h, &h->vecs.Dblk, (size_t)nf * G.n * gmax * gmax));
@@ -978,6 +1021,12 @@ int pngf_init_state(pngf_handle* h) {
         CK(dalloc_t(h, &h->d_Ar, nf * h->small_stride));
         h->bt_stride = (size_t)N * gmax;
         CK(dalloc_t(h, &h->d_Bt, nf * h->bt_stride));
+        for (int f = 0; f < nf; f++) {
+            HC(h, hipMemset(h->Aall[f], 0, sizeof(zd) * (size_t)N * h->Kmax));
+            HC(h, hipMemset(h->Mall[f], 0, sizeof(zd) * (size_t)h->Kmax * ld));
+        }
+        HC(h, hipMemset(h->d_Mc, 0, sizeof(zd) * nf * h->small_stride));
+        HC(h, hipMemset(h->d_Ar, 0, sizeof(zd) * nf * h->small_stride));
     } else {
         h->d.Kmax = 0;
         h->v_stride = (size_t)gmax * ld;
@@ -991,6 +1040,10 @@ int pngf_init_state(pngf_handle* h) {
         CK(dalloc_t(h, &h->d_partial, h->partial_elems));
     }
     CK(upload_tabs(h));
+    if (h->cfg.method == PNGF_METHOD_LOWRANK) {
+        if (h->cfg.tile_variant > 0) h->skv = h->cfg.tile_variant - 1;
+        else CK(autotune_skinny(h));
+    }
     std::vector<zd> e;
     CK(download_eobj(h, e));
     h->inited = true;
@@ -1222,9 +1275,10 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     CK(download_eobj(h, e));
     double F_best = host_objective(h, e.data());
     r->F_start = F_best;
-    const int bmin = p->batch_min > 0 ? p->batch_min : 4;
-    const int bmax_cfg = p->batch_max > 0 ? p->batch_max : (1 << 30);
     const bool plain = h->cfg.method == PNGF_METHOD_PLAIN;
+    // speculative batch after an accept: PLAIN scores are costly (4), LOWRANK ones cheap (32)
+    const int bmin = p->batch_min > 0 ? p->batch_min : (plain ? 4 : 32);
+    const int bmax_cfg = p->batch_max > 0 ? p->batch_max : (1 << 30);
     int64_t flip_count = 0, nacc = 0, spec = 0, batches = 0;
     const double acc0 = h->acc_seconds;
     h->flushes = 0;
@@ -1250,7 +1304,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             if (p->max_evals > 0 && flip_count >= p->max_evals) break;
             int64_t B = ncl - nonimp;
             if (p->max_evals > 0) B = std::min<int64_t>(B, p->max_evals - flip_count);
-            if (plain) B = std::min<int64_t>(B, std::min(bs, bmax_cfg));
+            B = std::min<int64_t>(B, std::min(bs, bmax_cfg));
             batch.resize(B);
             for (int64_t i = 0; i < B; i++) batch[i] = cl[(pos + i) % ncl];
             h->h_pin_d[8] = F_best;
@@ -1293,7 +1347,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             int lt = last_tried;
             batch.clear();
             int64_t cap = p->max_evals > 0 ? p->max_evals - flip_count : (int64_t)1 << 40;
-            if (plain) cap = std::min<int64_t>(cap, std::min(bs, bmax_cfg));
+            cap = std::min<int64_t>(cap, std::min(bs, bmax_cfg));
             while ((int64_t)batch.size() < cap) {
                 int rr = sim.draw();
                 if (rr < 0) break;
@@ -1490,6 +1544,14 @@ int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pn
     for (auto& t : th) t.join();
     for (int i = 0; i < n; i++)
         if (rc[i] != PNGF_OK) return rc[i];
+    return PNGF_OK;
+}
+
+int pngf_autotune_result(pngf_handle* h, int32_t* chosen, int32_t* nvariants, double* ms) {
+    if (!h || !chosen || !nvariants) return PNGF_ERR_ARG;
+    *chosen = h->skv;
+    *nvariants = (int32_t)h->tune_ms.size();
+    if (ms) for (size_t i = 0; i < h->tune_ms.size(); i++) ms[i] = h->tune_ms[i];
     return PNGF_OK;
 }
 

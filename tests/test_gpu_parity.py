@@ -31,13 +31,13 @@ def rel(a, b):
 
 
 def setup_pair(layout, T, freqs, method, ne=1, driver=P.DRIVER_RASTER, objective=P.OBJ_S11, max_flips=1,
-               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL):
+               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL, tile_variant=0):
     Gs = [P.synth_g_host(layout, T, ne, hz, soft=soft) for hz in freqs]
     if tm is None:
         tm = initial_pattern(T, seed)
     mid = (T - 1) // 2
     gpu = P.Optimizer(layout, T, num_freq=len(freqs), num_eobj=ne, max_flips=max_flips, method=method,
-                      objective=objective, driver=driver, max_deferred=max_deferred)
+                      objective=objective, driver=driver, max_deferred=max_deferred, tile_variant=tile_variant)
     ors = []
     for f, (hz, G) in enumerate(zip(freqs, Gs)):
         gpu.set_frequency(f, hz)
@@ -207,3 +207,22 @@ def test_dbs_run_many_matches_sequential():
         assert a["accepts"] > 0
         assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
         assert a["acc_F"].tolist() == b["acc_F"].tolist()
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_every_tile_variant_gives_the_same_run(variant):
+    """Each tile shape the autotuner may pick reproduces the oracle's DBS (k = 24)."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, tile_variant=variant + 1)
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
+def test_autotune_reports_variants():
+    T = 7
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, [30e9], P.METHOD_LOWRANK)
+    r = gpu.autotune_result()
+    assert len(r["ms"]) == 6 and 0 <= r["chosen"] < 6
+    assert r["ms"][r["chosen"]] == min(r["ms"])
