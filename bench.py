@@ -174,6 +174,13 @@ def main():
             s, r["flip_count"], r["accepts"], r["seconds"], r["F_start"], r["F_final"]))
     value = flips / secs
     last = runs[-1]
+    # the same start with the best-improvement accept rule (a different accepted sequence)
+    opt.restore()
+    rb = opt.dbs_run(max_evals=wl["max_evals"], order=P.ORDER_BEST)
+    best = dict(value=rb["flip_count"] / rb["seconds"], seconds=rb["seconds"], flip_count=rb["flip_count"],
+                accepts=rb["accepts"], F_final=rb["F_final"])
+    log("[bench] best-improvement: %d evals %d accepts %.3f s F %.6g" % (rb["flip_count"], rb["accepts"],
+                                                                          rb["seconds"], rb["F_final"]))
     roof, allk = None, {}
     if not args.no_profile:
         opt.restore()
@@ -236,6 +243,9 @@ def main():
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "best_improvement_variant": best,
+        "time_to_solution_s": {"first_improvement": last["seconds"], "best_improvement": rb["seconds"],
+                               "plain_fp64_gpu": plain["seconds"] if plain else None},
         "kernels": allk,
     }
     print(json.dumps(out))

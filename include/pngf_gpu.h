@@ -51,6 +51,11 @@ typedef struct { double re, im; } pngf_c128;
 #define PNGF_METHOD_LOWRANK 1 /* state W = (G-I) C^-1 kept as W0 - A M (deferred rank-k factors);
                                  candidates scored from k x k blocks of W in O(k^3) */
 
+/* accept rule inside the raster driver */
+#define PNGF_ORDER_FIRST 0 /* first improving flip in visit order (the reference loop) */
+#define PNGF_ORDER_BEST  1 /* best improving flip of a full exact sweep (steepest descent); a
+                              different accepted sequence, reported beside the reference one */
+
 /* DBS visit order */
 #define PNGF_DRIVER_REF    0 /* MagicHat order, mirror-symmetric 2/4-tile flips (main.cpp:338-459) */
 #define PNGF_DRIVER_RASTER 1 /* raster order, single-tile flips, cyclic until one full cycle
@@ -161,7 +166,7 @@ typedef struct {
     int32_t rng_discard;   /* PNGF_DRIVER_REF: outputs of the hat's mt19937 already consumed before
                               the loop (main.cpp:195 draws (T/2+1)^2 bits for INIT_RANDOM; 0 for
                               the default INIT_EMPTY) */
-    int32_t reserved;
+    int32_t order;         /* PNGF_ORDER_FIRST (reference semantics, main.cpp:429) or PNGF_ORDER_BEST */
     /* Optional training log (LOWRANK): for every scored candidate, its ranker inputs
        (pngf_ranker_dims din floats) and the exact change of |S11_f|^2 at every frequency
        (dout floats), appended until log_cap rows. */

@@ -1294,7 +1294,35 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     int rc = PNGF_OK;
     std::vector<int> batch;
     int bs = bmin;
-    if (h->cfg.driver == PNGF_DRIVER_RASTER) {
+    if (h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_BEST) {
+        // steepest descent: every step scores all candidates exactly and commits the best
+        std::vector<int> cl;
+        for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
+        const int ncl = (int)cl.size();
+        std::vector<double> F(ncl);
+        while (true) {
+            if (p->max_evals > 0 && flip_count + ncl > p->max_evals) break;
+            h->h_pin_d[8] = F_best;
+            HC(h, hipMemcpyAsync(h->d_Fbest, h->h_pin_d + 8, sizeof(double), hipMemcpyHostToDevice, h->st));
+            CK(eval_launch(h, cl.data(), ncl));
+            int sel; double Fs;
+            CK(sync_sel(h, ncl, &sel, &Fs, true));
+            batches++;
+            CK(log_batch(h, p, ncl, &r->logged));
+            HC(h, hipMemcpy(F.data(), h->d_F, sizeof(double) * ncl, hipMemcpyDeviceToHost));
+            flip_count += ncl;
+            int best = -1;
+            for (int i = 0; i < ncl; i++)
+                if (F[i] < F_best && (best < 0 || F[i] < F[best])) best = i;
+            if (best < 0) { r->converged = 1; break; }
+            flip_count -= 1;
+            if ((rc = record(cl[best], F[best])) != PNGF_OK) break;
+            flip_count += 1;
+            CK(accept_group(h, cl[best]));
+            nacc++;
+            F_best = F[best];
+        }
+    } else if (h->cfg.driver == PNGF_DRIVER_RASTER) {
         std::vector<int> cl;
         for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
         const int ncl = (int)cl.size();
