@@ -34,9 +34,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(wl, method, max_deferred=0):
+def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO):
     T, freqs = wl["T"], wl["freqs"]
-    opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred)
+    opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred,
+                      symmetry=symmetry)
     for f, hz in enumerate(freqs):
         opt.set_frequency(f, hz)
         opt.synth_g(f)
@@ -143,6 +144,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
+                    help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
@@ -155,10 +158,12 @@ def main():
     method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
 
     log("[bench] init %s method=%s" % (args.workload, args.method))
-    opt, init_s = make(wl, method, args.max_deferred)
+    opt, init_s = make(wl, method, args.max_deferred, P.SYM_AUTO if args.symmetry == "auto" else P.SYM_OFF)
     opt.checkpoint()
     tune = opt.autotune_result() if method == P.METHOD_LOWRANK else None
-    log("[bench] init %.2f s, N=%d, tile variant %s" % (init_s, opt.N, tune))
+    sym_used, sym_asym = opt.symmetry()
+    log("[bench] init %.2f s, N=%d, tile variant %s, symmetric accept %s (asym %.3g)" % (
+        init_s, opt.N, tune, sym_used, sym_asym))
     for w in range(args.warmup):
         r = opt.dbs_run(max_evals=wl["max_evals"])
         opt.restore()
@@ -181,12 +186,13 @@ def main():
                 accepts=rb["accepts"], F_final=rb["F_final"])
     log("[bench] best-improvement: %d evals %d accepts %.3f s F %.6g" % (rb["flip_count"], rb["accepts"],
                                                                           rb["seconds"], rb["F_final"]))
-    roof, allk = None, {}
+    roof, allk, acc_share = None, {}, None
     if not args.no_profile:
         opt.restore()
         opt.profile(True)
         rp = opt.dbs_run(max_evals=wl["max_evals"])
         roof, allk = roofline(opt)
+        acc_share = rp["accept_seconds"] / rp["seconds"]
         opt.profile(False)
         log("[bench] profiled run %.3f s; dominant %s" % (rp["seconds"], roof and roof["kernel"]))
     opt.close()
@@ -231,18 +237,20 @@ def main():
         "config": {"workload": args.workload, "tiles": wl["T"], "N": P.layout_size(wl["layout"], wl["T"]),
                    "freqs": len(wl["freqs"]), "layout": "disjoint" if wl["layout"] == 1 else "shared",
                    "method": args.method, "objective": "mean |S11|^2", "driver": "raster",
-                   "max_evals": wl["max_evals"]},
+                   "max_evals": wl["max_evals"], "symmetry": args.symmetry},
         "flip_count": last["flip_count"],
         "accepts": last["accepts"],
         "F_start": last["F_start"],
         "F_final": last["F_final"],
-        "accept_seconds_share": None,
+        "accept_seconds_share": acc_share,
+        "symmetric_accept": {"used": sym_used, "rel_asym": sym_asym},
         "init_s": init_s,
         "autotune": tune,
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "F_final_vs_plain": (last["F_final"] / plain["F_final"]) if plain else None,
         "best_improvement_variant": best,
         "time_to_solution_s": {"first_improvement": last["seconds"], "best_improvement": rb["seconds"],
                                "plain_fp64_gpu": plain["seconds"] if plain else None},

@@ -56,6 +56,12 @@ typedef struct { double re, im; } pngf_c128;
 #define PNGF_ORDER_BEST  1 /* best improving flip of a full exact sweep (steepest descent); a
                               different accepted sequence, reported beside the reference one */
 
+/* LOWRANK accept: for a complex-symmetric G (reciprocal medium) W = (G-I)(I+P(G-I))^-1 is
+   symmetric too, so the row strip W[t,:] that an accept needs is the transposed column strip
+   and one of the two N x K x k products is skipped */
+#define PNGF_SYM_AUTO 0 /* use it when every G is symmetric to 1e-14 of its largest entry */
+#define PNGF_SYM_OFF  1 /* always form both strips */
+
 /* DBS visit order */
 #define PNGF_DRIVER_REF    0 /* MagicHat order, mirror-symmetric 2/4-tile flips (main.cpp:338-459) */
 #define PNGF_DRIVER_RASTER 1 /* raster order, single-tile flips, cyclic until one full cycle
@@ -82,6 +88,8 @@ typedef struct {
     int64_t mem_limit;    /* refuse device allocations beyond this many bytes (0 = no limit) */
     double synth_soft;    /* synthetic G only: softening radius in cells (0 = 1.0); smaller values
                              give better-conditioned C */
+    int32_t symmetry;     /* LOWRANK: PNGF_SYM_AUTO or PNGF_SYM_OFF */
+    int32_t reserved;
 } pngf_config;
 
 /* Creates a handle; selects the device and allocates nothing large yet.
@@ -260,6 +268,10 @@ int pngf_profile(pngf_handle* h, int32_t enable);
    runs on the handle's own buffers; ms[i] = measured time of variant i (may be null). */
 int pngf_autotune_result(pngf_handle* h, int32_t* chosen, int32_t* nvariants, double* ms);
 int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out);
+/* After pngf_init_state: *used = 1 if the LOWRANK accept uses W = W^T (PNGF_SYM_AUTO and G
+   symmetric), *rel_asym = max over frequencies of max|G_ij - G_ji| / max|G_ij| (|z| =
+   |Re z| + |Im z|).  Either pointer may be null. */
+int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ OBJ_REF, OBJ_S11 = 0, 1
 METHOD_PLAIN, METHOD_LOWRANK = 0, 1
 DRIVER_REF, DRIVER_RASTER = 0, 1
 ORDER_FIRST, ORDER_BEST = 0, 1
+SYM_AUTO, SYM_OFF = 0, 1
 K_EVAL, K_COLBLK, K_ROWBLK, K_ACC_SMALL, K_VCINV, K_RANKK = range(6)
 K_NAMES = ["eval", "colblk", "rowblk", "acc_small", "vcinv", "rankk"]
 
@@ -35,7 +36,7 @@ EXPORTS = [
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many",
-    "pngf_autotune_result",
+    "pngf_autotune_result", "pngf_symmetry",
 ]
 
 
@@ -52,7 +53,8 @@ class Config(ct.Structure):
                 ("objective", ct.c_int32), ("driver", ct.c_int32), ("max_deferred", ct.c_int32),
                 ("device", ct.c_int32), ("tile_variant", ct.c_int32), ("dx", ct.c_double),
                 ("dy", ct.c_double), ("dz", ct.c_double), ("z0", ct.c_double),
-                ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double)]
+                ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double), ("symmetry", ct.c_int32),
+                ("reserved", ct.c_int32)]
 
 
 class DbsParams(ct.Structure):
@@ -130,6 +132,7 @@ def lib():
         "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
         "pngf_dbs_run_many": (I, [P, I, P, P]),
         "pngf_autotune_result": (I, [P, ct.POINTER(I), ct.POINTER(I), P]),
+        "pngf_symmetry": (I, [P, ct.POINTER(I), ct.POINTER(ct.c_double)]),
         "pngf_features": (I, [P, P, I, P]),
         "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
                                     ct.POINTER(DbsResult)]),
@@ -169,10 +172,10 @@ class Optimizer:
 
     def __init__(self, layout, tiles, num_freq=1, num_eobj=1, max_flips=1, method=METHOD_LOWRANK,
                  objective=OBJ_S11, driver=DRIVER_RASTER, max_deferred=0, device=0, mem_limit=0,
-                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0):
+                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0, symmetry=0):
         self.cfg = Config(layout, tiles, tiles, cells, cells, num_eobj, max_flips, num_freq, method,
                           objective, driver, max_deferred, device, tile_variant, dx, dy, dz, z0, mem_limit,
-                          synth_soft)
+                          synth_soft, symmetry, 0)
         self.h = ct.c_void_p()
         rc = lib().pngf_create(ct.byref(self.cfg), ct.byref(self.h))
         if rc != 0:
@@ -350,6 +353,13 @@ class Optimizer:
         ms = np.zeros(16)
         self._ck(lib().pngf_autotune_result(self.h, ct.byref(ch), ct.byref(nv), _p(ms)), "autotune_result")
         return dict(chosen=ch.value, ms=ms[:nv.value].tolist())
+
+    def symmetry(self):
+        """(used, rel_asym): whether the LOWRANK accept takes W[t,:] = W[:,t]^T, and the measured
+        max|G_ij - G_ji| / max|G_ij| over the frequencies."""
+        u, a = ct.c_int32(), ct.c_double()
+        self._ck(lib().pngf_symmetry(self.h, ct.byref(u), ct.byref(a)), "symmetry")
+        return bool(u.value), a.value
 
     def profile(self, enable=True):
         self._ck(lib().pngf_profile(self.h, int(enable)), "profile")

@@ -59,7 +59,8 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const in
 int skinny_variants(int k);
 void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
                          size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
-                         zd* out_single, size_t out_stride, int out_row0, int ld_out);
+                         zd* out_single, size_t out_stride, int out_row0, int ld_out,
+                         const zd* const* b_tab = nullptr, int b_col0 = 0);
 void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
                      int acc_stride, int k, int K);
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
@@ -89,6 +90,7 @@ void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos /*[N][3]*/
 void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* Gcm, zd* C);
 void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v);
 void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out);
+void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out);
 void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
                       zd* einc);
 void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe);

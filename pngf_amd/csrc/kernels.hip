@@ -525,17 +525,18 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int 
                        acc_stride, info, gscratch);
 }
 
-// Mc[K'][c] = M_all[K'][t_c] (K x k) and Ar[c][K'] = A_all[t_c][K'] (k x K).
+// Mc[K'][c] = M_all[K'][t_c] (K x k) and, unless Ar is null, Ar[c][K'] = A_all[t_c][K'] (k x K).
 __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, int K, zd* Mc,
                                  zd* Ar, int small_stride) {
     const int f = blockIdx.y;
     const zd* M = tb.Mall[f];
     const zd* A = tb.Aall[f];
     zd* mc = Mc + (size_t)f * small_stride;
-    zd* ar = Ar + (size_t)f * small_stride;
+    zd* ar = Ar ? Ar + (size_t)f * small_stride : nullptr;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * k; e += gridDim.x * blockDim.x) {
         const int kk = e / k, c = e - kk * k;
         mc[e] = M[(size_t)kk * d.ldN + t[c]];
+        if (!Ar) continue;
         const int c2 = e / K, k2 = e - c2 * K;
         ar[e] = A[(size_t)t[c2] * d.Kmax + k2];
     }
@@ -710,16 +711,19 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const in
     launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant);
 }
 
-// out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb)
+// out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb;
+// B = b_tab[f] + b_col0 when b_tab is given, else B + f * b_stride)
 // PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
-// LOWRANK: M_new = diag(s) cap^-1 W[t,:]
+// LOWRANK: M_new = diag(s) cap^-1 W[t,:]  (W[t,:]^T = W[:,t], the new A_all columns, when W
+//          is symmetric)
 __global__ void apply_capinv_kernel(Dims d, const zd* __restrict__ acc, int acc_stride, const zd* __restrict__ B,
-                                    size_t b_stride, int ldb, int k, const int32_t* __restrict__ s,
-                                    zd* const* out_tab, zd* out_single, size_t out_stride, int out_row0, int ld_out) {
+                                    size_t b_stride, const zd* const* b_tab, int b_col0, int ldb, int k,
+                                    const int32_t* __restrict__ s, zd* const* out_tab, zd* out_single,
+                                    size_t out_stride, int out_row0, int ld_out) {
     __shared__ zd Bs[32][97];
     const int f = blockIdx.y, tid = threadIdx.x;
     const int n0 = blockIdx.x * 32;
-    const zd* Bf = B + (size_t)f * b_stride;
+    const zd* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
     const zd* ci = acc + (size_t)f * acc_stride;
     for (int e = tid; e < 32 * k; e += 256) {
         const int nn = e / k, l = e - nn * k;
@@ -741,10 +745,11 @@ __global__ void apply_capinv_kernel(Dims d, const zd* __restrict__ acc, int acc_
 
 void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
                          size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
-                         zd* out_single, size_t out_stride, int out_row0, int ld_out) {
+                         zd* out_single, size_t out_stride, int out_row0, int ld_out, const zd* const* b_tab,
+                         int b_col0) {
     dim3 grid((d.N + 31) / 32, d.nf);
-    hipLaunchKernelGGL(apply_capinv_kernel, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, ldb, k, s,
-                       out_tab, out_single, out_stride, out_row0, ld_out);
+    hipLaunchKernelGGL(apply_capinv_kernel, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, b_tab, b_col0,
+                       ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
 }
 
 // q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept)
@@ -1087,6 +1092,46 @@ __global__ void transpose_kernel(int N, int ldN, const zd* __restrict__ in, zd* 
 void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out) {
     dim3 grid((N + 31) / 32, (N + 31) / 32);
     hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, N, ldN, in, out);
+}
+
+// out[0] = max |G_ij - G_ji|, out[1] = max |G_ij| over the column-major N x N matrix
+// (|z| = |Re z| + |Im z|), kept as the bit patterns of non-negative doubles, whose order is
+// that of the unsigned integers.  out must hold zeros before the launch.
+__global__ void __launch_bounds__(256) asym_kernel(int N, int ldN, const zd* __restrict__ G,
+                                                   unsigned long long* out) {
+    __shared__ zd t1[32][33], t2[32][33];
+    __shared__ double red[2][4];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r1 = bx + threadIdx.x, c1 = by + j;  // t1[j][x] = G(bx + x, by + j)
+        t1[j][threadIdx.x] = (r1 < N && c1 < N) ? G[(size_t)c1 * ldN + r1] : zmk(0.0, 0.0);
+        const int r2 = by + threadIdx.x, c2 = bx + j;  // t2[j][x] = G(by + x, bx + j)
+        t2[j][threadIdx.x] = (r2 < N && c2 < N) ? G[(size_t)c2 * ldN + r2] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+    double dmax = 0.0, amax = 0.0;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const zd a = t1[j][threadIdx.x], b = t2[threadIdx.x][j];  // G(bx+x, by+j), G(by+j, bx+x)
+        dmax = fmax(dmax, zabs1(zsub(a, b)));
+        amax = fmax(amax, zabs1(a));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+        amax = fmax(amax, __shfl_xor(amax, off, 64));
+    }
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    if ((tid & 63) == 0) { red[0][tid >> 6] = dmax; red[1][tid >> 6] = amax; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < 4; w++) { dmax = fmax(dmax, red[0][w]); amax = fmax(amax, red[1][w]); }
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(dmax));
+        atomicMax(&out[1], (unsigned long long)__double_as_longlong(amax));
+    }
+}
+
+void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out) {
+    dim3 grid((N + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL(asym_kernel, grid, dim3(32, 8), 0, st, N, ldN, G, out);
 }
 
 // Gobj row 0 = sum_p G(p, :),  Einc = sum_q G(:, q)   (pngfopt.hpp:584-595)

@@ -113,7 +113,10 @@ struct pngf_handle {
     size_t partial_elems = 0;
     int K = 0, Kmax = 0, flushes = 0;
     int skv = 0;                     // tile-shape variant of the skinny products (autotuned)
-    std::vector<double> tune_ms;     // measured time per variant (colblk + rowblk), ms
+    bool sym = false;                // LOWRANK accept takes W[t,:] = W[:,t]^T (G symmetric)
+    double asym = 0.0;               // max over frequencies of max|G_ij - G_ji| / max|G_ij|
+    unsigned long long* d_asym = nullptr;
+    std::vector<double> tune_ms;     // measured time per variant (colblk [+ rowblk]), ms
     // pinned host staging
     int32_t* h_pin_i = nullptr;   // batch ids / small ints
     double* h_pin_d = nullptr;
@@ -325,6 +328,8 @@ int group_for(pngf_handle* h, const int32_t* tiles, int nt, int* g) {
 const double kFref = 30.0e9;
 const double kEpsR = 3.5;
 const double kSoftDefault = 1.0;  // softening radius in cells
+// largest relative asymmetry of G for which the LOWRANK accept treats W as symmetric
+const double kSymTol = 1e-14;
 
 void synth_positions(const Layout& L, double dx, double dy, std::vector<double>& pos) {
     pos.resize((size_t)L.N * 3);
@@ -427,6 +432,17 @@ int init_freq(pngf_handle* h, int f) {
         h->Grow[f] = Gr;
         h->Cinv[f] = C;
     } else {
+        if (h->cfg.symmetry == PNGF_SYM_AUTO) {
+            unsigned long long m[2] = {0, 0};
+            HC(h, hipMemsetAsync(h->d_asym, 0, 2 * sizeof(unsigned long long), st));
+            launch_asym(st, N, ld, G, h->d_asym);
+            HC(h, hipMemcpyAsync(m, h->d_asym, sizeof(m), hipMemcpyDeviceToHost, st));
+            HC(h, hipStreamSynchronize(st));
+            double dm, am;
+            std::memcpy(&dm, &m[0], sizeof(double));
+            std::memcpy(&am, &m[1], sizeof(double));
+            h->asym = std::max(h->asym, am > 0.0 ? dm / am : 0.0);
+        }
         // W = (G - I) C^-1, stored row-major: W^T = C^-T (G - I)^T
         launch_add_diag(st, N, ld, G, -1.0);
         zd* W = nullptr;
@@ -462,7 +478,8 @@ int init_freq(pngf_handle* h, int f) {
 }
 
 // ------------------------------------------------------------ autotuner ---
-// Times every tile-shape variant of the two skinny products on this handle's own
+// Times every tile-shape variant of the skinny products (both, or the column strip alone
+// when W is symmetric) on this handle's own
 // buffers (N, frequencies, a representative factor depth K) and keeps the fastest.
 // The factor buffers are zero at this point, so the timed products read finite data;
 // the columns they write are rewritten by the first accepts that reach them.
@@ -486,8 +503,9 @@ int autotune_skinny(pngf_handle* h) {
         for (int rep = 0; rep < 3; rep++) {
             if (rep == 1) HC(h, hipEventRecord(e0, h->st));
             launch_lr_colblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Mc, (int)h->small_stride, v);
-            launch_lr_rowblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt,
-                             h->bt_stride, v);
+            if (!h->sym)
+                launch_lr_rowblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt,
+                                 h->bt_stride, v);
         }
         HC(h, hipEventRecord(e1, h->st));
         HC(h, hipEventSynchronize(e1));
@@ -572,21 +590,29 @@ int accept_group(pngf_handle* h, int g) {
         Timed tm{};
         prof_begin(h, tm);
         launch_lr_accept_prep(st, d, h->vecs, g, dt, ds, dlp, k, h->d_acc, h->acc_stride, h->d_info, h->d_lu);
-        launch_lr_gather(st, d, h->tabs, dt, k, K, h->d_Mc, h->d_Ar, (int)h->small_stride);
+        launch_lr_gather(st, d, h->tabs, dt, k, K, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride);
         prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
         const double N = d.N, kd = k, Kd = K;
         Timed t1{};
         prof_begin(h, t1);
         launch_lr_colblk(st, d, h->tabs, dt, k, K, h->d_Mc, (int)h->small_stride, h->skv);
         prof_end(h, t1, PNGF_K_COLBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
-        Timed t2{};
-        prof_begin(h, t2);
-        launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv);
-        prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+        if (!h->sym) {
+            Timed t2{};
+            prof_begin(h, t2);
+            launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv);
+            prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+        }
         Timed t3{};
         prof_begin(h, t3);
-        launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds, h->tabs.Mall,
-                            nullptr, 0, K, d.ldN);
+        // M_new = diag(s) cap^-1 W[t,:]; W[t,:]^T is the row strip Bt, or, for symmetric W, the
+        // column strip just written to A_all[:, K:K+k]
+        if (h->sym)
+            launch_apply_capinv(st, d, h->d_acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, h->tabs.Mall, nullptr, 0,
+                                K, d.ldN, h->tabs.Aall, K);
+        else
+            launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds, h->tabs.Mall,
+                                nullptr, 0, K, d.ldN);
         launch_lr_small(st, d, h->tabs, h->vecs, h->d_acc, h->acc_stride, k, K);
         launch_lr_blocks(st, d, h->tabs, h->vecs, gtab(h), k, K);
         launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
@@ -734,7 +760,11 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     if (h->cfg.dx == 0.0) { h->cfg.dx = 0.01 / 60.0; h->cfg.dy = 0.01 / 60.0; h->cfg.dz = 0.01 / 180.0; }
     if (!(h->cfg.synth_soft > 0.0)) h->cfg.synth_soft = kSoftDefault;
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
-    if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6) { delete h; return PNGF_ERR_ARG; }
+    if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6 ||
+        (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF)) {
+        delete h;
+        return PNGF_ERR_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) {
         (void)hipGetLastError();
@@ -994,7 +1024,12 @@ int pngf_init_state(pngf_handle* h) {
     if (h->cfg.method == PNGF_METHOD_LOWRANK)
         CK(dalloc_t(h, &h->vecs.Dblk, (size_t)nf * G.n * gmax * gmax));
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
+    h->asym = 0.0;
+    h->sym = false;
+    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && !h->d_asym)
+        CK(dalloc_t(h, &h->d_asym, 2));
     for (int f = 0; f < nf; f++) CK(init_freq(h, f));
+    h->sym = h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && h->asym <= kSymTol;
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
     if (h->cfg.method == PNGF_METHOD_LOWRANK) {
         size_t freeb = 0, totb = 0;
@@ -1572,6 +1607,13 @@ int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pn
     for (auto& t : th) t.join();
     for (int i = 0; i < n; i++)
         if (rc[i] != PNGF_OK) return rc[i];
+    return PNGF_OK;
+}
+
+int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym) {
+    if (!h) return PNGF_ERR_ARG;
+    if (used) *used = h->sym ? 1 : 0;
+    if (rel_asym) *rel_asym = h->asym;
     return PNGF_OK;
 }
 

@@ -85,3 +85,38 @@ def test_initial_pattern_convention():
     tm = initial_pattern(21, 0)
     assert tm[10 * 21 + 9] == 1 and tm[10 * 21 + 11] == 1 and tm[10 * 21 + 10] == 0
     assert tm[9 * 21 + 10] == 0 and tm[11 * 21 + 10] == 0
+
+
+def header_struct(name):
+    """(field, C type) pairs of `typedef struct { ... } name;` in the header."""
+    txt = open(os.path.join(ROOT, "include", "pngf_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + ";", txt).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"((?:const\s+)?\w+\s*\**)\s*(.*)", decl)
+        ctype = m.group(1).replace(" ", "")
+        for var in m.group(2).split(","):
+            var = var.strip()
+            stars = len(var) - len(var.lstrip("*"))
+            out.append((var.lstrip("*"), ctype + "*" * stars))
+    return out
+
+
+CT_OF = {"int32_t": "c_int", "int64_t": "c_long", "double": "c_double", "float": "c_float"}
+
+
+@pytest.mark.parametrize("cname,py", [("pngf_config", P.Config), ("pngf_dbs_params", P.DbsParams),
+                                      ("pngf_dbs_result", P.DbsResult), ("pngf_rank_params", P.RankParams),
+                                      ("pngf_kstat", P.KStat)])
+def test_struct_mirrors_match_header(cname, py):
+    fields = header_struct(cname)
+    assert [f for f, _ in fields] == [f for f, _ in py._fields_]
+    for (f, ct_), (_, pt) in zip(fields, py._fields_):
+        if ct_.endswith("*"):
+            assert pt.__name__ in ("c_void_p",) or pt.__name__.startswith("LP_"), (f, pt)
+        else:
+            assert pt.__name__ == CT_OF[ct_], (f, ct_, pt)

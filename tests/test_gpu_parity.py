@@ -31,13 +31,15 @@ def rel(a, b):
 
 
 def setup_pair(layout, T, freqs, method, ne=1, driver=P.DRIVER_RASTER, objective=P.OBJ_S11, max_flips=1,
-               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL, tile_variant=0):
-    Gs = [P.synth_g_host(layout, T, ne, hz, soft=soft) for hz in freqs]
+               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL, tile_variant=0, symmetry=P.SYM_AUTO, Gs=None):
+    if Gs is None:
+        Gs = [P.synth_g_host(layout, T, ne, hz, soft=soft) for hz in freqs]
     if tm is None:
         tm = initial_pattern(T, seed)
     mid = (T - 1) // 2
     gpu = P.Optimizer(layout, T, num_freq=len(freqs), num_eobj=ne, max_flips=max_flips, method=method,
-                      objective=objective, driver=driver, max_deferred=max_deferred, tile_variant=tile_variant)
+                      objective=objective, driver=driver, max_deferred=max_deferred, tile_variant=tile_variant,
+                      symmetry=symmetry)
     ors = []
     for f, (hz, G) in enumerate(zip(freqs, Gs)):
         gpu.set_frequency(f, hz)
@@ -131,11 +133,14 @@ def test_dbs_raster_identical_sequence(layout, method):
     assert g["converged"]
 
 
-def test_dbs_multifreq_and_flush():
-    """3 frequencies, deferred factors folded into W every 3 accepts."""
+@pytest.mark.parametrize("symmetry", [P.SYM_AUTO, P.SYM_OFF])
+def test_dbs_multifreq_and_flush(symmetry):
+    """3 frequencies, deferred factors folded into W every 3 accepts; the accept with the
+    row strip taken from the column strip (symmetric G) and with both strips formed."""
     T = 7
     freqs = freq_list(3, 28e9, 32e9)
-    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3, symmetry=symmetry)
+    assert gpu.symmetry()[0] == (symmetry == P.SYM_AUTO)
     g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
     assert g["flushes"] > 0
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
@@ -143,14 +148,15 @@ def test_dbs_multifreq_and_flush():
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
-@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
-def test_dbs_reference_driver(method):
+@pytest.mark.parametrize("method,symmetry", [(P.METHOD_LOWRANK, P.SYM_AUTO), (P.METHOD_LOWRANK, P.SYM_OFF),
+                                             (P.METHOD_PLAIN, P.SYM_AUTO)])
+def test_dbs_reference_driver(method, symmetry):
     """MagicHat order, mirror-symmetric 2/4-tile flips (k up to 96), REF objective, ne = 3."""
     T = 7
     freqs = freq_list(2, 29e9, 31e9)
     seed = -514213645
     gpu, ors, tm = setup_pair(P.LAYOUT_SHARED, T, freqs, method, ne=3, driver=P.DRIVER_REF,
-                              objective=P.OBJ_REF, max_flips=4, seed=1)
+                              objective=P.OBJ_REF, max_flips=4, seed=1, symmetry=symmetry)
     g, o = run_both(gpu, ors, freqs, O.OBJ_REF, seed=seed)
     assert g["accepts"] == o["accepts"] and g["accepts"] > 0
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
@@ -226,3 +232,32 @@ def test_autotune_reports_variants():
     r = gpu.autotune_result()
     assert len(r["ms"]) == 6 and 0 <= r["chosen"] < 6
     assert r["ms"][r["chosen"]] == min(r["ms"])
+
+
+def test_symmetry_detected_for_reciprocal_g():
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, 7, freq_list(2, 29e9, 31e9), P.METHOD_LOWRANK)
+    used, asym = gpu.symmetry()
+    assert used and asym == 0.0
+    plain, _, _ = setup_pair(P.LAYOUT_DISJOINT, 7, [30e9], P.METHOD_PLAIN)
+    assert not plain.symmetry()[0]
+
+
+def test_asymmetric_g_forms_both_strips():
+    """A G that is not symmetric (a random non-reciprocal perturbation of 1e-3 of its largest
+    entry) is detected, and the accept then forms both strips: same run as the oracle."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    rng = np.random.default_rng(7)
+    Gs = []
+    for hz in freqs:
+        G = P.synth_g_host(P.LAYOUT_DISJOINT, T, 1, hz, soft=SOFT_WELL)
+        N = G.shape[0]
+        G = G + 1e-3 * np.abs(G).max() * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N
+        Gs.append(np.asfortranarray(G))
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=4, Gs=Gs)
+    used, asym = gpu.symmetry()
+    assert not used and 1e-7 < asym < 1e-2
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0 and g["flushes"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
