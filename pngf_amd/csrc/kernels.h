@@ -27,6 +27,9 @@ struct DevVecs {
 
 struct Dims {
     int N, ldN, nf, ne, ngroups, gmax, Kmax;
+    // LOWRANK with symmetric W: W0 and every cap^-1-derived factor are kept exactly symmetric
+    // and the diagonal blocks are read as (D + D^T) / 2
+    int sym;
 };
 
 // gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
@@ -91,6 +94,7 @@ void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* G
 void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v);
 void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out);
 void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out);
+void launch_symmetrize(hipStream_t st, int N, int ldN, zd* W);
 void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
                       zd* einc);
 void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe);

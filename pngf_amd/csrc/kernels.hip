@@ -123,6 +123,17 @@ __device__ void lu_inverse_cols(const zd* a, int lda, int k, const int* piv, zd*
     __syncthreads();
 }
 
+// Entry (r, c) of the maintained diagonal block D (row-major, leading dimension gmax).  For
+// symmetric W the block is read as (D + D^T) / 2, which removes the rounding asymmetry that the
+// eager block updates accumulate (a symmetric-representation Woodbury update fed an asymmetric
+// block amplifies that asymmetry at every near-singular capacitance matrix).
+__device__ inline zd dblk(const zd* D, int gmax, int r, int c, int sym) {
+    const zd w = D[r * gmax + c];
+    if (!sym) return w;
+    const zd u = D[c * gmax + r];
+    return zmk(0.5 * (w.x + u.x), 0.5 * (w.y + u.y));
+}
+
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
 // gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
 // per-block slot of a global scratch buffer instead.
@@ -164,14 +175,14 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
     __syncthreads();
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
-        zd w = D[lp[r] * gmax + lp[c]];
+        zd w = dblk(D, gmax, lp[r], lp[c], d.sym);
         zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
         if (r == c) val.x += 1.0;
         a[r * lda + c] = val;
     }
     for (int r = tid; r < k; r += nth) {
         zd acc = q[t[r]];
-        for (int c = 0; c < k; c++) acc = zfma(D[lp[r] * gmax + lp[c]], dpe[c], acc);
+        for (int c = 0; c < k; c++) acc = zfma(dblk(D, gmax, lp[r], lp[c], d.sym), dpe[c], acc);
         a[r * lda + k] = acc;
     }
     __syncthreads();
@@ -481,14 +492,14 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     __syncthreads();
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
-        zd w = D[lpp[r] * gmax + lpp[c]];
+        zd w = dblk(D, gmax, lpp[r], lpp[c], d.sym);
         zd val = ss[c] > 0 ? w : zmk(-w.x, -w.y);
         if (r == c) val.x += 1.0;
         a[r * lda + c] = val;
     }
     for (int r = tid; r < k; r += nth) {
         zd s = q[tt[r]];
-        for (int c = 0; c < k; c++) s = zfma(D[lpp[r] * gmax + lpp[c]], dpe[c], s);
+        for (int c = 0; c < k; c++) s = zfma(dblk(D, gmax, lpp[r], lpp[c], d.sym), dpe[c], s);
         a[r * lda + k] = s;
     }
     __syncthreads();
@@ -513,7 +524,18 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     }
     zd* X = a + k * lda;
     lu_inverse_cols(a, lda, k, piv, X, k);
-    for (int e = tid; e < k * k; e += nth) st[e] = X[e];
+    // Lambda = diag(s) cap^-1 = (diag(s) + W[t,t])^-1 is symmetric with W; for symmetric W it is
+    // stored exactly symmetric: X'[r][c] = (X[r][c] + s_r s_c X[c][r]) / 2, so diag(s) X' = sym(Lambda)
+    for (int e = tid; e < k * k; e += nth) {
+        zd x = X[e];
+        if (d.sym) {
+            const int r = e / k, c = e - r * k;
+            zd u = X[c * k + r];
+            if (ss[r] != ss[c]) u = zmk(-u.x, -u.y);
+            x = zmk(0.5 * (x.x + u.x), 0.5 * (x.y + u.y));
+        }
+        st[e] = x;
+    }
     if (tid == 0 && info_out) info_out[f] = info;
 }
 
@@ -1132,6 +1154,37 @@ __global__ void __launch_bounds__(256) asym_kernel(int N, int ldN, const zd* __r
 void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out) {
     dim3 grid((N + 31) / 32, (N + 31) / 32);
     hipLaunchKernelGGL(asym_kernel, grid, dim3(32, 8), 0, st, N, ldN, G, out);
+}
+
+// W = (W + W^T) / 2 in place (N x N, leading dimension ldN).  The workgroup of tile pair
+// (bx, by), bx <= by, owns both tiles: it stages them in LDS and writes both back, and the two
+// entries of a pair get the same value (addition commutes), so W ends exactly symmetric.
+__global__ void __launch_bounds__(256) symmetrize_kernel(int N, int ldN, zd* W) {
+    if (blockIdx.x > blockIdx.y) return;
+    __shared__ zd t1[32][33], t2[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r1 = bx + j, c1 = by + threadIdx.x;  // t1[j][x] = W[bx + j][by + x]
+        t1[j][threadIdx.x] = (r1 < N && c1 < N) ? W[(size_t)r1 * ldN + c1] : zmk(0.0, 0.0);
+        const int r2 = by + j, c2 = bx + threadIdx.x;  // t2[j][x] = W[by + j][bx + x]
+        t2[j][threadIdx.x] = (r2 < N && c2 < N) ? W[(size_t)r2 * ldN + c2] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r1 = bx + j, c1 = by + threadIdx.x;
+        const zd a = t1[j][threadIdx.x], b = t2[threadIdx.x][j];  // W[r1][c1], W[c1][r1]
+        const zd m = zmk(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+        if (r1 < N && c1 < N) W[(size_t)r1 * ldN + c1] = m;
+        const int r2 = by + j, c2 = bx + threadIdx.x;
+        const zd a2 = t2[j][threadIdx.x], b2 = t1[threadIdx.x][j];  // W[r2][c2], W[c2][r2]
+        const zd m2 = zmk(0.5 * (a2.x + b2.x), 0.5 * (a2.y + b2.y));
+        if (r2 < N && c2 < N) W[(size_t)r2 * ldN + c2] = m2;
+    }
+}
+
+void launch_symmetrize(hipStream_t st, int N, int ldN, zd* W) {
+    dim3 grid((N + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(32, 8), 0, st, N, ldN, W);
 }
 
 // Gobj row 0 = sum_p G(p, :),  Einc = sum_q G(:, q)   (pngfopt.hpp:584-595)

@@ -432,6 +432,7 @@ int init_freq(pngf_handle* h, int f) {
         h->Grow[f] = Gr;
         h->Cinv[f] = C;
     } else {
+        double asym_f = 1.0;
         if (h->cfg.symmetry == PNGF_SYM_AUTO) {
             unsigned long long m[2] = {0, 0};
             HC(h, hipMemsetAsync(h->d_asym, 0, 2 * sizeof(unsigned long long), st));
@@ -441,7 +442,8 @@ int init_freq(pngf_handle* h, int f) {
             double dm, am;
             std::memcpy(&dm, &m[0], sizeof(double));
             std::memcpy(&am, &m[1], sizeof(double));
-            h->asym = std::max(h->asym, am > 0.0 ? dm / am : 0.0);
+            asym_f = am > 0.0 ? dm / am : 0.0;
+            h->asym = std::max(h->asym, asym_f);
         }
         // W = (G - I) C^-1, stored row-major: W^T = C^-T (G - I)^T
         launch_add_diag(st, N, ld, G, -1.0);
@@ -449,6 +451,9 @@ int init_freq(pngf_handle* h, int f) {
         CK(dalloc_t(h, &W, (size_t)ld * N));
         RB(h, rocblas_zgemm(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, N, N, N, &one, (rbz*)C,
                             ld, (rbz*)G, ld, &zero, (rbz*)W, ld));
+        // symmetric G: W is symmetric, and the computed one is made exactly so (its asymmetry is
+        // rounding, cond(C) eps) before the blocks and q are taken from it
+        if (asym_f <= kSymTol) launch_symmetrize(st, N, ld, W);
         RB(h, rocblas_zgemv(h->rb, rocblas_operation_transpose, N, N, &one, (rbz*)W, ld, (rbz*)h->d_pe, 1, &zero,
                             (rbz*)(h->vecs.q + (size_t)f * ld), 1));
         launch_extract_blocks(st, h->d, W, gtab(h), h->vecs.Dblk + (size_t)f * h->d.ngroups * h->d.gmax * h->d.gmax);
@@ -557,9 +562,11 @@ int flush_factors(pngf_handle* h) {
     const Dims& d = h->d;
     Timed tm{};
     prof_begin(h, tm);
-    for (int f = 0; f < d.nf; f++)
+    for (int f = 0; f < d.nf; f++) {
         RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
                             (rbz*)h->Mall[f], d.ldN, (rbz*)h->Aall[f], d.Kmax, &one, (rbz*)h->W0[f], d.ldN));
+        if (h->sym) launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
+    }
     prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (32.0 * d.N * (double)d.N + 32.0 * d.N * (double)h->K),
              (double)d.nf * 8.0 * h->K * (double)d.N * d.N);
     h->K = 0;
@@ -1026,10 +1033,12 @@ int pngf_init_state(pngf_handle* h) {
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
     h->asym = 0.0;
     h->sym = false;
+    h->d.sym = 0;
     if (h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && !h->d_asym)
         CK(dalloc_t(h, &h->d_asym, 2));
     for (int f = 0; f < nf; f++) CK(init_freq(h, f));
     h->sym = h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && h->asym <= kSymTol;
+    h->d.sym = h->sym ? 1 : 0;
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
     if (h->cfg.method == PNGF_METHOD_LOWRANK) {
         size_t freeb = 0, totb = 0;
