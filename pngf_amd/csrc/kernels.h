@@ -66,6 +66,10 @@ void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_s
                          const zd* const* b_tab = nullptr, int b_col0 = 0);
 void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
                      int acc_stride, int k, int K);
+// fused apply_capinv + lr_small + lr_blocks for gmax <= 32
+void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
+                      const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
+                      int K);
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       int k, int K);
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,

@@ -134,6 +134,91 @@ __device__ inline zd dblk(const zd* D, int gmax, int r, int c, int sym) {
     return zmk(0.5 * (w.x + u.x), 0.5 * (w.y + u.y));
 }
 
+// change_set computed by the 64 lanes of one wave (all of them call it): lane a tests
+// component a of the group, and a ballot keeps the changed ones in ascending list order.
+__device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, const int8_t* P,
+                               const int32_t* owners, int* t, int* sg, int* lp) {
+    const int lane = threadIdx.x & 63;
+    const int nt = gt.ntiles[g];
+    const int32_t* gtl = gt.tiles + 4 * g;
+    const int nc = gt.ncomp[g];
+    const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
+    int k = 0;
+    for (int base = 0; base < nc; base += 64) {
+        const int a = base + lane;
+        bool ch = false;
+        int sgn = 0, i = -1;
+        if (a < nc) {
+            i = cl[a];
+            int nv = 0;
+            for (int h = 0; h < 2; h++) {
+                const int o = owners[2 * i + h];
+                if (o < 0) continue;
+                int v = tile[o];
+                for (int q = 0; q < nt; q++) v ^= (gtl[q] == o) ? 1 : 0;
+                nv |= v;
+            }
+            const int ov = P[i];
+            ch = nv != ov;
+            sgn = nv - ov;
+        }
+        const unsigned long long m = __ballot(ch);
+        if (ch) {
+            const int pos = k + __popcll(m & ((1ull << lane) - 1ull));
+            t[pos] = i;
+            sg[pos] = sgn;
+            lp[pos] = a;
+        }
+        k += __popcll(m);
+    }
+    return k;
+}
+
+// Gauss-Jordan elimination with partial pivoting of the k x k leading block (k <= 32) of the
+// LDS matrix a (leading dimension lda), applying the row operations to columns k..ncol-1.
+// Rows stay in place: perm[i] is the storage row of logical row i, the order that zgetf2's
+// row interchanges produce (pivot = first remaining logical row of max |Re|+|Im|).  Step j
+// removes column j from every other row and writes only columns > j (column j of the other
+// rows is not read again), so a step needs two barriers.  On return storage row perm[i] holds
+// d_i in column i and d_i times the solution in columns k..ncol-1.  Returns LAPACK-style info
+// (0, or j+1 for the first zero pivot, whose elimination step is skipped).  All threads of the
+// block call it; blockDim.x is a multiple of 32 and at least 64.
+__device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
+    __shared__ int s_info;
+    const int tid = threadIdx.x;
+    const int ri = tid & 31, cg = tid >> 5, ncg = blockDim.x >> 5;
+    if (tid < k) perm[tid] = tid;
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    for (int j = 0; j < k; j++) {
+        if (tid < 64) {
+            const int i = j + tid;
+            double best = i < k ? zabs1(a[perm[i] * lda + j]) : -1.0;
+            int bi = i;
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (tid == 0) {
+                const int pj = perm[j];
+                perm[j] = perm[bi];
+                perm[bi] = pj;
+                if (best == 0.0 && s_info == 0) s_info = j + 1;
+            }
+        }
+        __syncthreads();
+        const int pr = perm[j];
+        const zd pv = a[pr * lda + j];
+        if ((pv.x != 0.0 || pv.y != 0.0) && ri < k && ri != pr) {
+            const zd l = zmul(a[ri * lda + j], zrecip(pv));
+            for (int c = j + 1 + cg; c < ncol; c += ncg) a[ri * lda + c] = zfms(l, a[pr * lda + c], a[ri * lda + c]);
+        }
+        __syncthreads();
+    }
+    return s_info;
+}
+
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
 // gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
 // per-block slot of a global scratch buffer instead.
@@ -161,7 +246,14 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
     zd* a = gscratch ? gscratch + ((size_t)f * gridDim.x + b) * gmax * (2 * gmax + 1) : (zd*)(piv + gmax);
     __shared__ int s_k;
     const int g = batch[b];
-    if (tid == 0) s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    if (!gscratch) {
+        if (tid < 64) {
+            const int kk = change_set_wave(gt, g, tile, P, owners, t, sg, lp);
+            if (tid == 0) s_k = kk;
+        }
+    } else if (tid == 0) {
+        s_k = change_set(gt, g, tile, P, owners, t, sg, lp);
+    }
     __syncthreads();
     const int k = s_k;
     const int lda = k + 1;
@@ -186,8 +278,20 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
         a[r * lda + k] = acc;
     }
     __syncthreads();
-    int info = lds_lu(a, lda, k, k + 1, piv);
-    lds_backsolve(a, lda, k, k);
+    int info;
+    if (!gscratch) {
+        // Gauss-Jordan in LDS; y_c then goes to column k of storage row c (after a barrier, so
+        // that no thread still reads the eliminated matrix)
+        info = lds_gj(a, lda, k, k + 1, piv);
+        zd y = zmk(0.0, 0.0);
+        if (tid < k) y = zdiv(a[piv[tid] * lda + k], a[piv[tid] * lda + tid]);
+        __syncthreads();
+        if (tid < k) a[tid * lda + k] = y;
+        __syncthreads();
+    } else {
+        info = lds_lu(a, lda, k, k + 1, piv);
+        lds_backsolve(a, lda, k, k);
+    }
     if (tid < d.ne) {
         const int r = tid;
         const zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
@@ -483,7 +587,9 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     zd* dpe = (zd*)(piv + 100);
     zd* tmp = dpe + 96;
     zd* a = gscratch ? gscratch + (size_t)f * gmax * (2 * gmax + 1) : tmp + 96;
-    const int lda = k + 1;
+    // k <= 32 (LDS): Gauss-Jordan on [cap | rhs | I]; larger k: LU, back substitution, inverse
+    const bool gj = gscratch == nullptr;
+    const int lda = gj ? 2 * k + 1 : k + 1;
     const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
     const zd* q = v.q + (size_t)f * d.ldN;
     const zd* einc = v.Einc + (size_t)f * d.ldN;
@@ -502,11 +608,21 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         for (int c = 0; c < k; c++) s = zfma(dblk(D, gmax, lpp[r], lpp[c], d.sym), dpe[c], s);
         a[r * lda + k] = s;
     }
+    if (gj)
+        for (int e = tid; e < k * k; e += nth) {
+            const int r = e / k, c = e - r * k;
+            a[r * lda + k + 1 + c] = zmk(r == c ? 1.0 : 0.0, 0.0);
+        }
     __syncthreads();
-    int info = lds_lu(a, lda, k, k + 1, piv);
-    lds_backsolve(a, lda, k, k);
+    int info;
+    if (gj) {
+        info = lds_gj(a, lda, k, 2 * k + 1, piv);
+    } else {
+        info = lds_lu(a, lda, k, k + 1, piv);
+        lds_backsolve(a, lda, k, k);
+    }
     for (int c = tid; c < k; c += nth) {
-        zd y = a[c * lda + k];
+        zd y = gj ? zdiv(a[piv[c] * lda + k], a[piv[c] * lda + c]) : a[c * lda + k];
         st[2 * k * k + c] = y;
         st[2 * k * k + k + c] = ss[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
     }
@@ -522,15 +638,19 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         st[2 * k * k + 2 * k + r] = s;
         v.Eobj[f * ne + r] = s;
     }
+    // cap^-1: X[r][c] = a[perm r][k+1+c] / d_r after Gauss-Jordan, else from the LU factors
     zd* X = a + k * lda;
-    lu_inverse_cols(a, lda, k, piv, X, k);
+    if (!gj) lu_inverse_cols(a, lda, k, piv, X, k);
+    auto xinv = [&](int r, int c) {
+        return gj ? zdiv(a[piv[r] * lda + k + 1 + c], a[piv[r] * lda + r]) : X[r * k + c];
+    };
     // Lambda = diag(s) cap^-1 = (diag(s) + W[t,t])^-1 is symmetric with W; for symmetric W it is
     // stored exactly symmetric: X'[r][c] = (X[r][c] + s_r s_c X[c][r]) / 2, so diag(s) X' = sym(Lambda)
     for (int e = tid; e < k * k; e += nth) {
-        zd x = X[e];
+        const int r = e / k, c = e - r * k;
+        zd x = xinv(r, c);
         if (d.sym) {
-            const int r = e / k, c = e - r * k;
-            zd u = X[c * k + r];
+            zd u = xinv(c, r);
             if (ss[r] != ss[c]) u = zmk(-u.x, -u.y);
             x = zmk(0.5 * (x.x + u.x), 0.5 * (x.y + u.y));
         }
@@ -543,7 +663,7 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int 
                            const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
                            int32_t* info, zd* gscratch) {
     size_t sm = 100 * sizeof(int) + 192 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
-    hipLaunchKernelGGL(lr_prep_kernel, dim3(d.nf), dim3(k <= 32 ? 64 : 256), sm, st, d, v, g, t, s, lp, k, acc,
+    hipLaunchKernelGGL(lr_prep_kernel, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
                        acc_stride, info, gscratch);
 }
 
@@ -857,6 +977,107 @@ void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const De
         hipLaunchKernelGGL((lr_blocks_kernel<4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
     else
         hipLaunchKernelGGL((lr_blocks_kernel<36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+}
+
+// LOWRANK accept epilogue for k <= 32 in one launch.  Blocks [0, nchunk) take 32 components
+// each: the new rows of M_all, q and R; blocks [nchunk, nchunk + ngroups) take one group's
+// diagonal block each, recomputing M_new at its components.
+//   M_new[c][n] = s_c sum_l capinv'[c][l] S[n][l]   with S[n][:] = W[t, n]: the new A_all
+//                 columns when W is symmetric, else the row strip Bt
+//   q[n] += sum_c A_new[n][c] w[c];   R[r][n] -= sum_c Rt[r][c] M_new[c][n]
+//   D_h[a][b] -= sum_c A_new[cl_a][c] M_new[c][cl_b]
+// Same operations in the same order as apply_capinv, lr_small and lr_blocks.
+__global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
+                                                        const zd* __restrict__ acc, int acc_stride,
+                                                        const zd* __restrict__ Bt, size_t bt_stride,
+                                                        const int32_t* __restrict__ s, int k, int K, int nchunk) {
+    __shared__ zd Xs[32][33];
+    __shared__ zd As[32][33];
+    __shared__ zd Ss[32][33];  // row strip, then M_new[c][row]
+    __shared__ int rows[32];
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const zd* stf = acc + (size_t)f * acc_stride;
+    const bool grp = (int)blockIdx.x >= nchunk;
+    const int h = grp ? (int)blockIdx.x - nchunk : 0;
+    const int n0 = grp ? 0 : (int)blockIdx.x * 32;
+    const int nrow = grp ? gt.ncomp[h] : min(32, d.N - n0);
+    if (nrow <= 0) return;
+    for (int e = tid; e < k * k; e += 256) {
+        const int r = e / k, c = e - r * k;
+        Xs[r][c] = stf[e];
+    }
+    if (tid < nrow) rows[tid] = grp ? gt.comps[(size_t)h * gt.gmax + tid] : n0 + tid;
+    __syncthreads();
+    const zd* A = tb.Aall[f];
+    const zd* S = Bt + (size_t)f * bt_stride;
+    for (int e = tid; e < nrow * k; e += 256) {
+        const int rr = e / k, c = e - rr * k;
+        const int n = rows[rr];
+        As[rr][c] = A[(size_t)n * d.Kmax + K + c];
+        if (!d.sym) Ss[rr][c] = S[(size_t)n * d.gmax + c];
+    }
+    __syncthreads();
+    zd m[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int e = tid + 256 * u;
+        m[u] = zmk(0.0, 0.0);
+        if (e < k * nrow) {
+            const int c = e / nrow, rr = e - c * nrow;
+            zd sacc = zmk(0.0, 0.0);
+            for (int l = 0; l < k; l++) sacc = zfma(Xs[c][l], d.sym ? As[rr][l] : Ss[rr][l], sacc);
+            m[u] = s[c] < 0 ? zmk(-sacc.x, -sacc.y) : sacc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int e = tid + 256 * u;
+        if (e < k * nrow) {
+            const int c = e / nrow, rr = e - c * nrow;
+            Ss[c][rr] = m[u];
+        }
+    }
+    __syncthreads();
+    if (!grp) {
+        zd* M = tb.Mall[f];
+        for (int e = tid; e < k * nrow; e += 256) {
+            const int c = e / nrow, rr = e - c * nrow;
+            M[(size_t)(K + c) * d.ldN + n0 + rr] = Ss[c][rr];
+        }
+        const zd* w = stf + 2 * k * k + k;
+        const zd* Rt = stf + 2 * k * k + 2 * k + 4;
+        if (tid < nrow) {
+            zd* qp = v.q + (size_t)f * d.ldN + n0 + tid;
+            zd qa = *qp;
+            for (int c = 0; c < k; c++) qa = zfma(As[tid][c], w[c], qa);
+            *qp = qa;
+        } else if (tid >= 64 && tid < 64 + 32 * d.ne) {
+            const int r = (tid - 64) >> 5, rr = (tid - 64) & 31;
+            if (rr < nrow) {
+                zd* rp = v.R + ((size_t)f * d.ne + r) * d.ldN + n0 + rr;
+                zd ra = *rp;
+                for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], Ss[c][rr], ra);
+                *rp = ra;
+            }
+        }
+    } else {
+        zd* Dh = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
+        for (int e = tid; e < nrow * nrow; e += 256) {
+            const int a = e / nrow, b = e - a * nrow;
+            zd sacc = zmk(0.0, 0.0);
+            for (int c = 0; c < k; c++) sacc = zfma(As[a][c], Ss[c][b], sacc);
+            Dh[a * gt.gmax + b] = zsub(Dh[a * gt.gmax + b], sacc);
+        }
+    }
+}
+
+void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
+                      const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
+                      int K) {
+    const int nchunk = (d.N + 31) / 32;
+    hipLaunchKernelGGL(lr_finish_kernel, dim3(nchunk + d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                       acc_stride, Bt, bt_stride, s, k, K, nchunk);
 }
 
 __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,

@@ -614,14 +614,19 @@ int accept_group(pngf_handle* h, int g) {
         prof_begin(h, t3);
         // M_new = diag(s) cap^-1 W[t,:]; W[t,:]^T is the row strip Bt, or, for symmetric W, the
         // column strip just written to A_all[:, K:K+k]
-        if (h->sym)
-            launch_apply_capinv(st, d, h->d_acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, h->tabs.Mall, nullptr, 0,
-                                K, d.ldN, h->tabs.Aall, K);
-        else
-            launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds, h->tabs.Mall,
-                                nullptr, 0, K, d.ldN);
-        launch_lr_small(st, d, h->tabs, h->vecs, h->d_acc, h->acc_stride, k, K);
-        launch_lr_blocks(st, d, h->tabs, h->vecs, gtab(h), k, K);
+        if (d.gmax <= kLdsGmax) {
+            launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, k,
+                             K);
+        } else {
+            if (h->sym)
+                launch_apply_capinv(st, d, h->d_acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, h->tabs.Mall, nullptr,
+                                    0, K, d.ldN, h->tabs.Aall, K);
+            else
+                launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds,
+                                    h->tabs.Mall, nullptr, 0, K, d.ldN);
+            launch_lr_small(st, d, h->tabs, h->vecs, h->d_acc, h->acc_stride, k, K);
+            launch_lr_blocks(st, d, h->tabs, h->vecs, gtab(h), k, K);
+        }
         launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
         prof_end(h, t3, PNGF_K_ACC_SMALL, 0, 0);
         h->K += k;
