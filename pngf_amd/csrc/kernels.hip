@@ -696,148 +696,148 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
 // (v_mfma_f64_16x16x4f64):
 //   colblk: A_all[n][K+c] = W0[n][t_c] - sum_K' A_all[n][K'] Mc[K'][c]
 //   rowblk: Bt[n][c]      = W0[t_c][n] - sum_K' Ar[c][K'] M_all[K'][n]
-// The complex product is run as a real one: out_re/out_im interleaved (2k real columns) =
-// A_real (n x 2K, the interleaved re/im storage of A) x B_real (2K x 2k) with
-//   B_real[2K'+0][2c+0] =  Re b   B_real[2K'+0][2c+1] = Im b
-//   B_real[2K'+1][2c+0] = -Im b   B_real[2K'+1][2c+1] = Re b      (b = B[K'][c]).
-// One wave owns 16 output rows and all CT 16-column tiles; 8 waves share the
-// B values staged in LDS.  The reduction order over the inner index is free, so
-// at sub-step s lane l feeds inner element 4*(l/16) + s: each lane then loads two
-// consecutive complex entries of A per 8-entry chunk (coalesced 128-256 B rows).
-// Fragment layouts (16x16x4 f64, measured on gfx950 with a lane probe): A[i][k] in
-// lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4
-// (each lane group holds every fourth row).
+// The complex product runs as a real one: out (n x 2k interleaved re/im) = A_real (n x 2K,
+// the interleaved storage of A) x B_real (2K x 2k) with, for b = B[K'][c],
+//   B_real[2K'][2c] = Re b   B_real[2K'][2c+1] = Im b   B_real[2K'+1][2c] = -Im b   B_real[2K'+1][2c+1] = Re b.
+// B is staged in LDS already in that operand form: Bp[kk][j] holds (B_real[2kk][j],
+// B_real[2kk+1][j]) for real column j, so one 16-byte LDS read feeds the two MFMAs of a
+// complex inner index with no VALU work.
+// One wave owns 16 output rows and all CT 16-column tiles.  Lane l feeds inner index
+// 2*(l/16)+j of each 8-entry sub-chunk (two consecutive complex A entries per lane).
+// Fragment layouts (16x16x4 f64, measured on gfx950 with a lane probe): A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4.
+// Memory pipeline: the A entries of the next DEPTH sub-chunks are in flight in a register
+// ring.  Loads go through global-address-space pointers (counted on vmcnt only), are
+// branch-free (row clamped to N-1, inner index clamped to K-1: the clamped entries meet zero
+// rows of the staged B), and each staged chunk of KC inner entries is straight-line code with a
+// scheduling barrier after every refill, so the waits before each MFMA group are exact.
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const double gdbl;
 
-// Tile shape: RT 16-row tiles per wave (B fragments reused RT times), KC complex inner
-// entries staged in LDS per barrier pair, WAVES waves per workgroup sharing them.  The A
-// entries of the next 8-entry sub-chunk are loaded while the current one is multiplied.
-template <int CT, bool ROW, int RT, int KC, int WAVES>
+__device__ inline zd ldg(const zd* p, size_t i) {
+    gdbl* q = (gdbl*)(p + i);
+    zd r;
+    r.x = q[0];
+    r.y = q[1];
+    return r;
+}
+
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
                                                                     int k, int K, const zd* __restrict__ small,
                                                                     int small_stride, zd* Bt, size_t bt_stride) {
-    constexpr int BW = CT * 8;  // complex columns held (>= k)
-    constexpr int NT = WAVES * 64;
-    __shared__ zd Bs[KC][BW + 1];
+    constexpr int BW = CT * 8;   // complex columns held (>= k)
+    constexpr int NJ = CT * 16;  // real columns
+    constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64, NA = SPLIT ? 2 : 1;
+    static_assert(NS % DEPTH == 0, "ring slots line up across chunks");
+    __shared__ zd Bp[KC][NJ + 2];
     const int f = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    const int n0 = (blockIdx.x * WAVES + wave) * 16 * RT;
-    int nc[RT];
-    bool nv[RT];
-#pragma unroll
-    for (int r = 0; r < RT; r++) {
-        const int n = n0 + 16 * r + lr;
-        nv[r] = n < d.N;
-        nc[r] = nv[r] ? n : d.N - 1;  // clamped row for loads
-    }
+    const int n0 = (blockIdx.x * WAVES + wave) * 16;
+    const int nc = min(n0 + lr, d.N - 1);  // clamped row for loads
     const zd* A = tb.Aall[f];
     const zd* M = tb.Mall[f];
     const zd* W0 = tb.W0[f];
     const zd* sm = small + (size_t)f * small_stride;
-    d4v acc[RT][CT];
+    const int Kc = K - 1;
+    d4v acc[NA][CT];
 #pragma unroll
-    for (int r = 0; r < RT; r++)
+    for (int h = 0; h < NA; h++)
 #pragma unroll
-        for (int c = 0; c < CT; c++) acc[r][c] = d4v{0.0, 0.0, 0.0, 0.0};
-    auto loadA = [&](int kbase, zd (&a)[RT][2]) {
+        for (int c = 0; c < CT; c++) acc[h][c] = d4v{0.0, 0.0, 0.0, 0.0};
+    auto ld = [&](int kb, zd (&a)[2]) {
 #pragma unroll
-        for (int r = 0; r < RT; r++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int Kp = kbase + 2 * lq + j;
-                zd v = zmk(0.0, 0.0);
-                if (Kp < K && nv[r]) v = ROW ? M[(size_t)Kp * d.ldN + nc[r]] : A[(size_t)nc[r] * d.Kmax + Kp];
-                a[r][j] = v;
-            }
+        for (int j = 0; j < 2; j++) {
+            const int Kp = min(kb + 2 * lq + j, Kc);
+            a[j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc) : ldg(A, (size_t)nc * d.Kmax + Kp);
+        }
     };
+    auto body = [&](const zd (&a)[2], int kloc) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            zd m[CT];
+#pragma unroll
+            for (int c = 0; c < CT; c++) m[c] = Bp[kloc + 2 * lq + j][c * 16 + lr];
+#pragma unroll
+            for (int c = 0; c < CT; c++) acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].x, m[c].x, acc[0][c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < CT; c++)
+                acc[NA - 1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].y, m[c].y, acc[NA - 1][c], 0, 0, 0);
+        }
+    };
+    zd buf[DEPTH][2];
+    if (K > 0) {
+#pragma unroll
+        for (int q = 0; q < DEPTH; q++) ld(q * KS, buf[q]);
+    }
     for (int k0 = 0; k0 < K; k0 += KC) {
         __syncthreads();
         for (int e = threadIdx.x; e < KC * BW; e += NT) {
             const int kk = e / BW, c = e - kk * BW;
-            zd v = zmk(0.0, 0.0);
-            if (k0 + kk < K && c < k) v = ROW ? sm[(size_t)c * K + k0 + kk] : sm[(size_t)(k0 + kk) * k + c];
-            Bs[kk][c] = v;
+            zd b = zmk(0.0, 0.0);
+            if (k0 + kk < K && c < k) b = ldg(sm, ROW ? (size_t)c * K + k0 + kk : (size_t)(k0 + kk) * k + c);
+            Bp[kk][2 * c] = zmk(b.x, -b.y);
+            Bp[kk][2 * c + 1] = zmk(b.y, b.x);
         }
         __syncthreads();
-        const int kend = min(KC, K - k0);
-        zd a[RT][2], an[RT][2];
-        loadA(k0, a);
-        for (int kk0 = 0; kk0 < kend; kk0 += 8) {
-            if (kk0 + 8 < kend) loadA(k0 + kk0 + 8, an);
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                zd m[CT];
-#pragma unroll
-                for (int c = 0; c < CT; c++) m[c] = Bs[kk0 + 2 * lq + j][(c * 16 + lr) >> 1];
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    const bool odd = lr & 1;
-#pragma unroll
-                    for (int c = 0; c < CT; c++) {
-                        const double bv = p == 0 ? (odd ? m[c].y : m[c].x) : (odd ? m[c].x : -m[c].y);
-#pragma unroll
-                        for (int r = 0; r < RT; r++) {
-                            const double av = p ? a[r][j].y : a[r][j].x;
-                            acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[r][c], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RT; r++) { a[r][0] = an[r][0]; a[r][1] = an[r][1]; }
+        for (int i = 0; i < NS; i++) {
+            body(buf[i % DEPTH], i * KS);
+            ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 #pragma unroll
-    for (int r = 0; r < RT; r++)
+    for (int c = 0; c < CT; c++) {
+        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+        if (cc >= k) continue;
 #pragma unroll
-        for (int c = 0; c < CT; c++) {
-            const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
-            if (cc >= k) continue;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = n0 + 16 * r + lq + 4 * v;
-                if (row >= d.N) continue;
-                if (!ROW) {
-                    const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
-                    ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - acc[r][c][v];
-                } else {
-                    const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
-                    ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - acc[r][c][v];
-                }
+        for (int v = 0; v < 4; v++) {
+            const int row = n0 + lq + 4 * v;
+            if (row >= d.N) continue;
+            const double a = NA == 2 ? acc[0][c][v] + acc[NA - 1][c][v] : acc[0][c][v];
+            if (!ROW) {
+                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
+                ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
+            } else {
+                const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
+                ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
             }
         }
+    }
 }
 
 // Tile-shape variants offered to the autotuner, per column-tile count CT.
-template <int CT, bool ROW, int RT, int KC, int WAVES>
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                            const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
-    const int rows = 16 * RT * WAVES;
+    const int rows = 16 * WAVES;
     dim3 grid((d.N + rows - 1) / rows, d.nf);
-    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, RT, KC, WAVES>), grid, dim3(WAVES * 64), 0, st, d, tb, t, k, K,
-                       small, small_stride, Bt, bt_stride);
+    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, SPLIT>), grid, dim3(WAVES * 64), 0, st, d, tb,
+                       t, k, K, small, small_stride, Bt, bt_stride);
 }
 
 template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                                const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant) {
-#define PNGF_SKV(CT, RT, KC, W) launch_variant<CT, ROW, RT, KC, W>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride)
+#define PNGF_SKV(CT, KC, DEPTH, W, SPLIT) \
+    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride)
     if (k <= 24) {
         switch (variant) {
-            case 1: PNGF_SKV(3, 2, 64, 8); break;
-            case 2: PNGF_SKV(3, 2, 64, 4); break;
-            case 3: PNGF_SKV(3, 1, 128, 8); break;
-            case 4: PNGF_SKV(3, 4, 32, 4); break;
-            case 5: PNGF_SKV(3, 2, 128, 4); break;
-            default: PNGF_SKV(3, 1, 64, 8); break;
+            case 1: PNGF_SKV(3, 64, 4, 8, true); break;
+            case 2: PNGF_SKV(3, 64, 2, 8, false); break;
+            case 3: PNGF_SKV(3, 32, 4, 8, false); break;
+            case 4: PNGF_SKV(3, 64, 4, 8, false); break;
+            case 5: PNGF_SKV(3, 16, 2, 8, false); break;
+            default: PNGF_SKV(3, 32, 2, 8, false); break;
         }
     } else if (k <= 48) {
-        if (variant % 2) PNGF_SKV(6, 2, 32, 4);
-        else PNGF_SKV(6, 1, 32, 8);
+        if (variant % 2) PNGF_SKV(6, 16, 2, 8, false);
+        else PNGF_SKV(6, 32, 2, 8, false);
     } else {
-        if (variant % 2) PNGF_SKV(12, 2, 16, 4);
-        else PNGF_SKV(12, 1, 16, 8);
+        if (variant % 2) PNGF_SKV(12, 16, 2, 4, false);
+        else PNGF_SKV(12, 16, 2, 8, false);
     }
 #undef PNGF_SKV
 }
