@@ -99,6 +99,19 @@ def cpu_baseline(wl, g0, cinv0, budget_s):
                        % (nf, r["flip_count"], r["accepts"], r["loop_s"], nf))
 
 
+def copy_probe():
+    """Three device-to-device copies of 1 GiB: the reference reading for the HBM byte
+    counters (FETCH_SIZE / WRITE_SIZE) that scale the kernels' measured traffic."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "copies": 3}))
+
+
 def bench_multi():
     """16 DBS instances of uneven size (11..25 tiles, 5 frequencies) on one GPU at once.
     Prints its own JSON line: total committed evaluations / wall seconds of the
@@ -144,6 +157,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
     args = ap.parse_args()
@@ -151,6 +166,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.workload == "multi16":
         bench_multi()
         return
