@@ -49,8 +49,9 @@ def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO):
     return opt, time.perf_counter() - t0
 
 
-def roofline(opt):
+def roofline(opt, gmax=24):
     """Dominant kernel category by event-timed duration -> roofline record."""
+    n_comp = opt.N
     best = None
     for k in range(6):
         s = opt.kernel_stats(k)
@@ -73,6 +74,13 @@ def roofline(opt):
                    traffic=None)
     rec.update(kernel=P.K_NAMES[k], avg_ms=avg_ms, launches=s["launches"], algo_bytes=bytes_l,
                algo_flops=flops_l)
+    if P.K_NAMES[k] in ("colblk", "rowblk"):
+        # HBM bytes per launch: the algorithmic bytes plus the staged K x k factor block that the
+        # 7 other XCD L2 caches fetch again, a factor 1 + 7k/N that matches rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE counts (copy-calibrated) at N = 10590 and N = 2910
+        # (profiles/colblk_hbm_traffic_antenna21x21.txt); not counted in this run
+        rec["traffic"] = bytes_l * (1.0 + 7.0 * gmax / n_comp)
+        rec["traffic_source"] = "model 1 + 7k/N fitted to rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE"
     allk = {P.K_NAMES[i]: opt.kernel_stats(i) for i in range(6)}
     return rec, allk
 
@@ -157,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--tiles", type=int, default=0, help="override the workload's tile count")
+    ap.add_argument("--nfreq", type=int, default=0, help="override the workload's frequency count")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
@@ -173,9 +183,17 @@ def main():
         bench_multi()
         return
     wl = dict(WORKLOADS[args.workload])
+    if args.tiles:
+        wl["T"] = args.tiles
+    if args.nfreq:
+        lo, hi = min(wl["freqs"]), max(wl["freqs"])
+        wl["freqs"] = list(np.linspace(lo, hi, args.nfreq)) if args.nfreq > 1 else [0.5 * (lo + hi)]
+    wl_name = args.workload
+    if args.tiles or args.nfreq:
+        wl_name = "antenna%dx%d_%df" % (wl["T"], wl["T"], len(wl["freqs"]))
     method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
 
-    log("[bench] init %s method=%s" % (args.workload, args.method))
+    log("[bench] init %s method=%s" % (wl_name, args.method))
     opt, init_s = make(wl, method, args.max_deferred, P.SYM_AUTO if args.symmetry == "auto" else P.SYM_OFF)
     opt.checkpoint()
     tune = opt.autotune_result() if method == P.METHOD_LOWRANK else None
@@ -252,7 +270,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "tiles": wl["T"], "N": P.layout_size(wl["layout"], wl["T"]),
+        "config": {"workload": wl_name, "tiles": wl["T"], "N": P.layout_size(wl["layout"], wl["T"]),
                    "freqs": len(wl["freqs"]), "layout": "disjoint" if wl["layout"] == 1 else "shared",
                    "method": args.method, "objective": "mean |S11|^2", "driver": "raster",
                    "max_evals": wl["max_evals"], "symmetry": args.symmetry},
