@@ -80,7 +80,8 @@ typedef struct {
     int32_t objective;    /* PNGF_OBJ_* */
     int32_t driver;       /* PNGF_DRIVER_*: fixes the candidate groups (flip sets) */
     int32_t max_deferred; /* LOWRANK: accepted flips kept as factors before they are folded
-                             into W0 (0 = choose from free memory) */
+                             into W0 (0 = N / flip size, the point beyond which folding pays,
+                             capped by free device memory) */
     int32_t device;       /* HIP device ordinal */
     int32_t tile_variant; /* LOWRANK skinny-product tile shape: 0 = autotune at init, v+1 = use v */
     double dx, dy, dz;    /* Yee cell size used by the port impedance (main.cpp:157-160) */
@@ -195,7 +196,9 @@ typedef struct {
     int32_t converged;         /* 1 if stopped because no flip improves */
     int64_t logged;            /* rows written to log_x / log_y */
     int64_t ranked_scored;     /* ranked driver: candidates scored by the network */
-    int64_t full_sweeps;       /* ranked driver: exact full sweeps (fallbacks and final check) */
+    int64_t full_sweeps;       /* ranked driver: exact full sweeps (hit-rate fallbacks) */
+    int64_t ranked_steps;      /* ranked driver: steps that followed the network's order */
+    int64_t ranked_hits;       /* ranked driver: those whose first top_k chunk held an improving flip */
 } pngf_dbs_result;
 
 /* Runs the DBS loop of main.cpp:338-459 on the device with the driver chosen at create.
@@ -243,9 +246,12 @@ typedef struct {
     const float* y_std;
 } pngf_rank_params;
 /* Ranked DBS (S11 objective, raster candidates): each step the network scores every
-   candidate, the top_k are scored exactly in FP64 and the best improving one is
-   accepted; if none improves, an exact full sweep decides (accept its best, or stop:
-   converged).  flip_count counts exact evaluations only. */
+   candidate and the candidates are scored exactly in FP64 in that order, top_k at a time;
+   the best improving flip of the first chunk that holds one is accepted.  A step whose
+   chunks run through every candidate without an improving one ends the run (converged: an
+   exact local minimum).  When fewer than min_hit_rate of the last `window` steps found an
+   improving flip in their first chunk, the next step is an exact full sweep instead
+   (accept its best).  flip_count counts exact evaluations only. */
 int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp, const pngf_dbs_params* p,
                         pngf_dbs_result* r);
 

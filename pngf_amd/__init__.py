@@ -70,7 +70,7 @@ class DbsResult(ct.Structure):
                 ("batches", ct.c_int64), ("F_start", ct.c_double), ("F_final", ct.c_double),
                 ("seconds", ct.c_double), ("accept_seconds", ct.c_double), ("flushes", ct.c_int32),
                 ("converged", ct.c_int32), ("logged", ct.c_int64), ("ranked_scored", ct.c_int64),
-                ("full_sweeps", ct.c_int64)]
+                ("full_sweeps", ct.c_int64), ("ranked_steps", ct.c_int64), ("ranked_hits", ct.c_int64)]
 
 
 class RankParams(ct.Structure):
@@ -310,7 +310,8 @@ class Optimizer:
                    batches=r.batches, F_start=r.F_start, F_final=r.F_final, seconds=r.seconds,
                    accept_seconds=r.accept_seconds, flushes=r.flushes, converged=bool(r.converged),
                    acc_tiles=acc[0][:na].copy(), acc_eval=acc[1][:na].copy(), acc_F=acc[2][:na].copy(),
-                   ranked_scored=r.ranked_scored, full_sweeps=r.full_sweeps)
+                   ranked_scored=r.ranked_scored, full_sweeps=r.full_sweeps, ranked_steps=r.ranked_steps,
+                   ranked_hits=r.ranked_hits)
         if log is not None:
             out["log_x"] = log[0][:r.logged].copy()
             out["log_y"] = log[1][:r.logged].copy()

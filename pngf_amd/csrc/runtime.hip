@@ -1048,14 +1048,18 @@ int pngf_init_state(pngf_handle* h) {
     if (h->cfg.method == PNGF_METHOD_LOWRANK) {
         size_t freeb = 0, totb = 0;
         HC(h, hipMemGetInfo(&freeb, &totb));
-        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (64u << 20);
+        // keep room for the scratch strips and for one checkpoint copy of W0
+        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (size_t)nf * ld * N * sizeof(zd) + (64u << 20);
         size_t avail = freeb > scratch ? freeb - scratch : 0;
         if (h->cfg.mem_limit > 0) {
             size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
             avail = std::min(avail, lim);
         }
         const size_t per_col = (size_t)nf * 2 * ld * sizeof(zd);  // A_all column + M_all row
-        int maxdef = h->cfg.max_deferred > 0 ? h->cfg.max_deferred : 384;
+        // Folding K deferred columns into W0 costs 8 N^2 K flop; not folding costs 8 N K k more per
+        // later accept, so folding pays only when more than N / k accepts remain: by default the
+        // factors hold N / gmax accepts (memory permitting) and a typical run never folds.
+        int maxdef = h->cfg.max_deferred > 0 ? h->cfg.max_deferred : (N + gmax - 1) / gmax;
         size_t cols = std::min<size_t>((size_t)maxdef * gmax, (size_t)(0.85 * avail) / per_col);
         cols = cols / gmax * gmax;
         if (cols < (size_t)gmax) return fail(h, PNGF_ERR_NOMEM, "no device memory for the deferred factors");
@@ -1578,12 +1582,23 @@ int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp,
             }
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return score[a] < score[b]; });
             scored += ncl;
-            std::vector<int> top(K);
-            for (int i = 0; i < K; i++) top[i] = cl[order[i]];
-            int best;
-            CK(exact(top, F.data(), &best));
-            hits.push_back(best >= 0 ? 1 : 0);
-            if (best >= 0) { CK(commit(top[best], F[best])); continue; }
+            // exact scores in rank order, top_k at a time, until a chunk holds an improving flip
+            // (its best is accepted); a step that runs through every chunk is an exact sweep that
+            // found none: a verified local minimum
+            int best = -1;
+            for (int pos = 0; pos < ncl && best < 0; pos += K) {
+                std::vector<int> chunk;
+                for (int i = pos; i < std::min(ncl, pos + K); i++) chunk.push_back(cl[order[i]]);
+                CK(exact(chunk, F.data(), &best));
+                if (pos == 0) {
+                    hits.push_back(best >= 0 ? 1 : 0);
+                    r->ranked_steps++;
+                    r->ranked_hits += best >= 0 ? 1 : 0;
+                }
+                if (best >= 0) CK(commit(chunk[best], F[best]));
+            }
+            if (best < 0) { r->converged = 1; break; }
+            continue;
         } else {
             hits.push_back(1);  // a fallback step re-arms the window
         }

@@ -76,9 +76,11 @@ def test_dbs_training_log_matches_oracle():
 
 def test_trained_ranker_gains():
     """Ranked DBS with the trained network against (a) exact full sweeps every step
-    (best-improvement DBS, the method without the network) and (b) the same ranked driver
-    with an untrained network: the trained ranking finds an improving flip in its top-K
-    more often, and needs fewer exact evaluations than full sweeps."""
+    (best-improvement DBS: the method without the network) and (b) the same ranked driver
+    with an untrained network.  Every accepted flip is exact and each run ends in an exact
+    local minimum; the trained ranking needs fewer exact evaluations than full sweeps for an
+    objective within 1 %, and finds an improving flip in its first chunk more often than an
+    untrained one."""
     T, freqs = 7, freq_list(3, 28e9, 32e9)
     X, Y = ranker.collect(T, freqs, range(1, 9))
     assert len(X) > 1000
@@ -92,12 +94,10 @@ def test_trained_ranker_gains():
     good = opt.dbs_run_ranked(mlp, norm, top_k=6, window=4, min_hit_rate=0.25)
     opt.restore()
     bad = opt.dbs_run_ranked(untrained, norm, top_k=6, window=4, min_hit_rate=0.25)
-    ncand = T * T
-    # every accepted flip was verified exactly and the runs end in a local minimum
     assert good["converged"] and full["converged"] and bad["converged"]
     assert good["flip_count"] < full["flip_count"]
     assert good["F_final"] <= 1.01 * full["F_final"]
-    assert ranker.hit_rate(good, ncand) > ranker.hit_rate(bad, ncand)
+    assert ranker.hit_rate(good) > ranker.hit_rate(bad)
 
 
 def test_best_order_reaches_a_local_minimum():
