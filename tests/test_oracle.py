@@ -253,3 +253,22 @@ def test_dbs_ends_in_local_minimum():
         t2 = fm.copy()
         t2[t] ^= 1
         assert F_of(t2) >= r["F_final"] * (1 - 1e-12)
+
+
+def test_empty_pattern_closed_form():
+    """P = 0 (every tile and the port neighbourhood void): C = I, PEinc = 0, so S = 0,
+    Eobj = 0 and Zin = -Vinc dx / 2 / (2 dy dz) with Vinc = sum of G over the port block
+    (pngfopt.hpp:584-617, main.cpp:277-279)."""
+    kind, T = O.LAYOUT_DISJOINT, 5
+    G, extra, fb, tm = random_problem(kind, T, 1, 41)
+    o = O.Optimizer(kind, T, fb, ne=1)
+    assert o.init_state(np.zeros(T * T, dtype=np.int32), (2, 2, 1, 2)) == 0
+    port = O.layout_port(kind, T, 2, 2)
+    vinc = G[np.ix_(port, port)].sum()
+    sc = o.scalars()
+    assert np.all(sc["Eobj"] == 0)
+    assert abs(sc["Vinc"] - vinc) < 1e-12 * abs(vinc)
+    assert np.all(o.vec("S") == 0) and np.all(o.vec("PEinc") == 0)
+    zin = -vinc * DX * 0.5 / (2 * DY * DZ)
+    F, s, _ = O.objective(O.OBJ_S11, sc["Eobj"][None, :], np.array([sc["Vinc"]]), None, [30e9], DX, DY, DZ)
+    assert abs(s[0] - (zin - 50) / (zin + 50)) < 1e-12
