@@ -75,29 +75,46 @@ def test_dbs_training_log_matches_oracle():
 
 
 def test_trained_ranker_gains():
-    """Ranked DBS with the trained network against (a) exact full sweeps every step
-    (best-improvement DBS: the method without the network) and (b) the same ranked driver
-    with an untrained network.  Every accepted flip is exact and each run ends in an exact
-    local minimum; the trained ranking needs fewer exact evaluations than full sweeps for an
-    objective within 1 %, and finds an improving flip in its first chunk more often than an
-    untrained one."""
+    """What the trained network gains, on three starts it was not trained on (7x7 tiles, 3
+    frequencies; training tuples logged by plain DBS from 16 other starts):
+    * ranking: along each start's steepest-descent trajectory the network's order correlates
+      with the exact objective changes (mean Spearman rho) where an untrained network's does
+      not, it places more improving flips in its top 6 than a random order would, and it
+      holds the exact best flip in its top 6 more often than the untrained network;
+    * ranked DBS against the method without it (exact full sweeps, best-improvement DBS):
+      fewer exact FP64 evaluations for a final objective within 2 % (mean over the starts).
+    Every accepted flip is an exact evaluation and every run ends in an exact local minimum."""
     T, freqs = 7, freq_list(3, 28e9, 32e9)
-    X, Y = ranker.collect(T, freqs, range(1, 9))
-    assert len(X) > 1000
+    X, Y = ranker.collect(T, freqs, range(1, 17))
+    assert len(X) > 5000
     norm = ranker.fit_norm(X, Y)
     mlp, loss = ranker.train(X, Y, norm, dh=64, epochs=40, batch=256, lr=2e-3)
     untrained = P.MLP(X.shape[1], 64, Y.shape[1], seed=5)
-    opt = ranker.make_optimizer(T, freqs, 11)
-    opt.checkpoint()
-    full = opt.dbs_run(order=P.ORDER_BEST)
-    opt.restore()
-    good = opt.dbs_run_ranked(mlp, norm, top_k=6, window=4, min_hit_rate=0.25)
-    opt.restore()
-    bad = opt.dbs_run_ranked(untrained, norm, top_k=6, window=4, min_hit_rate=0.25)
-    assert good["converged"] and full["converged"] and bad["converged"]
-    assert good["flip_count"] < full["flip_count"]
-    assert good["F_final"] <= 1.01 * full["F_final"]
-    assert ranker.hit_rate(good) > ranker.hit_rate(bad)
+    qg, qb, ev_full, ev_good, F_full, F_good = [], [], 0, 0, [], []
+    for seed in (11, 12, 13):
+        opt = ranker.make_optimizer(T, freqs, seed)
+        opt.checkpoint()
+        qg.append(ranker.ranking_quality(opt, mlp, norm, 6))
+        opt.restore()
+        qb.append(ranker.ranking_quality(opt, untrained, norm, 6))
+        opt.restore()
+        full = opt.dbs_run(order=P.ORDER_BEST)
+        opt.restore()
+        good = opt.dbs_run_ranked(mlp, norm, top_k=6, window=4, min_hit_rate=0.25)
+        assert full["converged"] and good["converged"]
+        assert np.all(np.diff(good["acc_F"]) < 0)
+        ev_full += full["flip_count"]
+        ev_good += good["flip_count"]
+        F_full.append(full["F_final"])
+        F_good.append(good["F_final"])
+        opt.close()
+    rho_g = np.mean([q["rho"] for q in qg])
+    rho_b = np.mean([q["rho"] for q in qb])
+    assert rho_g > 0.15 and rho_g > rho_b + 0.15
+    assert sum(q["improving_in_top_k"] for q in qg) > sum(q["random_expectation"] for q in qg)
+    assert sum(q["best_in_top_k"] for q in qg) > sum(q["best_in_top_k"] for q in qb)
+    assert ev_good < 0.85 * ev_full
+    assert np.mean(F_good) <= 1.02 * np.mean(F_full)
 
 
 def test_best_order_reaches_a_local_minimum():
