@@ -261,3 +261,39 @@ def test_asymmetric_g_forms_both_strips():
     assert g["accepts"] == o["accepts"] and g["accepts"] > 0 and g["flushes"] > 0
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
+def test_full_size_state_matches_fresh_init():
+    """Bench workload size (21x21 tiles, N = 10590, 9 frequencies): after 1500 evaluations of
+    DBS the maintained state (W0 plus deferred factors, symmetric accept) gives the same Eobj
+    and objective as a fresh init_state from the final tile map (C inverted again), and a flip
+    accepted twice returns Eobj to where it was.  cond(C) is ~1e5 here, so the bar is RTOL_ILL."""
+    from pngf_amd.problem import WORKLOADS
+    wl = WORKLOADS["antenna21x21_9f"]
+    T, freqs = wl["T"], wl["freqs"]
+
+    def make(tm):
+        o = P.Optimizer(P.LAYOUT_DISJOINT, T, num_freq=len(freqs), method=P.METHOD_LOWRANK)
+        for f, hz in enumerate(freqs):
+            o.set_frequency(f, hz)
+            o.synth_g(f)
+        o.set_tile_map(tm)
+        o.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+        o.init_state()
+        return o
+
+    a = make(initial_pattern(T, 0))
+    assert a.symmetry()[0]
+    r = a.dbs_run(max_evals=1500)
+    assert r["accepts"] > 20
+    sa = a.state()
+    b = make(a.tile_map())
+    sb = b.state()
+    b.close()
+    assert rel(sa["Eobj"], sb["Eobj"]) < RTOL_ILL
+    assert abs(sa["F"] - sb["F"]) < RTOL_ILL * sb["F"]
+    t = int(r["acc_tiles"][0])
+    a.accept([t])
+    a.accept([t])
+    assert rel(a.state()["Eobj"], sa["Eobj"]) < RTOL_ILL
+    a.close()
