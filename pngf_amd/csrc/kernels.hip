@@ -238,7 +238,8 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nth = blockDim.x;
     const int gmax = d.gmax;
     zd* dpe = smz;                                // [gmax]
-    int* t = (int*)(dpe + gmax);
+    zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
+    int* t = (int*)(wdg + gmax);
     int* sg = t + gmax;
     int* lp = sg + gmax;
     int* piv = lp + gmax;
@@ -265,16 +266,31 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
         dpe[c] = sg[c] > 0 ? zmk(-e.x, -e.y) : e;
     }
     __syncthreads();
+    // block gather (independent loads, several in flight per thread); the diagonal of W
+    // is kept apart (wdg) so the rhs below reads W exactly from LDS
+#pragma unroll 4
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
         zd w = dblk(D, gmax, lp[r], lp[c], d.sym);
         zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
-        if (r == c) val.x += 1.0;
+        if (r == c) {
+            wdg[r] = w;
+            val.x += 1.0;
+        }
         a[r * lda + c] = val;
     }
+    __syncthreads();
+    // rhs_r = q_t_r + sum_c W[t_r][t_c] dPE_c, with W[t_r][t_c] = s_c a[r][c] off the diagonal
     for (int r = tid; r < k; r += nth) {
         zd acc = q[t[r]];
-        for (int c = 0; c < k; c++) acc = zfma(dblk(D, gmax, lp[r], lp[c], d.sym), dpe[c], acc);
+        for (int c = 0; c < k; c++) {
+            zd w = wdg[r];
+            if (c != r) {
+                const zd v = a[r * lda + c];
+                w = sg[c] > 0 ? v : zmk(-v.x, -v.y);
+            }
+            acc = zfma(w, dpe[c], acc);
+        }
         a[r * lda + k] = acc;
     }
     __syncthreads();
@@ -310,8 +326,8 @@ void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
                     int32_t* out_info, zd* gscratch) {
     dim3 grid(nb, d.nf);
-    int nth = d.gmax <= 32 ? 64 : 256;
-    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
+    int nth = d.gmax <= 32 ? 128 : 256;
+    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)2 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
                        out_info, gscratch);
 }
@@ -596,16 +612,30 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     zd* st = acc + (size_t)f * acc_stride;
     for (int c = tid; c < k; c += nth) { zd e = einc[tt[c]]; dpe[c] = ss[c] > 0 ? zmk(-e.x, -e.y) : e; }
     __syncthreads();
+    zd* wdg = tmp;  // [k] W[t_r][t_r]
+#pragma unroll 4
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
         zd w = dblk(D, gmax, lpp[r], lpp[c], d.sym);
         zd val = ss[c] > 0 ? w : zmk(-w.x, -w.y);
-        if (r == c) val.x += 1.0;
+        if (r == c) {
+            wdg[r] = w;
+            val.x += 1.0;
+        }
         a[r * lda + c] = val;
     }
+    __syncthreads();
+    // rhs from the LDS copy of the block (W[t_r][t_c] = s_c a[r][c] off the diagonal)
     for (int r = tid; r < k; r += nth) {
         zd s = q[tt[r]];
-        for (int c = 0; c < k; c++) s = zfma(dblk(D, gmax, lpp[r], lpp[c], d.sym), dpe[c], s);
+        for (int c = 0; c < k; c++) {
+            zd w = wdg[r];
+            if (c != r) {
+                const zd v = a[r * lda + c];
+                w = ss[c] > 0 ? v : zmk(-v.x, -v.y);
+            }
+            s = zfma(w, dpe[c], s);
+        }
         a[r * lda + k] = s;
     }
     if (gj)
