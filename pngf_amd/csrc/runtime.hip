@@ -113,6 +113,8 @@ struct pngf_handle {
     size_t partial_elems = 0;
     int K = 0, Kmax = 0, flushes = 0;
     int skv = 0;                     // tile-shape variant of the skinny products (autotuned)
+    hipStream_t st2 = nullptr;       // side stream: the capacitance solve of an accept
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool sym = false;                // LOWRANK accept takes W[t,:] = W[:,t]^T (G symmetric)
     double asym = 0.0;               // max over frequencies of max|G_ij - G_ji| / max|G_ij|
     unsigned long long* d_asym = nullptr;
@@ -192,18 +194,18 @@ void dfree(pngf_handle* h, void* p) {
 }
 
 // ------------------------------------------------------------- profiling ---
-void prof_begin(pngf_handle* h, Timed& t) {
+void prof_begin(pngf_handle* h, Timed& t, hipStream_t s = nullptr) {
     if (!h->prof) return;
     if (h->pool.size() < 2) {
         for (int i = 0; i < 16; i++) { hipEvent_t e; if (hipEventCreate(&e) == hipSuccess) h->pool.push_back(e); }
     }
     t.a = h->pool.back(); h->pool.pop_back();
     t.b = h->pool.back(); h->pool.pop_back();
-    (void)hipEventRecord(t.a, h->st);
+    (void)hipEventRecord(t.a, s ? s : h->st);
 }
-void prof_end(pngf_handle* h, Timed& t, int which, double bytes, double flops) {
+void prof_end(pngf_handle* h, Timed& t, int which, double bytes, double flops, hipStream_t s = nullptr) {
     if (!h->prof) return;
-    (void)hipEventRecord(t.b, h->st);
+    (void)hipEventRecord(t.b, s ? s : h->st);
     t.which = which; t.bytes = bytes; t.flops = flops;
     h->pending.push_back(t);
 }
@@ -594,9 +596,17 @@ int accept_group(pngf_handle* h, int g) {
     if (k > 0 && h->cfg.method == PNGF_METHOD_LOWRANK) {
         if (h->K + k > h->Kmax) CK(flush_factors(h));
         const int K = h->K;
+        // the capacitance solve (one workgroup per frequency) runs on the side stream while the
+        // strips are formed; the epilogue waits for both
+        HC(h, hipEventRecord(h->ev_fork, st));
+        HC(h, hipStreamWaitEvent(h->st2, h->ev_fork, 0));
+        Timed tp{};
+        prof_begin(h, tp, h->st2);
+        launch_lr_accept_prep(h->st2, d, h->vecs, g, dt, ds, dlp, k, h->d_acc, h->acc_stride, h->d_info, h->d_lu);
+        prof_end(h, tp, PNGF_K_ACC_SMALL, 0, 0, h->st2);
+        HC(h, hipEventRecord(h->ev_join, h->st2));
         Timed tm{};
         prof_begin(h, tm);
-        launch_lr_accept_prep(st, d, h->vecs, g, dt, ds, dlp, k, h->d_acc, h->acc_stride, h->d_info, h->d_lu);
         launch_lr_gather(st, d, h->tabs, dt, k, K, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride);
         prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
         const double N = d.N, kd = k, Kd = K;
@@ -610,6 +620,7 @@ int accept_group(pngf_handle* h, int g) {
             launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv);
             prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
         }
+        HC(h, hipStreamWaitEvent(st, h->ev_join, 0));
         Timed t3{};
         prof_begin(h, t3);
         // M_new = diag(s) cap^-1 W[t,:]; W[t,:]^T is the row strip Bt, or, for symmetric W, the
@@ -787,6 +798,9 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     *out = h;
     HC(h, hipSetDevice(c.device));
     HC(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    HC(h, hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking));
+    HC(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    HC(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     RB(h, rocblas_create_handle(&h->rb));
     RB(h, rocblas_set_stream(h->rb, h->st));
     RB(h, rocblas_set_pointer_mode(h->rb, rocblas_pointer_mode_host));
@@ -850,6 +864,10 @@ int pngf_destroy(pngf_handle* h) {
     if (h->h_pin_i) (void)hipHostFree(h->h_pin_i);
     if (h->h_pin_d) (void)hipHostFree(h->h_pin_d);
     if (h->rb) rocblas_destroy_handle(h->rb);
+    if (h->st2) (void)hipStreamSynchronize(h->st2);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->st2) (void)hipStreamDestroy(h->st2);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
     return PNGF_OK;
