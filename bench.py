@@ -34,10 +34,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO):
+def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0):
     T, freqs = wl["T"], wl["freqs"]
     opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred,
-                      symmetry=symmetry)
+                      symmetry=symmetry, tile_variant=tile_variant)
     for f, hz in enumerate(freqs):
         opt.set_frequency(f, hz)
         opt.synth_g(f)
@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--tiles", type=int, default=0, help="override the workload's tile count")
     ap.add_argument("--nfreq", type=int, default=0, help="override the workload's frequency count")
+    ap.add_argument("--tile-variant", type=int, default=0,
+                    help="skinny-product tile shape: 0 = autotune at init, v+1 = use variant v")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
@@ -194,7 +196,8 @@ def main():
     method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
 
     log("[bench] init %s method=%s" % (wl_name, args.method))
-    opt, init_s = make(wl, method, args.max_deferred, P.SYM_AUTO if args.symmetry == "auto" else P.SYM_OFF)
+    opt, init_s = make(wl, method, args.max_deferred, P.SYM_AUTO if args.symmetry == "auto" else P.SYM_OFF,
+                       args.tile_variant)
     opt.checkpoint()
     tune = opt.autotune_result() if method == P.METHOD_LOWRANK else None
     sym_used, sym_asym = opt.symmetry()

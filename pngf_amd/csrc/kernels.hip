@@ -838,7 +838,9 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     }
 }
 
-// Tile-shape variants offered to the autotuner, per column-tile count CT.
+// Tile-shape variants offered to the autotuner, per column-tile count CT.  All of them issue
+// the MFMAs of an output tile in the same inner-index order into one accumulator, so every
+// variant gives bitwise the same result and the autotuner's choice cannot change a DBS run.
 template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                            const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
@@ -855,7 +857,7 @@ static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb,
     launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride)
     if (k <= 24) {
         switch (variant) {
-            case 1: PNGF_SKV(3, 64, 4, 8, true); break;
+            case 1: PNGF_SKV(3, 32, 2, 4, false); break;
             case 2: PNGF_SKV(3, 64, 2, 8, false); break;
             case 3: PNGF_SKV(3, 32, 4, 8, false); break;
             case 4: PNGF_SKV(3, 64, 4, 8, false); break;
