@@ -62,6 +62,12 @@ typedef struct { double re, im; } pngf_c128;
 #define PNGF_SYM_AUTO 0 /* use it when every G is symmetric to 1e-14 of its largest entry */
 #define PNGF_SYM_OFF  1 /* always form both strips */
 
+/* who drives the raster first-improvement loop (same accepted sequence either way) */
+#define PNGF_LOOP_AUTO 0 /* the device: batch, selection and accept bookkeeping run in kernels and the
+                            host synchronises once per 16 iterations (LOWRANK, no training log, no
+                            profiling, flips of at most 32 components); otherwise the host */
+#define PNGF_LOOP_HOST 1 /* the host: one synchronisation per scored batch */
+
 /* DBS visit order */
 #define PNGF_DRIVER_REF    0 /* MagicHat order, mirror-symmetric 2/4-tile flips (main.cpp:338-459) */
 #define PNGF_DRIVER_RASTER 1 /* raster order, single-tile flips, cyclic until one full cycle
@@ -176,6 +182,8 @@ typedef struct {
                               the loop (main.cpp:195 draws (T/2+1)^2 bits for INIT_RANDOM; 0 for
                               the default INIT_EMPTY) */
     int32_t order;         /* PNGF_ORDER_FIRST (reference semantics, main.cpp:429) or PNGF_ORDER_BEST */
+    int32_t loop;          /* PNGF_LOOP_AUTO or PNGF_LOOP_HOST */
+    int32_t reserved;
     /* Optional training log (LOWRANK): for every scored candidate, its ranker inputs
        (pngf_ranker_dims din floats) and the exact change of |S11_f|^2 at every frequency
        (dout floats), appended until log_cap rows. */

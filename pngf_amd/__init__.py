@@ -20,6 +20,7 @@ METHOD_PLAIN, METHOD_LOWRANK = 0, 1
 DRIVER_REF, DRIVER_RASTER = 0, 1
 ORDER_FIRST, ORDER_BEST = 0, 1
 SYM_AUTO, SYM_OFF = 0, 1
+LOOP_AUTO, LOOP_HOST = 0, 1
 K_EVAL, K_COLBLK, K_ROWBLK, K_ACC_SMALL, K_VCINV, K_RANKK = range(6)
 K_NAMES = ["eval", "colblk", "rowblk", "acc_small", "vcinv", "rankk"]
 
@@ -61,8 +62,8 @@ class DbsParams(ct.Structure):
     _fields_ = [("max_evals", ct.c_int64), ("seed", ct.c_int32), ("max_accepts", ct.c_int32),
                 ("acc_tiles", ct.c_void_p), ("acc_eval", ct.c_void_p), ("acc_F", ct.c_void_p),
                 ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32),
-                ("order", ct.c_int32), ("log_x", ct.c_void_p), ("log_y", ct.c_void_p),
-                ("log_cap", ct.c_int64)]
+                ("order", ct.c_int32), ("loop", ct.c_int32), ("reserved", ct.c_int32),
+                ("log_x", ct.c_void_p), ("log_y", ct.c_void_p), ("log_cap", ct.c_int64)]
 
 
 class DbsResult(ct.Structure):
@@ -291,7 +292,8 @@ class Optimizer:
         return out
 
     # ---- DBS loop (main.cpp:338-459)
-    def _dbs_params(self, max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap, order=0):
+    def _dbs_params(self, max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap, order=0,
+                    loop=0):
         acc = (np.zeros(max_accepts, dtype=np.int32), np.zeros(max_accepts, dtype=np.int64),
                np.zeros(max_accepts))
         log = None
@@ -299,7 +301,7 @@ class Optimizer:
             din, dout = self.ranker_dims()
             log = (np.zeros((log_cap, din), dtype=np.float32), np.zeros((log_cap, dout), dtype=np.float32))
         p = DbsParams(max_evals, seed, max_accepts, acc[0].ctypes.data, acc[1].ctypes.data, acc[2].ctypes.data,
-                      batch_min, batch_max, rng_discard, order, log[0].ctypes.data if log else None,
+                      batch_min, batch_max, rng_discard, order, loop, 0, log[0].ctypes.data if log else None,
                       log[1].ctypes.data if log else None, log_cap)
         return p, acc, log
 
@@ -318,9 +320,9 @@ class Optimizer:
         return out
 
     def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0,
-                rng_discard=0, log_cap=0, order=ORDER_FIRST):
+                rng_discard=0, log_cap=0, order=ORDER_FIRST, loop=LOOP_AUTO):
         p, acc, log = self._dbs_params(max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap,
-                                       order)
+                                       order, loop)
         r = DbsResult()
         self._ck(lib().pngf_dbs_run(self.h, ct.byref(p), ct.byref(r)), "dbs_run")
         return self._dbs_result(r, acc, log)

@@ -35,29 +35,54 @@ struct Dims {
 // gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
 constexpr int kLdsGmax = 32;
 
+// Device-resident accept record for the device-driven DBS loop: when a kernel gets a non-null
+// `dyn`, it reads k = dyn[0], K = dyn[1] and the group's tile count dyn[3] from device memory
+// and does nothing unless dyn[2] (an accept is pending) is set; batch-sized kernels likewise
+// read the batch size from `nb_dev`.  Null pointers mean the by-value arguments apply.
+enum { kDynK = 0, kDynKK = 1, kDynActive = 2, kDynNt = 3 };
+
+// State of the device-driven raster DBS loop (first improvement), in device memory.
+struct LoopState {
+    int64_t flip_count, spec, batches, max_evals;
+    double F_best, F_sel;
+    int32_t pos, nonimp, bs, nb, sel, done, nacc, ncl, bmin, bmax, acc_cap, status, g, pad;
+    int32_t dyn[4];
+};
+enum { kLoopOk = 0, kLoopFull = 1, kLoopSingular = 2 };
+
+// One iteration's host-free steps (see runtime.hip, dbs_run_device).
+void launch_loop_batch(hipStream_t st, LoopState* ls, const int32_t* cl, int32_t* batch);
+void launch_loop_after_select(hipStream_t st, const Dims& d, LoopState* ls, GroupTab gt, const int32_t* tile,
+                              const int8_t* P, const int32_t* owners, const int32_t* batch, const int32_t* info,
+                              int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
+                              int64_t* acc_eval, double* acc_F);
+
 // ---- eval
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
-                    int32_t* out_info, zd* gscratch);
+                    int32_t* out_info, zd* gscratch, const int32_t* nb_dev = nullptr);
 void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
                        const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
                        int nb, int nsplit, zd* partial, zd* out_eobj, int32_t* out_info,
                        zd* store /* null or [nf][acc_stride] */, int acc_stride, zd* gscratch);
 void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, const zd* eobj, int nb,
-                      double* F_out, const double* F_best, int32_t* sel, double* F_sel);
+                      double* F_out, const double* F_best, int32_t* sel, double* F_sel,
+                      const int32_t* nb_dev = nullptr);
 
 // ---- LOWRANK accept (all frequencies in grid.y)
 // acc buffer per frequency (zd): capinv [0,k^2), scratch [k^2,2k^2), y [2k^2,+k),
 // w = dPE - s o y [2k^2+k,+k), newEobj [2k^2+2k,+4), Rt = R[:,t] [2k^2+2k+4,+ne*k)
 void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
                            const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
-                           int32_t* info, zd* gscratch);
+                           int32_t* info, zd* gscratch, const int32_t* dyn = nullptr,
+                           const int32_t* g_dev = nullptr);
 void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      zd* Mc, zd* Ar, int small_stride);
+                      zd* Mc, zd* Ar, int small_stride, const int32_t* dyn = nullptr);
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Mc, int small_stride, int variant);
+                      const zd* Mc, int small_stride, int variant, const int32_t* dyn = nullptr);
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant);
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant,
+                      const int32_t* dyn = nullptr);
 // number of tile-shape variants of the skinny products for flips of k components
 int skinny_variants(int k);
 void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
@@ -69,11 +94,11 @@ void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const Dev
 // fused apply_capinv + lr_small + lr_blocks for gmax <= 32
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
-                      int K);
+                      int K, const int32_t* dyn = nullptr);
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       int k, int K);
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles);
+                        const int32_t* gtiles, int ntiles, const int32_t* dyn = nullptr);
 
 // ---- PLAIN accept
 void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, zd* V,

@@ -233,9 +233,10 @@ static size_t lds_matrix_bytes(int gmax, const zd* gscratch) {
 __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
                                const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
                                const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
-                               int32_t* __restrict__ out_info, zd* gscratch) {
+                               int32_t* __restrict__ out_info, zd* gscratch, const int32_t* nb_dev) {
     extern __shared__ zd smz[];
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nth = blockDim.x;
+    if (nb_dev && b >= *nb_dev) return;
     const int gmax = d.gmax;
     zd* dpe = smz;                                // [gmax]
     zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
@@ -324,12 +325,12 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
 
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
-                    int32_t* out_info, zd* gscratch) {
+                    int32_t* out_info, zd* gscratch, const int32_t* nb_dev) {
     dim3 grid(nb, d.nf);
     int nth = d.gmax <= 32 ? 128 : 256;
     size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)2 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
-                       out_info, gscratch);
+                       out_info, gscratch, nb_dev);
 }
 
 // ------------------------------------------------------------ PLAIN eval -----
@@ -552,16 +553,18 @@ void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const De
 // One thread per candidate: objective_of (common.h); then the first candidate in
 // batch order with F < F_best.
 __global__ void objective_kernel(ObjParams op, const FreqConst* __restrict__ fc, const zd* __restrict__ eobj,
-                                 int nb, double* __restrict__ F_out) {
+                                 int nb, double* __restrict__ F_out, const int32_t* nb_dev) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nb_dev) nb = *nb_dev;
     if (b >= nb) return;
     F_out[b] = objective_of(op, fc, eobj + (size_t)b * op.nf * op.ne);
 }
 
 __global__ void select_kernel(const double* __restrict__ F, int nb, const double* __restrict__ F_best,
-                              int32_t* sel, double* F_sel) {
+                              int32_t* sel, double* F_sel, const int32_t* nb_dev) {
     __shared__ int red[256];
     const int tid = threadIdx.x;
+    if (nb_dev) nb = *nb_dev;
     const double fb = *F_best;
     int best = 0x7fffffff;
     for (int b = tid; b < nb; b += 256)
@@ -580,9 +583,9 @@ __global__ void select_kernel(const double* __restrict__ F, int nb, const double
 }
 
 void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, const zd* eobj, int nb,
-                      double* F_out, const double* F_best, int32_t* sel, double* F_sel) {
-    hipLaunchKernelGGL(objective_kernel, dim3((nb + 127) / 128), dim3(128), 0, st, op, fc, eobj, nb, F_out);
-    if (sel) hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, st, F_out, nb, F_best, sel, F_sel);
+                      double* F_out, const double* F_best, int32_t* sel, double* F_sel, const int32_t* nb_dev) {
+    hipLaunchKernelGGL(objective_kernel, dim3((nb + 127) / 128), dim3(128), 0, st, op, fc, eobj, nb, F_out, nb_dev);
+    if (sel) hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, st, F_out, nb, F_best, sel, F_sel, nb_dev);
 }
 
 // ------------------------------------------------------- LOWRANK accept -----
@@ -595,8 +598,14 @@ void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, 
 //   [2k*k+2k+4, +ne*k)  Rt = R[:, t] (snapshot)
 __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restrict__ tt,
                                const int32_t* __restrict__ ss, const int32_t* __restrict__ lpp, int k, zd* acc,
-                               int acc_stride, int32_t* info_out, zd* gscratch) {
+                               int acc_stride, int32_t* info_out, zd* gscratch, const int32_t* dyn,
+                               const int32_t* g_dev) {
     extern __shared__ char smem[];
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        g = *g_dev;
+    }
     const int f = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int gmax = d.gmax, ne = d.ne;
     int* piv = (int*)smem;
@@ -691,16 +700,21 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
 
 void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
                            const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
-                           int32_t* info, zd* gscratch) {
+                           int32_t* info, zd* gscratch, const int32_t* dyn, const int32_t* g_dev) {
     size_t sm = 100 * sizeof(int) + 192 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(lr_prep_kernel, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
-                       acc_stride, info, gscratch);
+                       acc_stride, info, gscratch, dyn, g_dev);
 }
 
 // Mc[K'][c] = M_all[K'][t_c] (K x k) and, unless Ar is null, Ar[c][K'] = A_all[t_c][K'] (k x K).
 __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, int K, zd* Mc,
-                                 zd* Ar, int small_stride) {
+                                 zd* Ar, int small_stride, const int32_t* dyn) {
     const int f = blockIdx.y;
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
     const zd* M = tb.Mall[f];
     const zd* A = tb.Aall[f];
     zd* mc = Mc + (size_t)f * small_stride;
@@ -715,11 +729,12 @@ __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__
 }
 
 void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      zd* Mc, zd* Ar, int small_stride) {
-    if (K == 0) return;
-    int blocks = min(1024, (K * k + 255) / 256);
+                      zd* Mc, zd* Ar, int small_stride, const int32_t* dyn) {
+    if (!dyn && K == 0) return;
+    // grid-stride loop: with a device-resident K the grid is the largest one
+    int blocks = dyn ? 1024 : min(1024, (K * k + 255) / 256);
     hipLaunchKernelGGL(lr_gather_kernel, dim3(blocks, d.nf), dim3(256), 0, st, d, tb, t, k, K, Mc, Ar,
-                       small_stride);
+                       small_stride, dyn);
 }
 
 // Skinny products over the deferred factors (k <= 96 columns), on FP64 MFMA
@@ -755,7 +770,13 @@ __device__ inline zd ldg(const zd* p, size_t i) {
 template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
                                                                     int k, int K, const zd* __restrict__ small,
-                                                                    int small_stride, zd* Bt, size_t bt_stride) {
+                                                                    int small_stride, zd* Bt, size_t bt_stride,
+                                                                    const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
     constexpr int BW = CT * 8;   // complex columns held (>= k)
     constexpr int NJ = CT * 16;  // real columns
     constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64, NA = SPLIT ? 2 : 1;
@@ -843,18 +864,19 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 // variant gives bitwise the same result and the autotuner's choice cannot change a DBS run.
 template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                           const zd* small, int small_stride, zd* Bt, size_t bt_stride) {
+                           const zd* small, int small_stride, zd* Bt, size_t bt_stride, const int32_t* dyn) {
     const int rows = 16 * WAVES;
     dim3 grid((d.N + rows - 1) / rows, d.nf);
     hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, SPLIT>), grid, dim3(WAVES * 64), 0, st, d, tb,
-                       t, k, K, small, small_stride, Bt, bt_stride);
+                       t, k, K, small, small_stride, Bt, bt_stride, dyn);
 }
 
 template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                               const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant) {
+                               const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant,
+                               const int32_t* dyn) {
 #define PNGF_SKV(CT, KC, DEPTH, W, SPLIT) \
-    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride)
+    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
     if (k <= 24) {
         switch (variant) {
             case 1: PNGF_SKV(3, 32, 2, 4, false); break;
@@ -877,12 +899,12 @@ static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb,
 int skinny_variants(int k) { return k <= 24 ? 6 : 2; }
 
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Mc, int small_stride, int variant) {
-    launch_skinny_mfma<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0, variant);
+                      const zd* Mc, int small_stride, int variant, const int32_t* dyn) {
+    launch_skinny_mfma<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0, variant, dyn);
 }
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant) {
-    launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant);
+                      const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant, const int32_t* dyn) {
+    launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant, dyn);
 }
 
 // out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb;
@@ -1022,7 +1044,13 @@ void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const De
 __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
                                                         const zd* __restrict__ acc, int acc_stride,
                                                         const zd* __restrict__ Bt, size_t bt_stride,
-                                                        const int32_t* __restrict__ s, int k, int K, int nchunk) {
+                                                        const int32_t* __restrict__ s, int k, int K, int nchunk,
+                                                        const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
     __shared__ zd Xs[32][33];
     __shared__ zd As[32][33];
     __shared__ zd Ss[32][33];  // row strip, then M_new[c][row]
@@ -1106,14 +1134,19 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
 
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
-                      int K) {
+                      int K, const int32_t* dyn) {
     const int nchunk = (d.N + 31) / 32;
     hipLaunchKernelGGL(lr_finish_kernel, dim3(nchunk + d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
-                       acc_stride, Bt, bt_stride, s, k, K, nchunk);
+                       acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn);
 }
 
 __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                                   const int32_t* gtiles, int ntiles) {
+                                   const int32_t* gtiles, int ntiles, const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        ntiles = dyn[kDynNt];
+    }
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         for (int c = 0; c < k; c++) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
         for (int i = 0; i < ntiles; i++) tile[gtiles[i]] ^= 1;
@@ -1121,8 +1154,114 @@ __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, c
 }
 
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles) {
-    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(64), 0, st, P, tile, t, s, k, gtiles, ntiles);
+                        const int32_t* gtiles, int ntiles, const int32_t* dyn) {
+    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(64), 0, st, P, tile, t, s, k, gtiles, ntiles, dyn);
+}
+
+// ---------------------------------------------------- device-driven loop -----
+// One raster first-improvement iteration without a host round trip: this kernel commits the
+// previous iteration's accept to the factor count and writes the next batch (the candidates
+// after the current position, cyclically), the batch-sized kernels read its size from
+// ls->nb, select_kernel writes ls->sel / ls->F_sel, and loop_after_select_kernel turns the
+// selection into the accept record the accept kernels read (or into the non-improving
+// bookkeeping).  Same rules as the host loop in pngf_dbs_run.
+__global__ void loop_batch_kernel(LoopState* ls, const int32_t* __restrict__ cl, int32_t* batch) {
+    __shared__ int s_B, s_pos, s_ncl;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        if (ls->dyn[kDynActive]) {
+            ls->dyn[kDynKK] += ls->dyn[kDynK];
+            ls->dyn[kDynActive] = 0;
+        }
+        int64_t B = 0;
+        if (!ls->done) {
+            B = ls->ncl - ls->nonimp;
+            if (ls->max_evals > 0) B = min(B, ls->max_evals - ls->flip_count);
+            B = min(B, (int64_t)min(ls->bs, ls->bmax));
+            B = max(B, (int64_t)0);
+        }
+        ls->nb = (int)B;
+        s_B = (int)B;
+        s_pos = ls->pos;
+        s_ncl = ls->ncl;
+    }
+    __syncthreads();
+    for (int i = tid; i < s_B; i += blockDim.x) batch[i] = cl[(s_pos + i) % s_ncl];
+}
+
+void launch_loop_batch(hipStream_t st, LoopState* ls, const int32_t* cl, int32_t* batch) {
+    hipLaunchKernelGGL(loop_batch_kernel, dim3(1), dim3(256), 0, st, ls, cl, batch);
+}
+
+// One wave (64 threads).
+__global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, const int32_t* __restrict__ tile,
+                                         const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                                         const int32_t* __restrict__ batch, const int32_t* __restrict__ info,
+                                         int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
+                                         int64_t* acc_eval, double* acc_F) {
+    const int tid = threadIdx.x;
+    if (ls->done) {
+        if (tid == 0) ls->dyn[kDynActive] = 0;
+        return;
+    }
+    const int nb = ls->nb, sel = ls->sel;
+    // a singular capacitance matrix in the batch ends the run with an error, as in the host loop
+    bool bad = false;
+    for (int i = tid; i < nb * d.nf; i += 64) bad |= info[i] != 0;
+    if (__ballot(bad)) {
+        if (tid == 0) {
+            ls->status = kLoopSingular;
+            ls->done = 1;
+            ls->dyn[kDynActive] = 0;
+        }
+        return;
+    }
+    if (sel >= 0) {
+        const int g = batch[sel];
+        const int k = change_set_wave(gt, g, tile, P, owners, t, s, lp);
+        if (tid == 0) {
+            if (ls->nacc >= ls->acc_cap) {
+                ls->status = kLoopFull;
+                ls->done = 1;
+                ls->dyn[kDynActive] = 0;
+                return;
+            }
+            acc_group[ls->nacc] = g;
+            acc_eval[ls->nacc] = ls->flip_count + sel;
+            acc_F[ls->nacc] = ls->F_sel;
+            ls->nacc++;
+            ls->flip_count += sel + 1;
+            ls->spec += nb - sel - 1;
+            ls->F_best = ls->F_sel;
+            ls->pos = (ls->pos + sel + 1) % ls->ncl;
+            ls->nonimp = 0;
+            ls->bs = ls->bmin;
+            const int nt = gt.ntiles[g];
+            for (int i = 0; i < nt; i++) gtiles[i] = gt.tiles[4 * g + i];
+            ls->g = g;
+            ls->dyn[kDynK] = k;
+            ls->dyn[kDynNt] = nt;
+            ls->dyn[kDynActive] = 1;
+        }
+    } else if (tid == 0) {
+        ls->flip_count += nb;
+        ls->nonimp += nb;
+        ls->pos = (int)((ls->pos + nb) % ls->ncl);
+        ls->bs = min(2 * ls->bs, 1 << 30);
+        ls->dyn[kDynActive] = 0;
+    }
+    if (tid == 0) {
+        ls->batches += nb > 0 ? 1 : 0;
+        if (ls->nonimp >= ls->ncl || (ls->max_evals > 0 && ls->flip_count >= ls->max_evals)) ls->done = 1;
+    }
+}
+
+void launch_loop_after_select(hipStream_t st, const Dims& d, LoopState* ls, GroupTab gt, const int32_t* tile,
+                              const int8_t* P, const int32_t* owners, const int32_t* batch, const int32_t* info,
+                              int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
+                              int64_t* acc_eval, double* acc_F) {
+    hipLaunchKernelGGL(loop_after_select_kernel, dim3(1), dim3(64), 0, st, d, ls, gt, tile, P, owners, batch, info,
+                       t, s, lp, gtiles, acc_group, acc_eval, acc_F);
 }
 
 // --------------------------------------------------------- PLAIN accept -----

@@ -122,6 +122,13 @@ struct pngf_handle {
     // pinned host staging
     int32_t* h_pin_i = nullptr;   // batch ids / small ints
     double* h_pin_d = nullptr;
+    // device-driven raster loop (PNGF_LOOP_AUTO)
+    LoopState* d_loop = nullptr;
+    LoopState* h_loop = nullptr;  // pinned
+    int32_t *d_cl = nullptr, *d_accg = nullptr;
+    int64_t* d_acce = nullptr;
+    double* d_accF = nullptr;
+    int acc_alloc = 0;
     // profiling
     bool prof = false;
     std::vector<Timed> pending;
@@ -863,6 +870,7 @@ int pngf_destroy(pngf_handle* h) {
     for (auto& t : h->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     if (h->h_pin_i) (void)hipHostFree(h->h_pin_i);
     if (h->h_pin_d) (void)hipHostFree(h->h_pin_d);
+    if (h->h_loop) (void)hipHostFree(h->h_loop);
     if (h->rb) rocblas_destroy_handle(h->rb);
     if (h->st2) (void)hipStreamSynchronize(h->st2);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
@@ -1336,6 +1344,149 @@ int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out) {
     return PNGF_OK;
 }
 
+// ------------------------------------------------ device-driven raster loop ---
+// The raster first-improvement iteration of pngf_dbs_run with batch construction, selection
+// and the accept bookkeeping in kernels (LoopState in device memory; the accept kernels read
+// k, K and whether an accept is pending from it).  The host enqueues up to kLoopChunk
+// iterations, each of which accepts at most one flip, and synchronises once per chunk; the
+// factor capacity is checked (and folded) between chunks.  Same batches, same selection and
+// the same kernels as the host loop, so the accepted sequence is identical.
+constexpr int kLoopChunk = 16;
+
+bool device_loop_ok(const pngf_handle* h, const pngf_dbs_params* p) {
+    return p->loop == PNGF_LOOP_AUTO && h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_FIRST &&
+           h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !h->prof && h->d.gmax <= kLdsGmax;
+}
+
+int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best, int bmin, int bmax) {
+    if (!h->d_loop) {
+        CK(dalloc_t(h, &h->d_loop, 1));
+        HC(h, hipHostMalloc((void**)&h->h_loop, sizeof(LoopState), hipHostMallocDefault));
+        std::vector<int32_t> cl;
+        for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
+        CK(dalloc_t(h, &h->d_cl, std::max<size_t>(cl.size(), 1)));
+        HC(h, hipMemcpy(h->d_cl, cl.data(), cl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    const int cap = std::max(1, p->max_accepts);
+    if (cap > h->acc_alloc) {
+        dfree(h, h->d_accg);
+        dfree(h, h->d_acce);
+        dfree(h, h->d_accF);
+        h->d_accg = nullptr;
+        h->d_acce = nullptr;
+        h->d_accF = nullptr;
+        CK(dalloc_t(h, &h->d_accg, cap));
+        CK(dalloc_t(h, &h->d_acce, cap));
+        CK(dalloc_t(h, &h->d_accF, cap));
+        h->acc_alloc = cap;
+    }
+    LoopState& ls = *h->h_loop;
+    std::memset(&ls, 0, sizeof(ls));
+    ls.max_evals = p->max_evals;
+    ls.F_best = F_best;
+    ls.F_sel = F_best;
+    ls.bs = bmin;
+    ls.bmin = bmin;
+    ls.bmax = bmax;
+    ls.acc_cap = p->max_accepts;
+    int ncl = 0;
+    for (int g = 0; g < h->ngroups_drv; g++) ncl += h->grp.ntiles[g] > 0 ? 1 : 0;
+    ls.ncl = ncl;
+    ls.sel = -1;
+    ls.dyn[kDynKK] = h->K;
+    HC(h, hipMemcpyAsync(h->d_loop, &ls, sizeof(LoopState), hipMemcpyHostToDevice, h->st));
+    return PNGF_OK;
+}
+
+void device_loop_iteration(pngf_handle* h) {
+    hipStream_t st = h->st;
+    const Dims& d = h->d;
+    LoopState* ls = h->d_loop;
+    const int32_t* dyn = ls->dyn;
+    int32_t* dt = h->d_small;
+    int32_t* ds = h->d_small + 96;
+    int32_t* dlp = h->d_small + 192;
+    int32_t* dgt = h->d_small + 288;
+    const int ncl = h->h_loop->ncl;
+    launch_loop_batch(st, ls, h->d_cl, h->d_batch);
+    launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, ncl, h->d_eobj_out,
+                   h->d_info, h->d_lu, &ls->nb);
+    launch_objective(st, h->op, h->vecs.fc, h->d_eobj_out, ncl, h->d_F, &ls->F_best, &ls->sel, &ls->F_sel, &ls->nb);
+    launch_loop_after_select(st, d, ls, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_info, dt, ds, dlp,
+                             dgt, h->d_accg, h->d_acce, h->d_accF);
+    (void)hipEventRecord(h->ev_fork, st);
+    (void)hipStreamWaitEvent(h->st2, h->ev_fork, 0);
+    launch_lr_accept_prep(h->st2, d, h->vecs, 0, dt, ds, dlp, d.gmax, h->d_acc, h->acc_stride, h->d_info, h->d_lu, dyn,
+                          &ls->g);
+    (void)hipEventRecord(h->ev_join, h->st2);
+    // host k = gmax selects the column-tile count; the kernels read the actual k and K
+    launch_lr_gather(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride, dyn);
+    launch_lr_colblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, (int)h->small_stride, h->skv, dyn);
+    if (!h->sym)
+        launch_lr_rowblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv,
+                         dyn);
+    (void)hipStreamWaitEvent(st, h->ev_join, 0);
+    launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, d.gmax, 0,
+                     dyn);
+    launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, dyn);
+}
+
+// Runs the loop to its end; fills the counters and the accepted-flip arrays.
+int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_count, int64_t* nacc, int64_t* spec,
+                    int64_t* batches, double* F_best, int* converged) {
+    hipStream_t st = h->st;
+    const int gmax = h->d.gmax;
+    LoopState& ls = *h->h_loop;
+    int Kknown = h->K;
+    while (true) {
+        int S = (h->Kmax - Kknown) / gmax;
+        if (S < 1) {
+            // fold the factors (including a pending accept's columns) before the next chunk
+            h->K = Kknown;
+            CK(flush_factors(h));
+            ls.dyn[kDynKK] = 0;
+            ls.dyn[kDynActive] = 0;
+            HC(h, hipMemcpyAsync(h->d_loop, &ls, sizeof(LoopState), hipMemcpyHostToDevice, st));
+            Kknown = 0;
+            S = h->Kmax / gmax;
+        }
+        S = std::min(S, kLoopChunk);
+        for (int i = 0; i < S; i++) device_loop_iteration(h);
+        HC(h, hipGetLastError());
+        HC(h, hipMemcpyAsync(&ls, h->d_loop, sizeof(LoopState), hipMemcpyDeviceToHost, st));
+        HC(h, hipStreamSynchronize(st));
+        Kknown = ls.dyn[kDynKK] + (ls.dyn[kDynActive] ? ls.dyn[kDynK] : 0);
+        if (ls.done) break;
+    }
+    h->K = Kknown;
+    // host mirrors of the mask and tile map
+    HC(h, hipMemcpy(h->tile.data(), h->d_tile, h->tile.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HC(h, hipMemcpy(h->P.data(), h->d_P, h->P.size(), hipMemcpyDeviceToHost));
+    const int na = ls.nacc;
+    if (na > 0) {
+        std::vector<int32_t> g(na);
+        std::vector<int64_t> e(na);
+        std::vector<double> F(na);
+        HC(h, hipMemcpy(g.data(), h->d_accg, na * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HC(h, hipMemcpy(e.data(), h->d_acce, na * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HC(h, hipMemcpy(F.data(), h->d_accF, na * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < na; i++) {
+            if (p->acc_tiles) p->acc_tiles[i] = h->grp.prim[g[i]];
+            if (p->acc_eval) p->acc_eval[i] = e[i];
+            if (p->acc_F) p->acc_F[i] = F[i];
+        }
+    }
+    *flip_count = ls.flip_count;
+    *nacc = na;
+    *spec = ls.spec;
+    *batches = ls.batches;
+    *F_best = ls.F_best;
+    *converged = ls.nonimp >= ls.ncl ? 1 : 0;
+    if (ls.status == kLoopFull) return fail(h, PNGF_ERR_ARG, "accepted-flip arrays are full");
+    if (ls.status == kLoopSingular) return fail(h, PNGF_ERR_SINGULAR, "capacitance matrix is singular");
+    return PNGF_OK;
+}
+
 int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     if (!h || !p || !r) return fail(h, PNGF_ERR_ARG, "null argument");
     CK(check_ready(h));
@@ -1353,6 +1504,8 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     int64_t flip_count = 0, nacc = 0, spec = 0, batches = 0;
     const double acc0 = h->acc_seconds;
     h->flushes = 0;
+    const bool dev_loop = device_loop_ok(h, p);
+    if (dev_loop) CK(device_loop_prepare(h, p, F_best, bmin, (int)std::min<int64_t>(bmax_cfg, 1 << 30)));
     HC(h, hipStreamSynchronize(h->st));
     auto t0 = std::chrono::high_resolution_clock::now();
     auto record = [&](int g, double F) -> int {
@@ -1393,6 +1546,10 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             nacc++;
             F_best = F[best];
         }
+    } else if (dev_loop) {
+        int conv = 0;
+        rc = device_loop_run(h, p, &flip_count, &nacc, &spec, &batches, &F_best, &conv);
+        r->converged = conv;
     } else if (h->cfg.driver == PNGF_DRIVER_RASTER) {
         std::vector<int> cl;
         for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);

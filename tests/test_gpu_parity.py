@@ -316,3 +316,42 @@ def test_full_size_state_matches_fresh_init():
     a.accept([t])
     assert rel(a.state()["Eobj"], sa["Eobj"]) < 1e-9
     a.close()
+
+
+@pytest.mark.parametrize("layout,max_evals", [(P.LAYOUT_DISJOINT, 0), (P.LAYOUT_SHARED, 0), (P.LAYOUT_DISJOINT, 60)])
+def test_device_loop_matches_host_loop(layout, max_evals):
+    """The device-driven raster loop (batches, selection and accept bookkeeping in kernels) and
+    the host-driven one give bitwise the same run, on the ill-conditioned G where any
+    difference in rounding would show up as a different accepted sequence."""
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    gpu, ors, tm = setup_pair(layout, T, freqs, P.METHOD_LOWRANK, soft=0.0)
+    gpu.checkpoint()
+    a = gpu.dbs_run(max_evals=max_evals, loop=P.LOOP_HOST)
+    map_a, st_a = gpu.tile_map(), gpu.state()
+    gpu.restore()
+    b = gpu.dbs_run(max_evals=max_evals, loop=P.LOOP_AUTO)
+    assert a["accepts"] > 0
+    for key in ("flip_count", "accepts", "speculative_evals", "batches", "converged", "F_final"):
+        assert a[key] == b[key], key
+    assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
+    assert np.array_equal(a["acc_eval"], b["acc_eval"])
+    assert a["acc_F"].tolist() == b["acc_F"].tolist()
+    assert np.array_equal(map_a, gpu.tile_map())
+    assert np.array_equal(st_a["Eobj"], gpu.state()["Eobj"])
+    gpu.close()
+
+
+def test_device_loop_folds_factors():
+    """Factor capacity of 3 accepts: the device-driven loop folds between its chunks and still
+    follows the oracle's run."""
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3)
+    g = gpu.dbs_run(loop=P.LOOP_AUTO)
+    o = O.dbs_run(ors, O.OBJ_S11, freqs, DX, DY, DZ)
+    assert g["flushes"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    gpu.close()
