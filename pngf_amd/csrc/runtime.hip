@@ -129,6 +129,11 @@ struct pngf_handle {
     int64_t* d_acce = nullptr;
     double* d_accF = nullptr;
     int acc_alloc = 0;
+    // kLoopChunk iterations of that loop captured once and replayed (all their arguments are
+    // fixed device pointers and sizes: the loop state lives in device memory)
+    hipGraph_t loop_graph = nullptr;
+    hipGraphExec_t loop_exec = nullptr;
+    bool graph_failed = false;
     // profiling
     bool prof = false;
     std::vector<Timed> pending;
@@ -871,6 +876,8 @@ int pngf_destroy(pngf_handle* h) {
     if (h->h_pin_i) (void)hipHostFree(h->h_pin_i);
     if (h->h_pin_d) (void)hipHostFree(h->h_pin_d);
     if (h->h_loop) (void)hipHostFree(h->h_loop);
+    if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
+    if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
     if (h->rb) rocblas_destroy_handle(h->rb);
     if (h->st2) (void)hipStreamSynchronize(h->st2);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
@@ -1369,6 +1376,11 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
     }
     const int cap = std::max(1, p->max_accepts);
     if (cap > h->acc_alloc) {
+        // the captured iterations hold the old record arrays: capture again
+        if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
+        if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
+        h->loop_exec = nullptr;
+        h->loop_graph = nullptr;
         dfree(h, h->d_accg);
         dfree(h, h->d_acce);
         dfree(h, h->d_accF);
@@ -1451,7 +1463,27 @@ int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_coun
             S = h->Kmax / gmax;
         }
         S = std::min(S, kLoopChunk);
-        for (int i = 0; i < S; i++) device_loop_iteration(h);
+        if (S == kLoopChunk && !h->loop_exec && !h->graph_failed) {
+            // capture once; a capture the runtime refuses falls back to direct launches
+            bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                for (int i = 0; i < S; i++) device_loop_iteration(h);
+                ok = hipStreamEndCapture(st, &h->loop_graph) == hipSuccess && h->loop_graph;
+                ok = ok && hipGraphInstantiate(&h->loop_exec, h->loop_graph, nullptr, nullptr, 0) == hipSuccess;
+            }
+            if (!ok) {
+                (void)hipGetLastError();
+                if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
+                if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
+                h->loop_exec = nullptr;
+                h->loop_graph = nullptr;
+                h->graph_failed = true;
+            }
+        }
+        if (S == kLoopChunk && h->loop_exec)
+            HC(h, hipGraphLaunch(h->loop_exec, st));
+        else
+            for (int i = 0; i < S; i++) device_loop_iteration(h);
         HC(h, hipGetLastError());
         HC(h, hipMemcpyAsync(&ls, h->d_loop, sizeof(LoopState), hipMemcpyDeviceToHost, st));
         HC(h, hipStreamSynchronize(st));
