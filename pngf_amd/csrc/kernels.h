@@ -92,9 +92,13 @@ void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_s
 void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
                      int acc_stride, int k, int K);
 // fused apply_capinv + lr_small + lr_blocks for gmax <= 32
+// left != null: partition layout (groups [0, ngroups_part) have disjoint component lists; `left`
+// holds the nleft components outside every group); null: every group's block plus 32-component
+// chunks of all N
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
-                      int K, const int32_t* dyn = nullptr);
+                      int K, const int32_t* dyn = nullptr, const int32_t* left = nullptr, int nleft = 0,
+                      int ngroups_part = 0);
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       int k, int K);
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,

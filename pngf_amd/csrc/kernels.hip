@@ -1033,19 +1033,25 @@ void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const De
         hipLaunchKernelGGL((lr_blocks_kernel<36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
 }
 
-// LOWRANK accept epilogue for k <= 32 in one launch.  Blocks [0, nchunk) take 32 components
-// each: the new rows of M_all, q and R; blocks [nchunk, nchunk + ngroups) take one group's
-// diagonal block each, recomputing M_new at its components.
-//   M_new[c][n] = s_c sum_l capinv'[c][l] S[n][l]   with S[n][:] = W[t, n]: the new A_all
-//                 columns when W is symmetric, else the row strip Bt
-//   q[n] += sum_c A_new[n][c] w[c];   R[r][n] -= sum_c Rt[r][c] M_new[c][n]
-//   D_h[a][b] -= sum_c A_new[cl_a][c] M_new[c][cl_b]
-// Same operations in the same order as apply_capinv, lr_small and lr_blocks.
+// LOWRANK accept epilogue for k <= 32 in one launch.  A block takes a list of up to 32
+// components ("rows") and does
+//   rows part:  M_new[c][n] = s_c sum_l capinv'[c][l] S[n][l]  (S[n][:] = W[t, n]: the new A_all
+//               columns when W is symmetric, else the row strip Bt), written to M_all;
+//               q[n] += sum_c A_new[n][c] w[c];   R[r][n] -= sum_c Rt[r][c] M_new[c][n]
+//   block part: D_h[a][b] -= sum_c A_new[cl_a][c] M_new[c][cl_b] for the group h whose
+//               components are the rows
+// Default layout of the grid: blocks [0, nchunk) take 32 consecutive components (rows part),
+// blocks [nchunk, nchunk + ngroups) one group each (block part, M_new recomputed at its
+// components).  Partition layout (the candidate groups' component lists are disjoint, as in the
+// disjoint layout's raster groups): block h < ngroups takes group h and does both parts, the
+// remaining blocks take 32 of the components outside every group (`left`).  Same operations in
+// the same order as apply_capinv, lr_small and lr_blocks.
 __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
                                                         const zd* __restrict__ acc, int acc_stride,
                                                         const zd* __restrict__ Bt, size_t bt_stride,
                                                         const int32_t* __restrict__ s, int k, int K, int nchunk,
-                                                        const int32_t* dyn) {
+                                                        const int32_t* dyn, const int32_t* __restrict__ left,
+                                                        int nleft, int ngroups_part) {
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
@@ -1055,18 +1061,37 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     __shared__ zd As[32][33];
     __shared__ zd Ss[32][33];  // row strip, then M_new[c][row]
     __shared__ int rows[32];
-    const int f = blockIdx.y, tid = threadIdx.x;
-    const zd* stf = acc + (size_t)f * acc_stride;
-    const bool grp = (int)blockIdx.x >= nchunk;
-    const int h = grp ? (int)blockIdx.x - nchunk : 0;
-    const int n0 = grp ? 0 : (int)blockIdx.x * 32;
-    const int nrow = grp ? gt.ncomp[h] : min(32, d.N - n0);
+    const int f = blockIdx.y, tid = threadIdx.x, bx = blockIdx.x;
+    bool do_rows, do_block;
+    int h = -1, nrow, r0 = 0;
+    if (left) {
+        do_rows = true;
+        do_block = bx < ngroups_part;
+        if (do_block) {
+            h = bx;
+            nrow = gt.ncomp[h];
+        } else {
+            r0 = (bx - ngroups_part) * 32;
+            nrow = min(32, nleft - r0);
+        }
+    } else {
+        do_rows = bx < nchunk;
+        do_block = !do_rows;
+        if (do_block) {
+            h = bx - nchunk;
+            nrow = gt.ncomp[h];
+        } else {
+            r0 = bx * 32;
+            nrow = min(32, d.N - r0);
+        }
+    }
     if (nrow <= 0) return;
+    const zd* stf = acc + (size_t)f * acc_stride;
     for (int e = tid; e < k * k; e += 256) {
         const int r = e / k, c = e - r * k;
         Xs[r][c] = stf[e];
     }
-    if (tid < nrow) rows[tid] = grp ? gt.comps[(size_t)h * gt.gmax + tid] : n0 + tid;
+    if (tid < nrow) rows[tid] = h >= 0 ? gt.comps[(size_t)h * gt.gmax + tid] : (left ? left[r0 + tid] : r0 + tid);
     __syncthreads();
     const zd* A = tb.Aall[f];
     const zd* S = Bt + (size_t)f * bt_stride;
@@ -1099,29 +1124,30 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
         }
     }
     __syncthreads();
-    if (!grp) {
+    if (do_rows) {
         zd* M = tb.Mall[f];
         for (int e = tid; e < k * nrow; e += 256) {
             const int c = e / nrow, rr = e - c * nrow;
-            M[(size_t)(K + c) * d.ldN + n0 + rr] = Ss[c][rr];
+            M[(size_t)(K + c) * d.ldN + rows[rr]] = Ss[c][rr];
         }
         const zd* w = stf + 2 * k * k + k;
         const zd* Rt = stf + 2 * k * k + 2 * k + 4;
         if (tid < nrow) {
-            zd* qp = v.q + (size_t)f * d.ldN + n0 + tid;
+            zd* qp = v.q + (size_t)f * d.ldN + rows[tid];
             zd qa = *qp;
             for (int c = 0; c < k; c++) qa = zfma(As[tid][c], w[c], qa);
             *qp = qa;
         } else if (tid >= 64 && tid < 64 + 32 * d.ne) {
             const int r = (tid - 64) >> 5, rr = (tid - 64) & 31;
             if (rr < nrow) {
-                zd* rp = v.R + ((size_t)f * d.ne + r) * d.ldN + n0 + rr;
+                zd* rp = v.R + ((size_t)f * d.ne + r) * d.ldN + rows[rr];
                 zd ra = *rp;
                 for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], Ss[c][rr], ra);
                 *rp = ra;
             }
         }
-    } else {
+    }
+    if (do_block) {
         zd* Dh = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
         for (int e = tid; e < nrow * nrow; e += 256) {
             const int a = e / nrow, b = e - a * nrow;
@@ -1134,10 +1160,11 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
 
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
-                      int K, const int32_t* dyn) {
+                      int K, const int32_t* dyn, const int32_t* left, int nleft, int ngroups_part) {
     const int nchunk = (d.N + 31) / 32;
-    hipLaunchKernelGGL(lr_finish_kernel, dim3(nchunk + d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
-                       acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn);
+    const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
+    hipLaunchKernelGGL(lr_finish_kernel, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride, Bt,
+                       bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
 }
 
 __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,

@@ -131,6 +131,11 @@ struct pngf_handle {
     int acc_alloc = 0;
     // kLoopChunk iterations of that loop captured once and replayed (all their arguments are
     // fixed device pointers and sizes: the loop state lives in device memory)
+    // partition layout of the accept epilogue: the candidate groups' component lists are
+    // disjoint; d_left = the components outside every group
+    bool part = false;
+    int32_t* d_left = nullptr;
+    int nleft = 0;
     hipGraph_t loop_graph = nullptr;
     hipGraphExec_t loop_exec = nullptr;
     bool graph_failed = false;
@@ -639,7 +644,7 @@ int accept_group(pngf_handle* h, int g) {
         // column strip just written to A_all[:, K:K+k]
         if (d.gmax <= kLdsGmax) {
             launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, k,
-                             K);
+                             K, nullptr, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
         } else {
             if (h->sym)
                 launch_apply_capinv(st, d, h->d_acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, h->tabs.Mall, nullptr,
@@ -1044,6 +1049,24 @@ int pngf_init_state(pngf_handle* h) {
     const Groups& G = h->grp;
     h->d.ngroups = G.n;
     h->d.gmax = G.gmax;
+    {
+        std::vector<int> owner(h->d.N, -1);
+        bool disjoint = true;
+        for (int g = 0; g < h->ngroups_drv && disjoint; g++)
+            for (int a = 0; a < G.ncomp[g]; a++) {
+                int& o = owner[G.comps[(size_t)g * G.gmax + a]];
+                if (o >= 0) { disjoint = false; break; }
+                o = g;
+            }
+        std::vector<int32_t> left;
+        for (int i = 0; i < h->d.N && disjoint; i++) if (owner[i] < 0) left.push_back(i);
+        h->part = disjoint;
+        h->nleft = (int)left.size();
+        if (disjoint) {
+            CK(dalloc_t(h, &h->d_left, std::max<size_t>(left.size(), 1)));
+            HC(h, hipMemcpy(h->d_left, left.data(), left.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+    }
     CK(dalloc_t(h, &h->d_gtiles, G.tiles.size()));
     CK(dalloc_t(h, &h->d_gnt, G.ntiles.size()));
     CK(dalloc_t(h, &h->d_gcomps, G.comps.size()));
@@ -1439,7 +1462,7 @@ void device_loop_iteration(pngf_handle* h) {
                          dyn);
     (void)hipStreamWaitEvent(st, h->ev_join, 0);
     launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, d.gmax, 0,
-                     dyn);
+                     dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
     launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, dyn);
 }
 
