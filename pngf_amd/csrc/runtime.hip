@@ -1466,6 +1466,28 @@ void device_loop_iteration(pngf_handle* h) {
     launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, dyn);
 }
 
+// Captures kLoopChunk iterations into h->loop_exec once (after device_loop_prepare); a capture
+// the runtime refuses falls back to direct launches.  Relaxed capture mode: other threads
+// driving other handles keep calling the runtime meanwhile.
+void device_loop_capture(pngf_handle* h) {
+    if (h->loop_exec || h->graph_failed) return;
+    hipStream_t st = h->st;
+    bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (ok) {
+        for (int i = 0; i < kLoopChunk; i++) device_loop_iteration(h);
+        ok = hipStreamEndCapture(st, &h->loop_graph) == hipSuccess && h->loop_graph;
+        ok = ok && hipGraphInstantiate(&h->loop_exec, h->loop_graph, nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipGetLastError();
+        if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
+        if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
+        h->loop_exec = nullptr;
+        h->loop_graph = nullptr;
+        h->graph_failed = true;
+    }
+}
+
 // Runs the loop to its end; fills the counters and the accepted-flip arrays.
 int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_count, int64_t* nacc, int64_t* spec,
                     int64_t* batches, double* F_best, int* converged) {
@@ -1486,23 +1508,7 @@ int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_coun
             S = h->Kmax / gmax;
         }
         S = std::min(S, kLoopChunk);
-        if (S == kLoopChunk && !h->loop_exec && !h->graph_failed) {
-            // capture once; a capture the runtime refuses falls back to direct launches
-            bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
-            if (ok) {
-                for (int i = 0; i < S; i++) device_loop_iteration(h);
-                ok = hipStreamEndCapture(st, &h->loop_graph) == hipSuccess && h->loop_graph;
-                ok = ok && hipGraphInstantiate(&h->loop_exec, h->loop_graph, nullptr, nullptr, 0) == hipSuccess;
-            }
-            if (!ok) {
-                (void)hipGetLastError();
-                if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
-                if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
-                h->loop_exec = nullptr;
-                h->loop_graph = nullptr;
-                h->graph_failed = true;
-            }
-        }
+        if (S == kLoopChunk) device_loop_capture(h);
         if (S == kLoopChunk && h->loop_exec)
             HC(h, hipGraphLaunch(h->loop_exec, st));
         else
@@ -1855,6 +1861,16 @@ int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pn
     for (int i = 0; i < n; i++)
         for (int j = 0; j < i; j++)
             if (!hs[i] || hs[i] == hs[j]) return PNGF_ERR_ARG;  // one thread per handle
+    // device-loop buffers and graphs are set up here, one handle after another: allocations,
+    // synchronous copies and stream capture are then done before the threads run
+    for (int i = 0; i < n; i++) {
+        pngf_handle* h = hs[i];
+        if (!h->inited || !device_loop_ok(h, &ps[i])) continue;
+        (void)hipSetDevice(h->cfg.device);
+        CK(device_loop_prepare(h, &ps[i], 0.0, 1, 1));
+        HC(h, hipStreamSynchronize(h->st));
+        device_loop_capture(h);
+    }
     std::vector<int> rc(n, PNGF_OK);
     std::vector<std::thread> th;
     th.reserve(n);
