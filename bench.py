@@ -146,6 +146,19 @@ def bench_multi():
             opts.append(o)
             log("[multi] init %dx%d seed %d (%.1f s)" % (T, T, s, time.perf_counter() - t0))
     init_s = time.perf_counter() - t0
+    for o in opts:
+        o.checkpoint()
+    # one instance after another, then concurrently with 2, 4 (the default) and 16 workers
+    timings = {}
+    for w in (1, 2, 4, 16):
+        for o in opts:
+            o.restore()
+        t1 = time.perf_counter()
+        rs = P.dbs_run_many(opts, workers=w)
+        timings[w] = time.perf_counter() - t1
+        log("[multi] %2d workers: %.3f s" % (w, timings[w]))
+    for o in opts:
+        o.restore()
     t1 = time.perf_counter()
     rs = P.dbs_run_many(opts)
     conc_s = time.perf_counter() - t1
@@ -153,6 +166,7 @@ def bench_multi():
     out = {"metric": "multi-instance DBS evaluated flips/s (16 instances, 11..25 tiles, 5 freqs)",
            "value": flips / conc_s, "unit": "flips/s", "n_gpus": 1, "seconds": conc_s, "init_s": init_s,
            "flip_count": flips, "accepts": sum(r["accepts"] for r in rs),
+           "seconds_by_workers": {str(w): t for w, t in timings.items()},
            "instances": [dict(T=opts[i].T, flips=r["flip_count"], accepts=r["accepts"], seconds=r["seconds"],
                               flushes=r["flushes"], F_final=r["F_final"]) for i, r in enumerate(rs)],
            "sum_instance_seconds": sum(r["seconds"] for r in rs), "dtype": "f64", "data": "synthetic"}

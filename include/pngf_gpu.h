@@ -214,11 +214,16 @@ typedef struct {
    reference loop; candidates are scored in speculative batches against frozen state. */
 int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r);
 
-/* Runs pngf_dbs_run on n independent handles at once (one host thread per handle; each
-   handle's work stays on its own stream, so the device overlaps the instances).  For
+/* Runs pngf_dbs_run on n independent handles concurrently (host worker threads; each
+   handle's work stays on its own streams, so the device overlaps the instances).  For
    many DBS instances of uneven size on one GPU (reference: separate processes per run,
    main.cpp:173-175).  Returns the first non-zero status. */
 int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs);
+/* Same with an explicit number of host worker threads: the handles are taken largest
+   (N x frequencies) first by `workers` threads, each running one handle at a time.
+   pngf_dbs_run_many uses 4. */
+int pngf_dbs_run_pool(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs,
+                      int32_t workers);
 
 /* ---- learned flip ranker (BF16 MLP, hand-written HIP forward / backward / Adam) ----
    A din -> dh -> dh -> dout network with GELU, FP32 master weights, BF16 MFMA products

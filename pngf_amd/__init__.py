@@ -36,7 +36,7 @@ EXPORTS = [
     "pngf_profile", "pngf_kernel_stats", "pngf_get_g", "pngf_checkpoint", "pngf_restore",
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
-    "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many",
+    "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many", "pngf_dbs_run_pool",
     "pngf_autotune_result", "pngf_symmetry",
 ]
 
@@ -132,6 +132,7 @@ def lib():
         "pngf_mlp_train": (I, [P, P, P, I, I, I, ct.c_float, I, P]),
         "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
         "pngf_dbs_run_many": (I, [P, I, P, P]),
+        "pngf_dbs_run_pool": (I, [P, I, P, P, I]),
         "pngf_autotune_result": (I, [P, ct.POINTER(I), ct.POINTER(I), P]),
         "pngf_symmetry": (I, [P, ct.POINTER(I), ct.POINTER(ct.c_double)]),
         "pngf_features": (I, [P, P, I, P]),
@@ -441,8 +442,9 @@ class MLP:
         return out
 
 
-def dbs_run_many(opts, max_evals=0, max_accepts=100000):
-    """DBS on several independent optimizers at once (one host thread and stream each)."""
+def dbs_run_many(opts, max_evals=0, max_accepts=100000, workers=0):
+    """DBS on several independent optimizers concurrently (host worker threads, largest
+    problems first; workers = 0: the library default)."""
     n = len(opts)
     hs = (ct.c_void_p * n)(*[o.h for o in opts])
     ps = (DbsParams * n)()
@@ -452,7 +454,7 @@ def dbs_run_many(opts, max_evals=0, max_accepts=100000):
         p, acc, log = o._dbs_params(max_evals, 0, max_accepts, 0, 0, 0, 0)
         ps[i] = p
         keep.append(acc)
-    rc = lib().pngf_dbs_run_many(hs, n, ps, rs)
+    rc = lib().pngf_dbs_run_pool(hs, n, ps, rs, workers) if workers else lib().pngf_dbs_run_many(hs, n, ps, rs)
     if rc != 0:
         msgs = [lib().pngf_last_error(o.h).decode() for o in opts]
         raise PNGFError(rc, "dbs_run_many: " + "; ".join(m for m in msgs if m))

@@ -9,6 +9,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1382,6 +1383,8 @@ int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out) {
 // factor capacity is checked (and folded) between chunks.  Same batches, same selection and
 // the same kernels as the host loop, so the accepted sequence is identical.
 constexpr int kLoopChunk = 16;
+// host threads of pngf_dbs_run_many
+constexpr int kDefaultWorkers = 4;
 
 bool device_loop_ok(const pngf_handle* h, const pngf_dbs_params* p) {
     return p->loop == PNGF_LOOP_AUTO && h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_FIRST &&
@@ -1856,8 +1859,9 @@ int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp,
     return PNGF_OK;
 }
 
-int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs) {
-    if (!hs || !ps || !rs || n < 1) return PNGF_ERR_ARG;
+int pngf_dbs_run_pool(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs,
+                      int32_t workers) {
+    if (!hs || !ps || !rs || n < 1 || workers < 1) return PNGF_ERR_ARG;
     for (int i = 0; i < n; i++)
         for (int j = 0; j < i; j++)
             if (!hs[i] || hs[i] == hs[j]) return PNGF_ERR_ARG;  // one thread per handle
@@ -1871,18 +1875,35 @@ int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pn
         HC(h, hipStreamSynchronize(h->st));
         device_loop_capture(h);
     }
+    // largest problems first (N x frequencies), handed out to the workers in that order
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double)hs[a]->d.N * hs[a]->d.nf > (double)hs[b]->d.N * hs[b]->d.nf;
+    });
     std::vector<int> rc(n, PNGF_OK);
-    std::vector<std::thread> th;
-    th.reserve(n);
-    for (int i = 0; i < n; i++)
-        th.emplace_back([&, i]() {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        while (true) {
+            const int q = next.fetch_add(1);
+            if (q >= n) break;
+            const int i = order[q];
             (void)hipSetDevice(hs[i]->cfg.device);
             rc[i] = pngf_dbs_run(hs[i], &ps[i], &rs[i]);
-        });
+        }
+    };
+    const int nw = std::min<int>(workers, n);
+    std::vector<std::thread> th;
+    th.reserve(nw);
+    for (int w = 0; w < nw; w++) th.emplace_back(work);
     for (auto& t : th) t.join();
     for (int i = 0; i < n; i++)
         if (rc[i] != PNGF_OK) return rc[i];
     return PNGF_OK;
+}
+
+int pngf_dbs_run_many(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pngf_dbs_result* rs) {
+    return pngf_dbs_run_pool(hs, n, ps, rs, kDefaultWorkers);
 }
 
 int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym) {
