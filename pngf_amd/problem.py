@@ -73,6 +73,8 @@ WORKLOADS = {
     "antenna31x31_7f": dict(T=31, freqs=freq_list(7, 27e9, 33e9), layout=1, max_evals=0),
 }
 
-# 16 concurrent DBS instances: 4 seeds at each of 11, 15, 21 and 25 tiles, 5 frequencies
+# 16 concurrent DBS instances: 4 seeds at each of 11, 15, 21 and 25 tiles, 5 frequencies.
+# Factor capacity 208 accepts per instance (the largest runs accept ~200 flips): W0 of all 16
+# instances takes 120 GB, their factors ~110 GB more.
 MULTI16 = dict(sizes=[11, 15, 21, 25], seeds=[0, 1, 2, 3], freqs=freq_list(5, 28e9, 32e9), layout=1,
-               max_deferred=64)
+               max_deferred=208)
