@@ -279,6 +279,37 @@ def test_asymmetric_g_forms_both_strips():
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
+@pytest.mark.parametrize("asym", [False, True])
+def test_get_w_after_folds_and_pending_factors(asym):
+    """W = (G - I) C^-1 read back after accepts that folded the deferred factors into W0 (the
+    in-place fold) and left some pending (the fold into a copy): equal to the oracle's
+    (G - I) C^-1, row-major, for a symmetric and a non-symmetric G.  (Capacity 2 accepts of 24
+    components: the 5 accepts fold twice and leave one pending.)"""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    rng = np.random.default_rng(11)
+    Gs = []
+    for hz in freqs:
+        G = P.synth_g_host(P.LAYOUT_DISJOINT, T, 1, hz, soft=SOFT_WELL)
+        if asym:
+            N = G.shape[0]
+            G = G + 1e-3 * np.abs(G).max() * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N
+        Gs.append(np.asfortranarray(G))
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=2, Gs=Gs)
+    assert gpu.symmetry()[0] == (not asym)
+    for t in [3, 17, 30, 44, 8]:
+        for o in ors:
+            o.compute_updated_Eobj([t])
+            o.accept()
+        gpu.accept([t])
+    for f, G in enumerate(Gs):
+        N = ors[f].N
+        W_ref = (G[:N, :N] - np.eye(N)) @ ors[f].cinv()
+        assert rel(gpu.w(f), W_ref) < RTOL
+    E, V = oracle_state(ors)
+    assert rel(gpu.state()["Eobj"], E) < RTOL
+
+
 def test_full_size_state_matches_fresh_init():
     """Bench workload size (21x21 tiles, N = 10590, 9 frequencies; synthetic G with softening
     0.4 cells, cond(C) ~ 1e3): after 1500 evaluations of DBS the maintained state (W0 plus
