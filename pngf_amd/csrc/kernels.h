@@ -7,7 +7,7 @@ namespace pngf {
 // Per-frequency pointer tables live in device memory (arrays of nf pointers).
 struct DevTabs {
     zd** W0;     // LOWRANK: N x ldN row-major
-    zd** Aall;   // LOWRANK: N x Kmax in 16-row panels, element (n, K') at aix(n, K', Kmax)
+    zd** Aall;   // LOWRANK: N x Kmax row-major
     zd** Mall;   // LOWRANK: Kmax x ldN row-major
     zd** Grow;   // PLAIN: G row-major
     zd** Cinv;   // PLAIN: C^-1 column-major
@@ -31,15 +31,6 @@ struct Dims {
     // and the diagonal blocks are read as (D + D^T) / 2
     int sym;
 };
-
-// Deferred factor A_all (N x Kmax) is stored in panels of 16 rows: panel n / 16 is a 16 x Kmax
-// column-major block, so 16 consecutive rows of one inner index are 256 contiguous bytes and a
-// wave's 16 rows x 8 inner indices are 2 KB contiguous.  ceil(N / 16) panels are allocated in
-// full; the rows past N are zero.
-__host__ __device__ inline size_t aix(int n, int kk, int Kmax) {
-    return ((size_t)(n >> 4) * Kmax + kk) * 16 + (n & 15);
-}
-inline size_t a_elems(int N, int Kmax) { return (size_t)((N + 15) / 16) * 16 * Kmax; }
 
 // gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
 constexpr int kLdsGmax = 32;
@@ -92,9 +83,6 @@ void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const in
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant,
                       const int32_t* dyn = nullptr);
-// out_tab[f] = W0 - A_all[:, :K] M_all[:K, :] for every frequency, in column strips of 96 on the
-// skinny product (out_tab may be the W0 table: each element is read and written by one lane)
-void launch_lr_fold(hipStream_t st, const Dims& d, const DevTabs& tb, int K, zd* const* out_tab);
 // number of tile-shape variants of the skinny products for flips of k components
 int skinny_variants(int k);
 void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,

@@ -724,7 +724,7 @@ __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__
         mc[e] = M[(size_t)kk * d.ldN + t[c]];
         if (!Ar) continue;
         const int c2 = e / K, k2 = e - c2 * K;
-        ar[e] = A[aix(t[c2], k2, d.Kmax)];
+        ar[e] = A[(size_t)t[c2] * d.Kmax + k2];
     }
 }
 
@@ -771,8 +771,7 @@ template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
                                                                     int k, int K, const zd* __restrict__ small,
                                                                     int small_stride, zd* Bt, size_t bt_stride,
-                                                                    const int32_t* dyn, int fold_j0,
-                                                                    zd* const* fold_out) {
+                                                                    const int32_t* dyn) {
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
@@ -791,11 +790,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     const zd* A = tb.Aall[f];
     const zd* M = tb.Mall[f];
     const zd* W0 = tb.W0[f];
-    // B[K'][c] = bsrc[K' * bs_k + c * bs_c]: the gathered K x k block (colblk), the gathered
-    // k x K block (rowblk) or columns fold_j0.. of M_all (fold)
-    const zd* bsrc = fold_out ? M + fold_j0 : small + (size_t)f * small_stride;
-    const size_t bs_k = fold_out ? (size_t)d.ldN : ROW ? 1 : (size_t)k;
-    const size_t bs_c = ROW ? (size_t)K : 1;
+    const zd* sm = small + (size_t)f * small_stride;
     const int Kc = K - 1;
     d4v acc[NA][CT];
 #pragma unroll
@@ -806,7 +801,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int Kp = min(kb + 2 * lq + j, Kc);
-            a[j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc) : ldg(A, aix(nc, Kp, d.Kmax));
+            a[j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc) : ldg(A, (size_t)nc * d.Kmax + Kp);
         }
     };
     auto body = [&](const zd (&a)[2], int kloc) {
@@ -832,7 +827,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
         for (int e = threadIdx.x; e < KC * BW; e += NT) {
             const int kk = e / BW, c = e - kk * BW;
             zd b = zmk(0.0, 0.0);
-            if (k0 + kk < K && c < k) b = ldg(bsrc, (size_t)(k0 + kk) * bs_k + (size_t)c * bs_c);
+            if (k0 + kk < K && c < k) b = ldg(sm, ROW ? (size_t)c * K + k0 + kk : (size_t)(k0 + kk) * k + c);
             Bp[kk][2 * c] = zmk(b.x, -b.y);
             Bp[kk][2 * c + 1] = zmk(b.y, b.x);
         }
@@ -844,7 +839,6 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    double* out = fold_out ? (double*)fold_out[f] : nullptr;
 #pragma unroll
     for (int c = 0; c < CT; c++) {
         const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
@@ -854,15 +848,12 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             const int row = n0 + lq + 4 * v;
             if (row >= d.N) continue;
             const double a = NA == 2 ? acc[0][c][v] + acc[NA - 1][c][v] : acc[0][c][v];
-            if (ROW) {
+            if (!ROW) {
+                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
+                ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
+            } else {
                 const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
                 ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
-            } else if (out) {
-                const size_t o = (size_t)row * d.ldN + fold_j0 + cc;
-                out[o * 2 + q] = ((const double*)W0)[o * 2 + q] - a;
-            } else {
-                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
-                ((double*)A)[aix(row, K + cc, d.Kmax) * 2 + q] = w[q] - a;
             }
         }
     }
@@ -873,21 +864,19 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 // variant gives bitwise the same result and the autotuner's choice cannot change a DBS run.
 template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                           const zd* small, int small_stride, zd* Bt, size_t bt_stride, const int32_t* dyn,
-                           int fold_j0, zd* const* fold_out) {
+                           const zd* small, int small_stride, zd* Bt, size_t bt_stride, const int32_t* dyn) {
     const int rows = 16 * WAVES;
     dim3 grid((d.N + rows - 1) / rows, d.nf);
     hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, SPLIT>), grid, dim3(WAVES * 64), 0, st, d, tb,
-                       t, k, K, small, small_stride, Bt, bt_stride, dyn, fold_j0, fold_out);
+                       t, k, K, small, small_stride, Bt, bt_stride, dyn);
 }
 
 template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                                const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant,
-                               const int32_t* dyn, int fold_j0 = 0, zd* const* fold_out = nullptr) {
+                               const int32_t* dyn) {
 #define PNGF_SKV(CT, KC, DEPTH, W, SPLIT) \
-    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn, fold_j0, \
-                                                 fold_out)
+    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
     if (k <= 24) {
         switch (variant) {
             case 1: PNGF_SKV(3, 32, 2, 4, false); break;
@@ -917,16 +906,9 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const in
                       const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant, const int32_t* dyn) {
     launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant, dyn);
 }
-void launch_lr_fold(hipStream_t st, const Dims& d, const DevTabs& tb, int K, zd* const* out_tab) {
-    constexpr int kStrip = 96;
-    for (int j0 = 0; j0 < d.N; j0 += kStrip)
-        launch_skinny_mfma<false>(st, d, tb, nullptr, std::min(kStrip, d.N - j0), K, nullptr, 0, nullptr, 0, 0,
-                                  nullptr, j0, out_tab);
-}
 
-// out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb, B + f *
-// b_stride; when b_tab is given, B[n][l] = A_all element (n, b_col0 + l) of b_tab[f], with
-// ldb = Kmax)
+// out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb;
+// B = b_tab[f] + b_col0 when b_tab is given, else B + f * b_stride)
 // PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
 // LOWRANK: M_new = diag(s) cap^-1 W[t,:]  (W[t,:]^T = W[:,t], the new A_all columns, when W
 //          is symmetric)
@@ -937,12 +919,12 @@ __global__ void apply_capinv_kernel(Dims d, const zd* __restrict__ acc, int acc_
     __shared__ zd Bs[32][97];
     const int f = blockIdx.y, tid = threadIdx.x;
     const int n0 = blockIdx.x * 32;
-    const zd* Bf = b_tab ? b_tab[f] : B + (size_t)f * b_stride;
+    const zd* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
     const zd* ci = acc + (size_t)f * acc_stride;
     for (int e = tid; e < 32 * k; e += 256) {
         const int nn = e / k, l = e - nn * k;
         const int n = n0 + nn;
-        Bs[nn][l] = n >= d.N ? zmk(0.0, 0.0) : b_tab ? Bf[aix(n, b_col0 + l, ldb)] : Bf[(size_t)n * ldb + l];
+        Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zmk(0.0, 0.0);
     }
     __syncthreads();
     zd* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
@@ -975,10 +957,10 @@ __global__ void lr_small_kernel(Dims d, DevTabs tb, DevVecs v, const zd* __restr
     const zd* st = acc + (size_t)f * acc_stride;
     const zd* w = st + 2 * k * k + k;
     const zd* Rt = st + 2 * k * k + 2 * k + 4;
-    const zd* A = tb.Aall[f];
+    const zd* A = tb.Aall[f] + (size_t)n * d.Kmax + K;
     const zd* M = tb.Mall[f] + (size_t)K * d.ldN + n;
     zd qa = v.q[(size_t)f * d.ldN + n];
-    for (int c = 0; c < k; c++) qa = zfma(A[aix(n, K + c, d.Kmax)], w[c], qa);
+    for (int c = 0; c < k; c++) qa = zfma(A[c], w[c], qa);
     v.q[(size_t)f * d.ldN + n] = qa;
     for (int r = 0; r < d.ne; r++) {
         zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
@@ -1013,7 +995,7 @@ __global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabs tb, DevV
         const int cl_ = min(16, k - c0);
         for (int e = tid; e < nc * 16; e += 256) {
             const int a = e / 16, c = e - a * 16;
-            Ac[a][c] = c < cl_ ? A[aix(cl[a], K + c0 + c, d.Kmax)] : zmk(0.0, 0.0);
+            Ac[a][c] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zmk(0.0, 0.0);
         }
         for (int e = tid; e < 16 * nc; e += 256) {
             const int c = e / nc, bb = e - c * nc;
@@ -1116,7 +1098,7 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     for (int e = tid; e < nrow * k; e += 256) {
         const int rr = e / k, c = e - rr * k;
         const int n = rows[rr];
-        As[rr][c] = A[aix(n, K + c, d.Kmax)];
+        As[rr][c] = A[(size_t)n * d.Kmax + K + c];
         if (!d.sym) Ss[rr][c] = S[(size_t)n * d.gmax + c];
     }
     __syncthreads();
@@ -1688,7 +1670,7 @@ __global__ void lr_block_current_kernel(Dims d, DevTabs tb, GroupTab gt, int g, 
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nc * nc; e += gridDim.x * blockDim.x) {
         const int a = e / nc, b = e - a * nc;
         zd s = W0[(size_t)cl[a] * d.ldN + cl[b]];
-        for (int kk = 0; kk < K; kk++) s = zfms(A[aix(cl[a], kk, d.Kmax)], M[(size_t)kk * d.ldN + cl[b]], s);
+        for (int kk = 0; kk < K; kk++) s = zfms(A[(size_t)cl[a] * d.Kmax + kk], M[(size_t)kk * d.ldN + cl[b]], s);
         D[a * gt.gmax + b] = s;
     }
 }

@@ -578,12 +578,15 @@ int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
 // ---------------------------------------------------------------- accept ---
 int flush_factors(pngf_handle* h) {
     if (h->K == 0) return PNGF_OK;
+    const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
     const Dims& d = h->d;
     Timed tm{};
     prof_begin(h, tm);
-    launch_lr_fold(h->st, d, h->tabs, h->K, h->tabs.W0);
-    if (h->sym)
-        for (int f = 0; f < d.nf; f++) launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
+    for (int f = 0; f < d.nf; f++) {
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
+                            (rbz*)h->Mall[f], d.ldN, (rbz*)h->Aall[f], d.Kmax, &one, (rbz*)h->W0[f], d.ldN));
+        if (h->sym) launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
+    }
     prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (32.0 * d.N * (double)d.N + 32.0 * d.N * (double)h->K),
              (double)d.nf * 8.0 * h->K * (double)d.N * d.N);
     h->K = 0;
@@ -1109,7 +1112,7 @@ int pngf_init_state(pngf_handle* h) {
             size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
             avail = std::min(avail, lim);
         }
-        const size_t per_col = (size_t)nf * (a_elems(N, 1) + ld) * sizeof(zd);  // A_all column + M_all row
+        const size_t per_col = (size_t)nf * 2 * ld * sizeof(zd);  // A_all column + M_all row
         // Folding K deferred columns into W0 costs 8 N^2 K flop; not folding costs 8 N K k more per
         // later accept, so folding pays only when more than N / k accepts remain: by default the
         // factors hold N / gmax accepts (memory permitting) and a typical run never folds.
@@ -1120,7 +1123,7 @@ int pngf_init_state(pngf_handle* h) {
         h->Kmax = (int)cols;
         h->d.Kmax = h->Kmax;
         for (int f = 0; f < nf; f++) {
-            CK(dalloc_t(h, &h->Aall[f], a_elems(N, h->Kmax)));
+            CK(dalloc_t(h, &h->Aall[f], (size_t)N * h->Kmax));
             CK(dalloc_t(h, &h->Mall[f], (size_t)h->Kmax * ld));
         }
         h->small_stride = (size_t)h->Kmax * gmax;
@@ -1129,7 +1132,7 @@ int pngf_init_state(pngf_handle* h) {
         h->bt_stride = (size_t)N * gmax;
         CK(dalloc_t(h, &h->d_Bt, nf * h->bt_stride));
         for (int f = 0; f < nf; f++) {
-            HC(h, hipMemset(h->Aall[f], 0, sizeof(zd) * a_elems(N, h->Kmax)));
+            HC(h, hipMemset(h->Aall[f], 0, sizeof(zd) * (size_t)N * h->Kmax));
             HC(h, hipMemset(h->Mall[f], 0, sizeof(zd) * (size_t)h->Kmax * ld));
         }
         HC(h, hipMemset(h->d_Mc, 0, sizeof(zd) * nf * h->small_stride));
@@ -1273,21 +1276,12 @@ int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out) {
     zd* tmp = nullptr;
     CK(dalloc_t(h, &tmp, ld * N));
     HC(h, hipMemcpyAsync(tmp, h->W0[f], ld * N * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
-    zd** out_tab = nullptr;
     if (h->K > 0) {
-        // fold the pending factors of this one frequency into the copy
-        CK(dalloc_t(h, &out_tab, 1));
-        HC(h, hipMemcpyAsync(out_tab, &tmp, sizeof(zd*), hipMemcpyHostToDevice, h->st));
-        Dims d1 = h->d;
-        d1.nf = 1;
-        DevTabs t1 = h->tabs;
-        t1.W0 += f;
-        t1.Aall += f;
-        t1.Mall += f;
-        launch_lr_fold(h->st, d1, t1, h->K, out_tab);
+        const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, N, N, h->K, &mone,
+                            (rbz*)h->Mall[f], ld, (rbz*)h->Aall[f], h->Kmax, &one, (rbz*)tmp, ld));
     }
     HC(h, hipStreamSynchronize(h->st));
-    dfree(h, out_tab);
     HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
     dfree(h, tmp);
     return PNGF_OK;
