@@ -45,17 +45,31 @@ enum { kDynK = 0, kDynKK = 1, kDynActive = 2, kDynNt = 3 };
 struct LoopState {
     int64_t flip_count, spec, batches, max_evals;
     double F_best, F_sel;
-    int32_t pos, nonimp, bs, nb, sel, done, nacc, ncl, bmin, bmax, acc_cap, status, g, pad;
+    int32_t pos, nonimp, bs, nb, sel, done, nacc, ncl, bmin, bmax, acc_cap, status, g;
+    int32_t pad;
     int32_t dyn[4];
 };
 enum { kLoopOk = 0, kLoopFull = 1, kLoopSingular = 2 };
 
-// One iteration's host-free steps (see runtime.hip, dbs_run_device).
+// Loop control of the device-driven loop (loop_control_kernel, after the scoring kernel):
+// every candidate's objective, the first improving one, the accept record and the next batch.
+struct LoopCtl {
+    LoopState* ls;
+    ObjParams op;
+    const FreqConst* fc;
+    double* F;              // [ncl] objective per candidate of the batch
+    const int32_t* cl;      // candidate groups in visit order
+    int32_t *t, *s, *lp, *gtiles;  // accept record: change set and tiles of the selected group
+    int32_t* acc_group;
+    int64_t* acc_eval;
+    double* acc_F;
+};
+
+// First batch of the device-driven loop (later batches come from the loop control kernel).
 void launch_loop_batch(hipStream_t st, LoopState* ls, const int32_t* cl, int32_t* batch);
-void launch_loop_after_select(hipStream_t st, const Dims& d, LoopState* ls, GroupTab gt, const int32_t* tile,
-                              const int8_t* P, const int32_t* owners, const int32_t* batch, const int32_t* info,
-                              int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
-                              int64_t* acc_eval, double* acc_F);
+void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32_t* tile, const int8_t* P,
+                         const int32_t* owners, int32_t* batch, const zd* eobj, const int32_t* info,
+                         const LoopCtl& c);
 
 // ---- eval
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
@@ -101,8 +115,10 @@ void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const De
                       int ngroups_part = 0);
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       int k, int K);
+// With dyn (device-driven loop) it is the accept's last kernel: it also commits the accept to
+// the factor count (dyn[K] += dyn[k]) and clears the pending flag.
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles, const int32_t* dyn = nullptr);
+                        const int32_t* gtiles, int ntiles, int32_t* dyn = nullptr);
 
 // ---- PLAIN accept
 void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, zd* V,

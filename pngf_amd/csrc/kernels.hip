@@ -230,13 +230,12 @@ static size_t lds_matrix_bytes(int gmax, const zd* gscratch) {
 // One workgroup per (candidate b, frequency f): cap = I + D[t,t] diag(s),
 // rhs = q_t + D[t,t] dPE, y = cap^-1 rhs, newEobj = Eobj + R[:,t] (dPE - s y),
 // where dPE = -s Einc_t and D the maintained block of W over the group's components.
-__global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
-                               const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
-                               const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
-                               int32_t* __restrict__ out_info, zd* gscratch, const int32_t* nb_dev) {
+__device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt, const int32_t* __restrict__ tile,
+                            const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                            const int32_t* batch, zd* __restrict__ out_eobj, int32_t* __restrict__ out_info,
+                            zd* gscratch) {
     extern __shared__ zd smz[];
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nth = blockDim.x;
-    if (nb_dev && b >= *nb_dev) return;
     const int gmax = d.gmax;
     zd* dpe = smz;                                // [gmax]
     zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
@@ -321,6 +320,71 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
         out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
     }
     if (tid == 0 && out_info) out_info[b * d.nf + f] = info;
+}
+
+__device__ void loop_after_select_wave(const Dims& d, LoopState* ls, const GroupTab& gt, const int32_t* tile,
+                                       const int8_t* P, const int32_t* owners, const int32_t* batch,
+                                       const int32_t* info, const LoopCtl& c);
+__device__ void loop_next_batch(LoopState* ls, int* B_out, int* pos_out, int* ncl_out);
+
+// One workgroup per (candidate b, frequency f); workgroups past the batch size (read from
+// nb_dev) return at once.
+__global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
+                               const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+                               const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
+                               int32_t* __restrict__ out_info, zd* gscratch, const int32_t* nb_dev) {
+    if (nb_dev && (int)blockIdx.x >= *nb_dev) return;
+    lr_eval_one(d, v, gt, tile, P, owners, batch, out_eobj, out_info, gscratch);
+}
+
+// Loop control of the device-driven loop, one workgroup of 256 after the scoring kernel: every
+// candidate's objective, the first improving one in batch order, the accept record and the
+// next batch.  (A kernel of its own rather than the scoring kernel's last workgroup: on a
+// multi-XCD device the device-scope fence that pattern needs in every workgroup writes back
+// the XCD's L2 and cost more than the launch it saves.)
+__global__ void __launch_bounds__(256) loop_control_kernel(Dims d, GroupTab gt, const int32_t* __restrict__ tile,
+                                                           const int8_t* __restrict__ P,
+                                                           const int32_t* __restrict__ owners, int32_t* batch,
+                                                           const zd* __restrict__ eobj,
+                                                           const int32_t* __restrict__ info, LoopCtl c) {
+    __shared__ int s_red[256];
+    __shared__ double s_F;
+    __shared__ int s_B, s_pos, s_ncl;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    LoopState* ls = c.ls;
+    const int nb = ls->nb;
+    const double fb = ls->F_best;
+    int best = 0x7fffffff;
+    for (int b = tid; b < nb; b += nth) {
+        const double F = objective_of(c.op, c.fc, eobj + (size_t)b * d.nf * d.ne);
+        c.F[b] = F;
+        if (F < fb && best == 0x7fffffff) best = b;
+    }
+    s_red[tid] = best;
+    __syncthreads();
+    for (int off = nth >> 1; off > 0; off >>= 1) {
+        if (tid < off) s_red[tid] = min(s_red[tid], s_red[tid + off]);
+        __syncthreads();
+    }
+    const int bi = s_red[0];
+    if (bi != 0x7fffffff && bi % nth == tid) s_F = c.F[bi];
+    __syncthreads();
+    if (tid == 0) {
+        ls->sel = bi == 0x7fffffff ? -1 : bi;
+        ls->F_sel = bi == 0x7fffffff ? fb : s_F;
+    }
+    __syncthreads();
+    if (tid < 64) loop_after_select_wave(d, ls, gt, tile, P, owners, batch, info, c);
+    __syncthreads();
+    if (tid == 0) loop_next_batch(ls, &s_B, &s_pos, &s_ncl);
+    __syncthreads();
+    for (int i = tid; i < s_B; i += nth) batch[i] = c.cl[(s_pos + i) % s_ncl];
+}
+
+void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32_t* tile, const int8_t* P,
+                         const int32_t* owners, int32_t* batch, const zd* eobj, const int32_t* info,
+                         const LoopCtl& c) {
+    hipLaunchKernelGGL(loop_control_kernel, dim3(1), dim3(256), 0, st, d, gt, tile, P, owners, batch, eobj, info, c);
 }
 
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
@@ -1046,6 +1110,7 @@ void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const De
 // disjoint layout's raster groups): block h < ngroups takes group h and does both parts, the
 // remaining blocks take 32 of the components outside every group (`left`).  Same operations in
 // the same order as apply_capinv, lr_small and lr_blocks.
+template <int G>
 __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
                                                         const zd* __restrict__ acc, int acc_stride,
                                                         const zd* __restrict__ Bt, size_t bt_stride,
@@ -1057,10 +1122,10 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
         k = dyn[kDynK];
         K = dyn[kDynKK];
     }
-    __shared__ zd Xs[32][33];
-    __shared__ zd As[32][33];
-    __shared__ zd Ss[32][33];  // row strip, then M_new[c][row]
-    __shared__ int rows[32];
+    __shared__ zd Xs[G][G + 1];
+    __shared__ zd As[G][G + 1];
+    __shared__ zd Ss[G][G + 1];  // row strip, then M_new[c][row]
+    __shared__ int rows[G];
     const int f = blockIdx.y, tid = threadIdx.x, bx = blockIdx.x;
     bool do_rows, do_block;
     int h = -1, nrow, r0 = 0;
@@ -1071,8 +1136,8 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
             h = bx;
             nrow = gt.ncomp[h];
         } else {
-            r0 = (bx - ngroups_part) * 32;
-            nrow = min(32, nleft - r0);
+            r0 = (bx - ngroups_part) * G;
+            nrow = min(G, nleft - r0);
         }
     } else {
         do_rows = bx < nchunk;
@@ -1081,8 +1146,8 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
             h = bx - nchunk;
             nrow = gt.ncomp[h];
         } else {
-            r0 = bx * 32;
-            nrow = min(32, d.N - r0);
+            r0 = bx * G;
+            nrow = min(G, d.N - r0);
         }
     }
     if (nrow <= 0) return;
@@ -1161,14 +1226,22 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
                       int K, const int32_t* dyn, const int32_t* left, int nleft, int ngroups_part) {
-    const int nchunk = (d.N + 31) / 32;
-    const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
-    hipLaunchKernelGGL(lr_finish_kernel, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride, Bt,
-                       bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
+    // G: components per workgroup (the group size bound); k <= 24 halves the LDS footprint
+    if (d.gmax <= 24) {
+        const int nchunk = (d.N + 23) / 24;
+        const int nblk = left ? ngroups_part + (nleft + 23) / 24 : nchunk + d.ngroups;
+        hipLaunchKernelGGL(lr_finish_kernel<24>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
+                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
+    } else {
+        const int nchunk = (d.N + 31) / 32;
+        const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
+        hipLaunchKernelGGL(lr_finish_kernel<32>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
+                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
+    }
 }
 
 __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                                   const int32_t* gtiles, int ntiles, const int32_t* dyn) {
+                                   const int32_t* gtiles, int ntiles, int32_t* dyn) {
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
@@ -1177,41 +1250,43 @@ __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, c
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         for (int c = 0; c < k; c++) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
         for (int i = 0; i < ntiles; i++) tile[gtiles[i]] ^= 1;
+        if (dyn) {
+            dyn[kDynKK] += k;
+            dyn[kDynActive] = 0;
+        }
     }
 }
 
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles, const int32_t* dyn) {
+                        const int32_t* gtiles, int ntiles, int32_t* dyn) {
     hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(64), 0, st, P, tile, t, s, k, gtiles, ntiles, dyn);
 }
 
 // ---------------------------------------------------- device-driven loop -----
-// One raster first-improvement iteration without a host round trip: this kernel commits the
-// previous iteration's accept to the factor count and writes the next batch (the candidates
-// after the current position, cyclically), the batch-sized kernels read its size from
-// ls->nb, select_kernel writes ls->sel / ls->F_sel, and loop_after_select_kernel turns the
-// selection into the accept record the accept kernels read (or into the non-improving
-// bookkeeping).  Same rules as the host loop in pngf_dbs_run.
+// Raster first-improvement iterations without a host round trip.  Per iteration: the scoring
+// kernel (lr_eval_kernel) reads the batch size from ls->nb, and its last workgroup selects,
+// writes the accept record and the next batch; the accept kernels read k, K and "accept
+// pending" from ls->dyn; mask_update_kernel commits the accept to the factor count.
+// Size and start of the next batch: the candidates after the current position, cyclically,
+// at most bs of them (doubling after each non-improving batch), none once the run is done.
+__device__ void loop_next_batch(LoopState* ls, int* B_out, int* pos_out, int* ncl_out) {
+    int64_t B = 0;
+    if (!ls->done) {
+        B = ls->ncl - ls->nonimp;
+        if (ls->max_evals > 0) B = min(B, ls->max_evals - ls->flip_count);
+        B = min(B, (int64_t)min(ls->bs, ls->bmax));
+        B = max(B, (int64_t)0);
+    }
+    ls->nb = (int)B;
+    *B_out = (int)B;
+    *pos_out = ls->pos;
+    *ncl_out = ls->ncl;
+}
+
 __global__ void loop_batch_kernel(LoopState* ls, const int32_t* __restrict__ cl, int32_t* batch) {
     __shared__ int s_B, s_pos, s_ncl;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        if (ls->dyn[kDynActive]) {
-            ls->dyn[kDynKK] += ls->dyn[kDynK];
-            ls->dyn[kDynActive] = 0;
-        }
-        int64_t B = 0;
-        if (!ls->done) {
-            B = ls->ncl - ls->nonimp;
-            if (ls->max_evals > 0) B = min(B, ls->max_evals - ls->flip_count);
-            B = min(B, (int64_t)min(ls->bs, ls->bmax));
-            B = max(B, (int64_t)0);
-        }
-        ls->nb = (int)B;
-        s_B = (int)B;
-        s_pos = ls->pos;
-        s_ncl = ls->ncl;
-    }
+    if (tid == 0) loop_next_batch(ls, &s_B, &s_pos, &s_ncl);
     __syncthreads();
     for (int i = tid; i < s_B; i += blockDim.x) batch[i] = cl[(s_pos + i) % s_ncl];
 }
@@ -1220,12 +1295,12 @@ void launch_loop_batch(hipStream_t st, LoopState* ls, const int32_t* cl, int32_t
     hipLaunchKernelGGL(loop_batch_kernel, dim3(1), dim3(256), 0, st, ls, cl, batch);
 }
 
-// One wave (64 threads).
-__global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, const int32_t* __restrict__ tile,
-                                         const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
-                                         const int32_t* __restrict__ batch, const int32_t* __restrict__ info,
-                                         int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
-                                         int64_t* acc_eval, double* acc_F) {
+// The selection (ls->sel, ls->F_sel) turned into the accept record the accept kernels read,
+// or into the non-improving bookkeeping; one wave (lanes 0..63).  Same rules as the host loop
+// in pngf_dbs_run.
+__device__ void loop_after_select_wave(const Dims& d, LoopState* ls, const GroupTab& gt, const int32_t* tile,
+                                       const int8_t* P, const int32_t* owners, const int32_t* batch,
+                                       const int32_t* info, const LoopCtl& c) {
     const int tid = threadIdx.x;
     if (ls->done) {
         if (tid == 0) ls->dyn[kDynActive] = 0;
@@ -1245,7 +1320,7 @@ __global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, con
     }
     if (sel >= 0) {
         const int g = batch[sel];
-        const int k = change_set_wave(gt, g, tile, P, owners, t, s, lp);
+        const int k = change_set_wave(gt, g, tile, P, owners, c.t, c.s, c.lp);
         if (tid == 0) {
             if (ls->nacc >= ls->acc_cap) {
                 ls->status = kLoopFull;
@@ -1253,9 +1328,9 @@ __global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, con
                 ls->dyn[kDynActive] = 0;
                 return;
             }
-            acc_group[ls->nacc] = g;
-            acc_eval[ls->nacc] = ls->flip_count + sel;
-            acc_F[ls->nacc] = ls->F_sel;
+            c.acc_group[ls->nacc] = g;
+            c.acc_eval[ls->nacc] = ls->flip_count + sel;
+            c.acc_F[ls->nacc] = ls->F_sel;
             ls->nacc++;
             ls->flip_count += sel + 1;
             ls->spec += nb - sel - 1;
@@ -1264,7 +1339,7 @@ __global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, con
             ls->nonimp = 0;
             ls->bs = ls->bmin;
             const int nt = gt.ntiles[g];
-            for (int i = 0; i < nt; i++) gtiles[i] = gt.tiles[4 * g + i];
+            for (int i = 0; i < nt; i++) c.gtiles[i] = gt.tiles[4 * g + i];
             ls->g = g;
             ls->dyn[kDynK] = k;
             ls->dyn[kDynNt] = nt;
@@ -1281,14 +1356,6 @@ __global__ void loop_after_select_kernel(Dims d, LoopState* ls, GroupTab gt, con
         ls->batches += nb > 0 ? 1 : 0;
         if (ls->nonimp >= ls->ncl || (ls->max_evals > 0 && ls->flip_count >= ls->max_evals)) ls->done = 1;
     }
-}
-
-void launch_loop_after_select(hipStream_t st, const Dims& d, LoopState* ls, GroupTab gt, const int32_t* tile,
-                              const int8_t* P, const int32_t* owners, const int32_t* batch, const int32_t* info,
-                              int32_t* t, int32_t* s, int32_t* lp, int32_t* gtiles, int32_t* acc_group,
-                              int64_t* acc_eval, double* acc_F) {
-    hipLaunchKernelGGL(loop_after_select_kernel, dim3(1), dim3(64), 0, st, d, ls, gt, tile, P, owners, batch, info,
-                       t, s, lp, gtiles, acc_group, acc_eval, acc_F);
 }
 
 // --------------------------------------------------------- PLAIN accept -----

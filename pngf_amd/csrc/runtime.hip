@@ -1433,6 +1433,7 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
     ls.sel = -1;
     ls.dyn[kDynKK] = h->K;
     HC(h, hipMemcpyAsync(h->d_loop, &ls, sizeof(LoopState), hipMemcpyHostToDevice, h->st));
+    launch_loop_batch(h->st, h->d_loop, h->d_cl, h->d_batch);
     return PNGF_OK;
 }
 
@@ -1446,12 +1447,22 @@ void device_loop_iteration(pngf_handle* h) {
     int32_t* dlp = h->d_small + 192;
     int32_t* dgt = h->d_small + 288;
     const int ncl = h->h_loop->ncl;
-    launch_loop_batch(st, ls, h->d_cl, h->d_batch);
+    LoopCtl c{};
+    c.ls = ls;
+    c.op = h->op;
+    c.fc = h->vecs.fc;
+    c.F = h->d_F;
+    c.cl = h->d_cl;
+    c.t = dt;
+    c.s = ds;
+    c.lp = dlp;
+    c.gtiles = dgt;
+    c.acc_group = h->d_accg;
+    c.acc_eval = h->d_acce;
+    c.acc_F = h->d_accF;
     launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, ncl, h->d_eobj_out,
                    h->d_info, h->d_lu, &ls->nb);
-    launch_objective(st, h->op, h->vecs.fc, h->d_eobj_out, ncl, h->d_F, &ls->F_best, &ls->sel, &ls->F_sel, &ls->nb);
-    launch_loop_after_select(st, d, ls, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_info, dt, ds, dlp,
-                             dgt, h->d_accg, h->d_acce, h->d_accF);
+    launch_loop_control(st, d, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_eobj_out, h->d_info, c);
     (void)hipEventRecord(h->ev_fork, st);
     (void)hipStreamWaitEvent(h->st2, h->ev_fork, 0);
     launch_lr_accept_prep(h->st2, d, h->vecs, 0, dt, ds, dlp, d.gmax, h->d_acc, h->acc_stride, h->d_info, h->d_lu, dyn,
@@ -1466,7 +1477,7 @@ void device_loop_iteration(pngf_handle* h) {
     (void)hipStreamWaitEvent(st, h->ev_join, 0);
     launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, d.gmax, 0,
                      dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
-    launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, dyn);
+    launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, ls->dyn);
 }
 
 // Captures kLoopChunk iterations into h->loop_exec once (after device_loop_prepare); a capture
