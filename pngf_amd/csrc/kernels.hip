@@ -819,7 +819,9 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
 // ring.  Loads go through global-address-space pointers (counted on vmcnt only), are
 // branch-free (row clamped to N-1, inner index clamped to K-1: the clamped entries meet zero
 // rows of the staged B), and each staged chunk of KC inner entries is straight-line code with a
-// scheduling barrier after every refill, so the waits before each MFMA group are exact.
+// scheduling barrier after every refill, so the waits before each MFMA group are exact.  The
+// double-buffered variants load the next staged chunk into registers during the current one
+// (its L2 latency hidden behind the MFMAs) and need one barrier per chunk instead of two.
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const double gdbl;
 
@@ -831,7 +833,7 @@ __device__ inline zd ldg(const zd* p, size_t i) {
     return r;
 }
 
-template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB>
 __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
                                                                     int k, int K, const zd* __restrict__ small,
                                                                     int small_stride, zd* Bt, size_t bt_stride,
@@ -843,9 +845,11 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     }
     constexpr int BW = CT * 8;   // complex columns held (>= k)
     constexpr int NJ = CT * 16;  // real columns
-    constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64, NA = SPLIT ? 2 : 1;
+    constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64;
+    constexpr int NBUF = DB ? 2 : 1;
+    constexpr int PER = (KC * BW + NT - 1) / NT;  // staged B entries per thread and chunk
     static_assert(NS % DEPTH == 0, "ring slots line up across chunks");
-    __shared__ zd Bp[KC][NJ + 2];
+    __shared__ zd Bp[NBUF][KC][NJ + 2];
     const int f = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
@@ -856,11 +860,9 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     const zd* W0 = tb.W0[f];
     const zd* sm = small + (size_t)f * small_stride;
     const int Kc = K - 1;
-    d4v acc[NA][CT];
+    d4v acc[CT];
 #pragma unroll
-    for (int h = 0; h < NA; h++)
-#pragma unroll
-        for (int c = 0; c < CT; c++) acc[h][c] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < CT; c++) acc[c] = d4v{0.0, 0.0, 0.0, 0.0};
     auto ld = [&](int kb, zd (&a)[2]) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -868,17 +870,38 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             a[j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc) : ldg(A, (size_t)nc * d.Kmax + Kp);
         }
     };
-    auto body = [&](const zd (&a)[2], int kloc) {
+    auto body = [&](const zd (&a)[2], int kloc, int bsel) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             zd m[CT];
 #pragma unroll
-            for (int c = 0; c < CT; c++) m[c] = Bp[kloc + 2 * lq + j][c * 16 + lr];
+            for (int c = 0; c < CT; c++) m[c] = Bp[bsel][kloc + 2 * lq + j][c * 16 + lr];
 #pragma unroll
-            for (int c = 0; c < CT; c++) acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].x, m[c].x, acc[0][c], 0, 0, 0);
+            for (int c = 0; c < CT; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].x, m[c].x, acc[c], 0, 0, 0);
 #pragma unroll
-            for (int c = 0; c < CT; c++)
-                acc[NA - 1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].y, m[c].y, acc[NA - 1][c], 0, 0, 0);
+            for (int c = 0; c < CT; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].y, m[c].y, acc[c], 0, 0, 0);
+        }
+    };
+    // B chunk k0..k0+KC-1 (rows past K and columns past k are zero) into registers, then
+    // into LDS in operand form
+    auto bload = [&](int k0, zd (&r)[PER]) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            const int kk = e / BW, c = e - kk * BW;
+            r[u] = zmk(0.0, 0.0);
+            if (e < KC * BW && k0 + kk < K && c < k)
+                r[u] = ldg(sm, ROW ? (size_t)c * K + k0 + kk : (size_t)(k0 + kk) * k + c);
+        }
+    };
+    auto bstore = [&](const zd (&r)[PER], int bsel) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            if (e >= KC * BW) continue;
+            const int kk = e / BW, c = e - kk * BW;
+            Bp[bsel][kk][2 * c] = zmk(r[u].x, -r[u].y);
+            Bp[bsel][kk][2 * c + 1] = zmk(r[u].y, r[u].x);
         }
     };
     zd buf[DEPTH][2];
@@ -886,21 +909,45 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 #pragma unroll
         for (int q = 0; q < DEPTH; q++) ld(q * KS, buf[q]);
     }
-    for (int k0 = 0; k0 < K; k0 += KC) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < KC * BW; e += NT) {
-            const int kk = e / BW, c = e - kk * BW;
-            zd b = zmk(0.0, 0.0);
-            if (k0 + kk < K && c < k) b = ldg(sm, ROW ? (size_t)c * K + k0 + kk : (size_t)(k0 + kk) * k + c);
-            Bp[kk][2 * c] = zmk(b.x, -b.y);
-            Bp[kk][2 * c + 1] = zmk(b.y, b.x);
+    if (!DB) {
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            __syncthreads();
+            {
+                zd r[PER];
+                bload(k0, r);
+                bstore(r, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+                body(buf[i % DEPTH], i * KS, 0);
+                ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    } else if (K > 0) {
+        // double-buffered stage: the next chunk is loaded into registers while this one is
+        // multiplied and written to the other buffer afterwards; one barrier per chunk
+        {
+            zd r[PER];
+            bload(0, r);
+            bstore(r, 0);
         }
         __syncthreads();
+        int cur = 0;
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            const bool more = k0 + KC < K;
+            zd r[PER];
+            if (more) bload(k0 + KC, r);
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-            body(buf[i % DEPTH], i * KS);
-            ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
-            __builtin_amdgcn_sched_barrier(0);
+            for (int i = 0; i < NS; i++) {
+                body(buf[i % DEPTH], i * KS, cur);
+                ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (more) bstore(r, cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
         }
     }
 #pragma unroll
@@ -911,7 +958,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
         for (int v = 0; v < 4; v++) {
             const int row = n0 + lq + 4 * v;
             if (row >= d.N) continue;
-            const double a = NA == 2 ? acc[0][c][v] + acc[NA - 1][c][v] : acc[0][c][v];
+            const double a = acc[c][v];
             if (!ROW) {
                 const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
                 ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
@@ -923,15 +970,17 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     }
 }
 
-// Tile-shape variants offered to the autotuner, per column-tile count CT.  All of them issue
-// the MFMAs of an output tile in the same inner-index order into one accumulator, so every
-// variant gives bitwise the same result and the autotuner's choice cannot change a DBS run.
-template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool SPLIT>
+// Tile-shape variants offered to the autotuner, per column-tile count CT: (KC inner entries
+// per staged chunk, DEPTH sub-chunks in flight, waves per workgroup, double-buffered stage).
+// All of them issue the MFMAs of an output tile in the same inner-index order into one
+// accumulator, so every variant gives bitwise the same result and the autotuner's choice
+// cannot change a DBS run.
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                            const zd* small, int small_stride, zd* Bt, size_t bt_stride, const int32_t* dyn) {
     const int rows = 16 * WAVES;
     dim3 grid((d.N + rows - 1) / rows, d.nf);
-    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, SPLIT>), grid, dim3(WAVES * 64), 0, st, d, tb,
+    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, DB>), grid, dim3(WAVES * 64), 0, st, d, tb,
                        t, k, K, small, small_stride, Bt, bt_stride, dyn);
 }
 
@@ -939,20 +988,20 @@ template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                                const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant,
                                const int32_t* dyn) {
-#define PNGF_SKV(CT, KC, DEPTH, W, SPLIT) \
-    launch_variant<CT, ROW, KC, DEPTH, W, SPLIT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
+#define PNGF_SKV(CT, KC, DEPTH, W, DB) \
+    launch_variant<CT, ROW, KC, DEPTH, W, DB>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
     if (k <= 24) {
         switch (variant) {
-            case 1: PNGF_SKV(3, 32, 2, 4, false); break;
+            case 1: PNGF_SKV(3, 32, 4, 8, true); break;
             case 2: PNGF_SKV(3, 64, 2, 8, false); break;
-            case 3: PNGF_SKV(3, 32, 4, 8, false); break;
-            case 4: PNGF_SKV(3, 64, 4, 8, false); break;
-            case 5: PNGF_SKV(3, 16, 2, 8, false); break;
-            default: PNGF_SKV(3, 32, 2, 8, false); break;
+            case 3: PNGF_SKV(3, 32, 2, 8, false); break;
+            case 4: PNGF_SKV(3, 16, 2, 8, true); break;
+            case 5: PNGF_SKV(3, 32, 2, 4, true); break;
+            default: PNGF_SKV(3, 32, 2, 8, true); break;
         }
     } else if (k <= 48) {
         if (variant % 2) PNGF_SKV(6, 16, 2, 8, false);
-        else PNGF_SKV(6, 32, 2, 8, false);
+        else PNGF_SKV(6, 16, 2, 8, true);
     } else {
         if (variant % 2) PNGF_SKV(12, 16, 2, 4, false);
         else PNGF_SKV(12, 16, 2, 8, false);
