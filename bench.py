@@ -137,7 +137,7 @@ def bench_multi():
     for T in sorted(m["sizes"], reverse=True):
         for s in m["seeds"]:
             o = P.Optimizer(m["layout"], T, num_freq=len(m["freqs"]), method=P.METHOD_LOWRANK,
-                            max_deferred=m["max_deferred"])
+                            max_deferred=m["max_deferred"][T])
             for f, hz in enumerate(m["freqs"]):
                 o.set_frequency(f, hz)
                 o.synth_g(f)
