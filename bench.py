@@ -126,7 +126,7 @@ def copy_probe():
     print(json.dumps({"copy_bytes": n * 8, "copies": 3}))
 
 
-def bench_multi():
+def bench_multi(workers=4):
     """16 DBS instances of uneven size (11..25 tiles, 5 frequencies) on one GPU at once.
     Prints its own JSON line: total committed evaluations / wall seconds of the concurrent
     run (pngf_dbs_run_many: host worker threads, largest instances first)."""
@@ -148,7 +148,8 @@ def bench_multi():
             log("[multi] init %dx%d seed %d (%.1f s)" % (T, T, s, time.perf_counter() - t0))
     init_s = time.perf_counter() - t0
     t1 = time.perf_counter()
-    rs = P.dbs_run_many(opts)
+    rs = P.dbs_run_many(opts, workers=workers)
+    log("[multi] %d host workers" % workers)
     conc_s = time.perf_counter() - t1
     flips = sum(r["flip_count"] for r in rs)
     log("[multi] %d evaluations, %d accepts, %d folds in %.3f s" % (
@@ -181,6 +182,7 @@ def main():
                     help="skinny-product tile shape: 0 = autotune at init, v+1 = use variant v")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
+    ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
     args = ap.parse_args()
@@ -192,7 +194,7 @@ def main():
         copy_probe()
         return
     if args.workload == "multi16":
-        bench_multi()
+        bench_multi(args.workers)
         return
     wl = dict(WORKLOADS[args.workload])
     if args.tiles:
