@@ -190,6 +190,17 @@ typedef struct {
     float* log_x;
     float* log_y;
     int64_t log_cap;
+    /* Optional optimization history, the records of the reference's Logger (logger.hpp:64-73;
+       main.cpp:304-328, 401-426): record 0 is the start state (tile -1, decision -1), then one
+       record per committed evaluation in order.  hist_tile = primary tile of the flip,
+       hist_dec = 1 if F < F_best, hist_perf[i * (2 + 3 nf) + ...] = F, F_best before this
+       evaluation, then per frequency Re S11, Im S11 and the directivity D (0 for
+       PNGF_OBJ_S11).  Written until hist_cap records (pngf_dbs_run, first-improvement order;
+       the loop then runs host-driven). */
+    int32_t* hist_tile;
+    int32_t* hist_dec;
+    double* hist_perf;
+    int64_t hist_cap;
 } pngf_dbs_params;
 
 typedef struct {
@@ -207,6 +218,7 @@ typedef struct {
     int64_t full_sweeps;       /* ranked driver: exact full sweeps (hit-rate fallbacks) */
     int64_t ranked_steps;      /* ranked driver: steps that followed the network's order */
     int64_t ranked_hits;       /* ranked driver: those whose first top_k chunk held an improving flip */
+    int64_t hist_count;        /* history records written (pngf_dbs_params.hist_*) */
 } pngf_dbs_result;
 
 /* Runs the DBS loop of main.cpp:338-459 on the device with the driver chosen at create.

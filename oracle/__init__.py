@@ -66,7 +66,7 @@ def lib():
             "or_hat_rand_binary": (_I, [_P]),
             "or_mt19937_nth": (ct.c_uint, [ct.c_uint, _I]),
             "or_dbs_run": (_I, [_P, _I, _I, _P, _D, _D, _D, _I, _I, _I,
-                                _P, _P, _P, _P, _I, _P, _P, _D]),
+                                _P, _P, _P, _P, _I, _P, _P, _D, _P, _P, _P, _I, _P]),
             "or_num_threads": (_I, []),
         }
         for name, (res, args) in sig.items():
@@ -257,7 +257,7 @@ def mt19937_nth(seed, n):
 
 
 def dbs_run(opts, mode, freqs, dx, dy, dz, seed=0, max_evals=0, max_acc=100000, trace_cap=0,
-            max_seconds=0.0):
+            max_seconds=0.0, hist_cap=0):
     """Full DBS loop over all frequencies (main.cpp:338-459 or the raster S11 driver)."""
     nf = len(opts)
     arr = (ct.c_void_p * nf)(*[o.h for o in opts])
@@ -269,14 +269,22 @@ def dbs_run(opts, mode, freqs, dx, dy, dz, seed=0, max_evals=0, max_acc=100000, 
     fm = np.zeros(T * T, dtype=np.int32)
     st = np.zeros(6)
     fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    ht = np.zeros(max(hist_cap, 1), dtype=np.int32)
+    hd = np.zeros(max(hist_cap, 1), dtype=np.int32)
+    hp = np.zeros((max(hist_cap, 1), 2 + 3 * nf))
+    hc = np.zeros(1, dtype=np.int32)
+    h_on = hist_cap > 0
     rc = lib().or_dbs_run(arr, nf, mode, _ptr(fr), dx, dy, dz, seed, max_evals, max_acc,
                           _ptr(acc_t), _ptr(acc_e), _ptr(acc_F), _ptr(tr) if trace_cap else None,
-                          trace_cap, _ptr(fm), _ptr(st), max_seconds)
+                          trace_cap, _ptr(fm), _ptr(st), max_seconds,
+                          _ptr(ht) if h_on else None, _ptr(hd) if h_on else None, _ptr(hp) if h_on else None,
+                          hist_cap, _ptr(hc))
     na = int(st[1])
+    nh = int(hc[0])
     return dict(rc=rc, flip_count=int(st[0]), accepts=na, F_start=st[2], F_final=st[3],
                 loop_s=st[4], eval_s=st[5], acc_tiles=acc_t[:na].copy(), acc_eval=acc_e[:na].copy(),
                 acc_F=acc_F[:na].copy(), F_trace=tr[: min(trace_cap, int(st[0]))].copy(),
-                final_map=fm)
+                final_map=fm, hist_tile=ht[:nh].copy(), hist_dec=hd[:nh].copy(), hist_perf=hp[:nh].copy())
 
 
 def num_threads():

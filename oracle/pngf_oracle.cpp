@@ -583,6 +583,11 @@ unsigned or_mt19937_nth(unsigned seed, int n) {  // n-th output (1-based) of mt1
 //   acc_eval[]   flip_count at which it was accepted
 //   acc_F[]      objective after each accept
 //   F_trace[]    objective of every evaluation (may be null)
+//   hist_*       the Logger's history (logger.hpp:64-73; main.cpp:304-328, 401-426), may be
+//                null: record 0 is the start state (tile -1, decision -1), then one record per
+//                evaluation: primary tile, F < F_best, perf = F, F_best (before the
+//                evaluation), then per frequency Re S11, Im S11, D (0 in mode 1); at most
+//                hist_cap records, hist_count[0] receives the number written
 //   stats[0..5]  flip_count, accepts, F_start, F_final, loop seconds, eval seconds
 // max_seconds > 0 stops the loop before the first evaluation that would start
 // after that much wall time (a bounded sample for timing).
@@ -590,7 +595,8 @@ unsigned or_mt19937_nth(unsigned seed, int n) {  // n-th output (1-based) of mt1
 int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, double dy, double dz,
                int seed, int max_evals, int max_acc,
                int* acc_tiles, int* acc_eval, double* acc_F, double* F_trace, int trace_cap,
-               int* final_map, double* stats, double max_seconds) {
+               int* final_map, double* stats, double max_seconds,
+               int* hist_tile, int* hist_dec, double* hist_perf, int hist_cap, int* hist_count) {
     Opt** O = (Opt**)opts;
     Geo g{dx, dy, dz};
     const int T = O[0]->L.T, ne = O[0]->ne;
@@ -604,8 +610,35 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
         or_objective(mode, nf, (double*)E.data(), ne, (double*)Vi.data(), (double*)LN.data(), freqs, g.dx, g.dy, g.dz, &F, nullptr, nullptr);
         return F;
     };
+    const int nperf = 2 + 3 * nf;
+    std::vector<double> s11(2 * nf), dir(nf, 0.0);
+    int nhist = 0;
+    // one history record: objective and per-frequency S11 / D of the current (use_new = false)
+    // or the candidate's (use_new = true) Eobj
+    auto hist = [&](int tile, int dec, bool use_new, double F_best_now) {
+        if (!hist_perf || nhist >= hist_cap) return;
+        for (int f = 0; f < nf; f++) {
+            for (int r = 0; r < ne; r++) E[(size_t)f * ne + r] = use_new ? O[f]->newEobj[r] : O[f]->Eobj[r];
+            Vi[f] = O[f]->Vinc; LN[2 * f] = O[f]->LN1inc; LN[2 * f + 1] = O[f]->LN2inc;
+        }
+        double F;
+        or_objective(mode, nf, (double*)E.data(), ne, (double*)Vi.data(), (double*)LN.data(), freqs, g.dx, g.dy,
+                     g.dz, &F, s11.data(), dir.data());
+        double* pr = hist_perf + (size_t)nhist * nperf;
+        pr[0] = use_new ? F : F_best_now;
+        pr[1] = F_best_now;
+        for (int f = 0; f < nf; f++) {
+            pr[2 + 3 * f] = s11[2 * f];
+            pr[2 + 3 * f + 1] = s11[2 * f + 1];
+            pr[2 + 3 * f + 2] = mode == 0 ? dir[f] : 0.0;
+        }
+        hist_tile[nhist] = tile;
+        hist_dec[nhist] = dec;
+        nhist++;
+    };
     double F_best = objective(false);
     stats[2] = F_best;
+    hist(-1, -1, false, F_best);
     int flip_count = 0, nacc = 0;
     double eval_s = 0.0;
     auto t0 = std::chrono::high_resolution_clock::now();
@@ -659,6 +692,7 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
         double F = objective(true);
         eval_s += std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - e0).count();
         if (F_trace && flip_count < trace_cap) F_trace[flip_count] = F;
+        hist(prim, F < F_best ? 1 : 0, true, F_best);
         if (F < F_best) {
             if (nacc >= max_acc) { rc = -1; break; }
             acc_tiles[nacc] = prim; acc_eval[nacc] = flip_count; acc_F[nacc] = F;
@@ -675,6 +709,7 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
     double loop_s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
     for (int i = 0; i < T * T; i++) final_map[i] = O[0]->tile_map[i];
     stats[0] = flip_count; stats[1] = nacc; stats[3] = F_best; stats[4] = loop_s; stats[5] = eval_s;
+    if (hist_count) hist_count[0] = nhist;
     return rc;
 }
 

@@ -63,7 +63,9 @@ class DbsParams(ct.Structure):
                 ("acc_tiles", ct.c_void_p), ("acc_eval", ct.c_void_p), ("acc_F", ct.c_void_p),
                 ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32),
                 ("order", ct.c_int32), ("loop", ct.c_int32), ("reserved", ct.c_int32),
-                ("log_x", ct.c_void_p), ("log_y", ct.c_void_p), ("log_cap", ct.c_int64)]
+                ("log_x", ct.c_void_p), ("log_y", ct.c_void_p), ("log_cap", ct.c_int64),
+                ("hist_tile", ct.c_void_p), ("hist_dec", ct.c_void_p), ("hist_perf", ct.c_void_p),
+                ("hist_cap", ct.c_int64)]
 
 
 class DbsResult(ct.Structure):
@@ -71,7 +73,8 @@ class DbsResult(ct.Structure):
                 ("batches", ct.c_int64), ("F_start", ct.c_double), ("F_final", ct.c_double),
                 ("seconds", ct.c_double), ("accept_seconds", ct.c_double), ("flushes", ct.c_int32),
                 ("converged", ct.c_int32), ("logged", ct.c_int64), ("ranked_scored", ct.c_int64),
-                ("full_sweeps", ct.c_int64), ("ranked_steps", ct.c_int64), ("ranked_hits", ct.c_int64)]
+                ("full_sweeps", ct.c_int64), ("ranked_steps", ct.c_int64), ("ranked_hits", ct.c_int64),
+                ("hist_count", ct.c_int64)]
 
 
 class RankParams(ct.Structure):
@@ -167,6 +170,18 @@ def synth_g_host(layout, tiles, num_eobj, hz, cells=3, soft=0.0):
     if rc != 0:
         raise PNGFError(rc, "pngf_synth_g_host")
     return buf
+
+
+def write_history(path, seed, start_map, r, end_map):
+    """The reference Logger's text file (logger.hpp:75-107) from dbs_run(..., hist_cap=n):
+    "seed records", the start tile map as x/. in index order, one line "i tile decision
+    perf..." per record (C++ stream formatting: %g), the final tile map."""
+    with open(path, "w") as fh:
+        fh.write("%d %d\n" % (seed, len(r["hist_tile"])))
+        fh.write("".join("x" if v else "." for v in start_map) + "\n")
+        for i, (t, d, pr) in enumerate(zip(r["hist_tile"], r["hist_dec"], r["hist_perf"])):
+            fh.write("%d %d %d " % (i, t, d) + "".join("%g " % v for v in pr) + "\n")
+        fh.write("".join("x" if v else "." for v in end_map) + "\n")
 
 
 class Optimizer:
@@ -294,20 +309,27 @@ class Optimizer:
 
     # ---- DBS loop (main.cpp:338-459)
     def _dbs_params(self, max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap, order=0,
-                    loop=0):
+                    loop=0, hist_cap=0):
         acc = (np.zeros(max_accepts, dtype=np.int32), np.zeros(max_accepts, dtype=np.int64),
                np.zeros(max_accepts))
         log = None
         if log_cap:
             din, dout = self.ranker_dims()
             log = (np.zeros((log_cap, din), dtype=np.float32), np.zeros((log_cap, dout), dtype=np.float32))
+        hist = None
+        if hist_cap:
+            hist = (np.zeros(hist_cap, dtype=np.int32), np.zeros(hist_cap, dtype=np.int32),
+                    np.zeros((hist_cap, 2 + 3 * self.nf)))
         p = DbsParams(max_evals, seed, max_accepts, acc[0].ctypes.data, acc[1].ctypes.data, acc[2].ctypes.data,
                       batch_min, batch_max, rng_discard, order, loop, 0, log[0].ctypes.data if log else None,
-                      log[1].ctypes.data if log else None, log_cap)
-        return p, acc, log
+                      log[1].ctypes.data if log else None, log_cap,
+                      hist[0].ctypes.data if hist else None, hist[1].ctypes.data if hist else None,
+                      hist[2].ctypes.data if hist else None, hist_cap)
+        return p, acc, (log, hist)
 
     @staticmethod
-    def _dbs_result(r, acc, log):
+    def _dbs_result(r, acc, logs):
+        log, hist = logs
         na = r.accepts
         out = dict(flip_count=r.flip_count, accepts=na, speculative_evals=r.speculative_evals,
                    batches=r.batches, F_start=r.F_start, F_final=r.F_final, seconds=r.seconds,
@@ -318,12 +340,19 @@ class Optimizer:
         if log is not None:
             out["log_x"] = log[0][:r.logged].copy()
             out["log_y"] = log[1][:r.logged].copy()
+        if hist is not None:
+            n = r.hist_count
+            out["hist_tile"] = hist[0][:n].copy()
+            out["hist_dec"] = hist[1][:n].copy()
+            out["hist_perf"] = hist[2][:n].copy()
         return out
 
     def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0,
-                rng_discard=0, log_cap=0, order=ORDER_FIRST, loop=LOOP_AUTO):
+                rng_discard=0, log_cap=0, order=ORDER_FIRST, loop=LOOP_AUTO, hist_cap=0):
+        """DBS to convergence (or max_evals).  hist_cap > 0 also returns the reference Logger's
+        records (hist_tile, hist_dec, hist_perf; see write_history)."""
         p, acc, log = self._dbs_params(max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap,
-                                       order, loop)
+                                       order, loop, hist_cap)
         r = DbsResult()
         self._ck(lib().pngf_dbs_run(self.h, ct.byref(p), ct.byref(r)), "dbs_run")
         return self._dbs_result(r, acc, log)
@@ -458,4 +487,4 @@ def dbs_run_many(opts, max_evals=0, max_accepts=100000, workers=0):
     if rc != 0:
         msgs = [lib().pngf_last_error(o.h).decode() for o in opts]
         raise PNGFError(rc, "dbs_run_many: " + "; ".join(m for m in msgs if m))
-    return [Optimizer._dbs_result(rs[i], keep[i], None) for i in range(n)]
+    return [Optimizer._dbs_result(rs[i], keep[i], (None, None)) for i in range(n)]

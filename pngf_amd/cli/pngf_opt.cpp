@@ -8,12 +8,16 @@
 //     --driver ref|raster, --objective ref|s11, --method lowrank|plain
 //     --seed S           MagicHat seed (-514213645, main.cpp:154)
 //     --init empty|random (main.cpp:155, 187-219)
+//     --log INDEX        write the optimization history to log_INDEX.txt in the reference
+//                        Logger's format (logger.hpp:75-107; main.cpp:470-472), at most 10000
+//                        records (main.cpp:185)
 //
 // Prints the reference's start/finish report: initial and final design as ASCII art,
 // per-frequency |S11| (dB) and directivity, time elapsed, flips evaluated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <cstring>
 #include <random>
@@ -29,10 +33,29 @@ static void print_map(const std::vector<int>& tm, int T) {
     }
 }
 
+// The Logger's text file (logger.hpp:75-107): "seed records", the start tile map as x/. in
+// index order, one line "i tile decision perf..." per record, the final tile map.
+static void save_history(const std::string& fname, int seed, const std::vector<int>& start,
+                         const std::vector<int>& end, int64_t n, const int32_t* tile, const int32_t* dec,
+                         const double* perf, int nperf) {
+    std::ofstream out(fname);
+    out << seed << " " << n << std::endl;
+    for (int v : start) out << (v ? "x" : ".");
+    out << std::endl;
+    for (int64_t i = 0; i < n; i++) {
+        out << i << " " << tile[i] << " " << dec[i] << " ";
+        for (int j = 0; j < nperf; j++) out << perf[i * nperf + j] << " ";
+        out << std::endl;
+    }
+    for (int v : end) out << (v ? "x" : ".");
+    out << std::endl;
+}
+
 int main(int argc, char** argv) {
     int T = 21, layout = PNGF_LAYOUT_SHARED, driver = PNGF_DRIVER_REF, objective = PNGF_OBJ_REF;
     int method = PNGF_METHOD_LOWRANK, seed = -514213645;
     bool random_init = false;
+    int log_index = -1;
     std::string gdir;
     std::vector<double> freqs = {25.0, 27.5, 30.0, 32.5, 35.0};
     for (int i = 1; i < argc; i++) {
@@ -46,6 +69,7 @@ int main(int argc, char** argv) {
         else if (a == "--seed") seed = std::atoi(next().c_str());
         else if (a == "--init") random_init = next() == "random";
         else if (a == "--gdir") gdir = next();
+        else if (a == "--log") log_index = std::atoi(next().c_str());
         else if (a == "--freqs") {
             freqs.clear();
             std::string s = next();
@@ -113,7 +137,23 @@ int main(int argc, char** argv) {
         p.acc_eval = acc_e.data();
         p.acc_F = acc_F.data();
         p.rng_discard = random_init ? ((T - 1) / 2 + 1) * ((T - 1) / 2 + 1) : 0;
+        const int nperf = 2 + 3 * nf;
+        std::vector<int32_t> h_tile, h_dec;
+        std::vector<double> h_perf;
+        if (log_index >= 0) {
+            const int cap = 10000;
+            h_tile.resize(cap);
+            h_dec.resize(cap);
+            h_perf.resize((size_t)cap * nperf);
+            p.hist_tile = h_tile.data();
+            p.hist_dec = h_dec.data();
+            p.hist_perf = h_perf.data();
+            p.hist_cap = cap;
+        }
         pngf_dbs_result r = opt.dbs_run(p);
+        if (log_index >= 0)
+            save_history("log_" + std::to_string(log_index) + ".txt", seed, tm, opt.tile_map(), r.hist_count,
+                         h_tile.data(), h_dec.data(), h_perf.data(), nperf);
         std::printf("starting F: %g\n", r.F_start);
         for (int64_t i = 0; i < r.accepts; i++)
             std::printf("%lld: tile %d F %.10g\n", (long long)acc_e[i], acc_t[i], acc_F[i]);

@@ -59,22 +59,32 @@ constexpr double kC0 = 2.997924579998211E8;
 constexpr double kEta0 = 3.767303134118051e+02;
 constexpr double kPi = 3.14159265358979323846;
 
+// S11 at frequency f from that frequency's newEobj E (main.cpp:402-403) and, for mode 0, the
+// directivity D (main.cpp:404-406; 0 in mode 1).
+__host__ __device__ inline zd s11_of(const ObjParams& op, const FreqConst* fc, const zd* E, int f, double* D) {
+    zd u = zadd(E[0], fc[f].vinc);
+    zd zin = zmk(-u.x * op.zfac, -u.y * op.zfac);
+    zd s11 = zdiv(zmk(zin.x - op.z0, zin.y), zmk(zin.x + op.z0, zin.y));
+    *D = 0.0;
+    if (op.mode == 0) {
+        double k0 = fc[f].k0;
+        zd a = zadd(E[1], fc[f].ln1), b2 = zadd(E[2], fc[f].ln2);
+        double dd = (k0 * k0 / (32 * kPi * kPi * kEta0)) * (zabs2(a) + zabs2(b2));
+        double pin = -u.x * op.pfac;
+        *D = 4.0 * kPi * dd / pin;
+    }
+    return s11;
+}
+
 // Objective of one candidate from its newEobj at every frequency, in frequency
 // order (main.cpp:401-423): Zin, S11 and, for mode 0, the directivity D.
 __host__ __device__ inline double objective_of(const ObjParams& op, const FreqConst* fc, const zd* e) {
     double acc = 0.0;
     for (int f = 0; f < op.nf; f++) {
-        const zd* E = e + f * op.ne;
-        zd u = zadd(E[0], fc[f].vinc);
-        zd zin = zmk(-u.x * op.zfac, -u.y * op.zfac);
-        zd s11 = zdiv(zmk(zin.x - op.z0, zin.y), zmk(zin.x + op.z0, zin.y));
+        double D;
+        const zd s11 = s11_of(op, fc, e + f * op.ne, f, &D);
         double m2 = zabs2(s11);
         if (op.mode == 0) {
-            double k0 = fc[f].k0;
-            zd a = zadd(E[1], fc[f].ln1), b2 = zadd(E[2], fc[f].ln2);
-            double dd = (k0 * k0 / (32 * kPi * kPi * kEta0)) * (zabs2(a) + zabs2(b2));
-            double pin = -u.x * op.pfac;
-            double D = 4.0 * kPi * dd / pin;
             double m = sqrt(m2);
             acc += 200.0 * m * m + (15.0 - D) * (15.0 - D);
         } else {

@@ -764,6 +764,45 @@ int log_batch(pngf_handle* h, const pngf_dbs_params* p, int nb, int64_t* logged)
     return PNGF_OK;
 }
 
+// One record of the reference Logger (logger.hpp:64-73; main.cpp:304-328, 401-426): tile,
+// decision, F, F_best before the evaluation, then per frequency Re S11, Im S11 and D.
+void hist_record(pngf_handle* h, const pngf_dbs_params* p, int64_t* cnt, int tile, int dec, double F,
+                 double F_best, const zd* e) {
+    if (*cnt >= p->hist_cap) return;
+    const int nf = h->d.nf, ne = h->d.ne;
+    double* pr = p->hist_perf + (size_t)*cnt * (2 + 3 * nf);
+    pr[0] = F;
+    pr[1] = F_best;
+    for (int f = 0; f < nf; f++) {
+        double D;
+        const zd s = s11_of(h->op, h->fc.data(), e + (size_t)f * ne, f, &D);
+        pr[2 + 3 * f] = s.x;
+        pr[3 + 3 * f] = s.y;
+        pr[4 + 3 * f] = D;
+    }
+    p->hist_tile[*cnt] = tile;
+    p->hist_dec[*cnt] = dec;
+    (*cnt)++;
+}
+
+bool hist_on(const pngf_dbs_params* p) { return p->hist_tile && p->hist_dec && p->hist_perf && p->hist_cap > 0; }
+
+// Records for the first n candidates of the batch just scored (the ones the sequential loop
+// evaluates: up to and including the accepted one), all against the same F_best.
+int hist_batch(pngf_handle* h, const pngf_dbs_params* p, const int* groups, int n, double F_best, int64_t* cnt) {
+    if (!hist_on(p) || *cnt >= p->hist_cap) return PNGF_OK;
+    n = (int)std::min<int64_t>(n, p->hist_cap - *cnt);
+    const size_t per = (size_t)h->d.nf * h->d.ne;
+    std::vector<zd> e((size_t)n * per);
+    std::vector<double> F(n);
+    HC(h, hipMemcpyAsync(e.data(), h->d_eobj_out, e.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
+    HC(h, hipMemcpyAsync(F.data(), h->d_F, F.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    for (int b = 0; b < n; b++)
+        hist_record(h, p, cnt, h->grp.prim[groups[b]], F[b] < F_best ? 1 : 0, F[b], F_best, e.data() + b * per);
+    return PNGF_OK;
+}
+
 int download_eobj(pngf_handle* h, std::vector<zd>& e) {
     e.resize((size_t)h->d.nf * h->d.ne);
     HC(h, hipMemcpyAsync(e.data(), h->vecs.Eobj, e.size() * sizeof(zd), hipMemcpyDeviceToHost, h->st));
@@ -1388,7 +1427,7 @@ constexpr int kDefaultWorkers = 4;
 
 bool device_loop_ok(const pngf_handle* h, const pngf_dbs_params* p) {
     return p->loop == PNGF_LOOP_AUTO && h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_FIRST &&
-           h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !h->prof && h->d.gmax <= kLdsGmax;
+           h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !hist_on(p) && !h->prof && h->d.gmax <= kLdsGmax;
 }
 
 int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best, int bmin, int bmax) {
@@ -1568,10 +1607,13 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     std::memset(r, 0, sizeof(*r));
     if (p->log_x && h->cfg.method != PNGF_METHOD_LOWRANK)
         return fail(h, PNGF_ERR_STATE, "the training log needs the LOWRANK method");
+    if (hist_on(p) && p->order == PNGF_ORDER_BEST)
+        return fail(h, PNGF_ERR_ARG, "the optimization history follows first-improvement order");
     std::vector<zd> e;
     CK(download_eobj(h, e));
     double F_best = host_objective(h, e.data());
     r->F_start = F_best;
+    if (hist_on(p)) hist_record(h, p, &r->hist_count, -1, -1, F_best, F_best, e.data());
     const bool plain = h->cfg.method == PNGF_METHOD_PLAIN;
     // speculative batch after an accept: PLAIN scores are costly (4), LOWRANK ones cheap (32)
     const int bmin = p->batch_min > 0 ? p->batch_min : (plain ? 4 : 32);
@@ -1645,6 +1687,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             CK(sync_sel(h, (int)B, &sel, &Fs, true));
             batches++;
             CK(log_batch(h, p, (int)B, &r->logged));
+            CK(hist_batch(h, p, batch.data(), sel >= 0 ? sel + 1 : (int)B, F_best, &r->hist_count));
             if (sel >= 0) {
                 flip_count += sel;
                 if ((rc = record(batch[sel], Fs)) != PNGF_OK) break;
@@ -1696,6 +1739,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             batches++;
             CK(log_batch(h, p, (int)batch.size(), &r->logged));
             const int upto = sel >= 0 ? sel : (int)batch.size() - 1;
+            CK(hist_batch(h, p, batch.data(), upto + 1, F_best, &r->hist_count));
             // replay the real hat through the evaluated draws
             int cnt = 0;
             while (true) {

@@ -120,3 +120,19 @@ def test_struct_mirrors_match_header(cname, py):
             assert pt.__name__ in ("c_void_p",) or pt.__name__.startswith("LP_"), (f, pt)
         else:
             assert pt.__name__ == CT_OF[ct_], (f, ct_, pt)
+
+
+def test_write_history_layout(tmp_path):
+    """write_history follows the Logger's text layout (logger.hpp:75-107) with C++ stream
+    formatting of the values."""
+    r = dict(hist_tile=np.array([-1, 4, 9], dtype=np.int32), hist_dec=np.array([-1, 0, 1], dtype=np.int32),
+             hist_perf=np.array([[0.5, 0.5, 0.1, -0.2, 0.0], [0.6, 0.5, 0.125, 1e-5, 0.0],
+                                 [0.25, 0.5, -1.5, 2.0, 12.345678]]))
+    P.write_history(str(tmp_path / "log.txt"), -514213645, [1, 0, 1, 1], r, [0, 0, 1, 1])
+    assert (tmp_path / "log.txt").read_text() == (
+        "-514213645 3\n"
+        "x.xx\n"
+        "0 -1 -1 0.5 0.5 0.1 -0.2 0 \n"
+        "1 4 0 0.6 0.5 0.125 1e-05 0 \n"
+        "2 9 1 0.25 0.5 -1.5 2 12.3457 \n"
+        "..xx\n")

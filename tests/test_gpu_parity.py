@@ -9,6 +9,8 @@ has cond(C) ~ 1e5..1e6; there two FP64 implementations that round differently
 agree only to ~cond * eps * (accepts), so that case is checked for the identical
 accepted sequence and the objective within RTOL_ILL.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -162,6 +164,53 @@ def test_dbs_reference_driver(method, symmetry):
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
     assert np.array_equal(g["acc_eval"], o["acc_eval"])
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
+@pytest.mark.parametrize("case", ["raster_s11_lowrank", "raster_s11_plain", "ref_lowrank"])
+def test_history_matches_oracle(case):
+    """The reference Logger's records (start state, then one per evaluation: tile, decision,
+    F, F_best, per-frequency Re S11, Im S11, D) from the GPU loop equal the oracle's: same
+    tiles and decisions, values within RTOL."""
+    freqs = freq_list(2, 29e9, 31e9)
+    if case.startswith("raster"):
+        method = P.METHOD_LOWRANK if case.endswith("lowrank") else P.METHOD_PLAIN
+        gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, 7, freqs, method)
+        mode, seed, cap = O.OBJ_S11, 0, 400
+    else:
+        gpu, ors, tm = setup_pair(P.LAYOUT_SHARED, 7, freqs, P.METHOD_LOWRANK, ne=3, driver=P.DRIVER_REF,
+                                  objective=P.OBJ_REF, max_flips=4, seed=1)
+        mode, seed, cap = O.OBJ_REF, -514213645, 400
+    g = gpu.dbs_run(seed=seed, max_evals=cap - 1, hist_cap=cap)
+    o = O.dbs_run(ors, mode, freqs, DX, DY, DZ, seed=seed, max_evals=cap - 1, hist_cap=cap)
+    n = len(o["hist_tile"])
+    assert n == o["flip_count"] + 1 and len(g["hist_tile"]) == n and g["flip_count"] == o["flip_count"]
+    assert np.array_equal(g["hist_tile"], o["hist_tile"])
+    assert np.array_equal(g["hist_dec"], o["hist_dec"])
+    assert g["hist_perf"].shape == o["hist_perf"].shape == (n, 2 + 3 * len(freqs))
+    assert rel(g["hist_perf"], o["hist_perf"]) < RTOL
+    assert np.array_equal(np.nonzero(g["hist_dec"][1:] == 1)[0], g["acc_eval"])
+
+
+def test_history_file_from_cli(tmp_path):
+    """pngf_opt --log 3 writes log_3.txt in the Logger's layout: 'seed records', the start map,
+    one line per record (index, tile, decision, 2 + 3 nf values), the final map."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(P.__file__), "cli", "pngf_opt")
+    cmd = [exe, "--tiles", "7", "--layout", "disjoint", "--driver", "raster", "--objective", "s11",
+           "--freqs", "29,31", "--log", "3"]
+    out = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = (tmp_path / "log_3.txt").read_text().splitlines()
+    seed, n = map(int, lines[0].split())
+    assert seed == -514213645 and n == len(lines) - 3 and n > 1
+    assert len(lines[1]) == 49 and set(lines[1]) <= set("x.")
+    assert len(lines[-1]) == 49 and set(lines[-1]) <= set("x.")
+    flips = int(out.stdout.split("Converged after ")[1].split()[0])
+    assert n == flips + 1
+    rec = [l.split() for l in lines[2:-1]]
+    assert all(len(r) == 3 + 2 + 3 * 2 for r in rec)
+    assert rec[0][:3] == ["0", "-1", "-1"]
+    assert [int(r[0]) for r in rec] == list(range(n))
 
 
 @pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])

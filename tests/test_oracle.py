@@ -272,3 +272,45 @@ def test_empty_pattern_closed_form():
     zin = -vinc * DX * 0.5 / (2 * DY * DZ)
     F, s, _ = O.objective(O.OBJ_S11, sc["Eobj"][None, :], np.array([sc["Vinc"]]), None, [30e9], DX, DY, DZ)
     assert abs(s[0] - (zin - 50) / (zin + 50)) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [O.OBJ_S11, O.OBJ_REF])
+def test_history_records(mode):
+    """The Logger history of the oracle's DBS loop (main.cpp:304-328, 401-426): a start record,
+    then one record per evaluation whose F column is the evaluation's objective, whose F_best
+    column is the best objective before it, whose decision is F < F_best, whose tile is the
+    accepted tile where the loop accepted, and whose per-frequency S11 / D reproduce F through
+    the objective (mean |S11|^2, or sum 200 |S11|^2 + (15 - D)^2)."""
+    kind, T, ne = (O.LAYOUT_DISJOINT, 3, 1) if mode == O.OBJ_S11 else (O.LAYOUT_SHARED, 5, 3)
+    freqs = [29e9, 31e9]
+    opts = []
+    for f in range(len(freqs)):
+        G, extra, fb, tm = random_problem(kind, T, ne, 40 + f)
+        fb *= 1e-3
+        o = O.Optimizer(kind, T, fb, ne=ne, max_flips=4)   # the reference driver flips up to 4 tiles
+        mid = (T - 1) // 2
+        tm0 = random_problem(kind, T, ne, 40)[3]
+        assert o.init_state(tm0, (mid, mid, 1, 2)) == 0
+        opts.append(o)
+    r = O.dbs_run(opts, mode, freqs, DX, DY, DZ, seed=-514213645, max_evals=40, trace_cap=64, hist_cap=64)
+    n = r["flip_count"]
+    assert n > 0 and len(r["hist_tile"]) == n + 1
+    perf, dec, tile = r["hist_perf"], r["hist_dec"], r["hist_tile"]
+    assert perf.shape[1] == 2 + 3 * len(freqs)
+    assert tile[0] == -1 and dec[0] == -1 and perf[0, 0] == r["F_start"] and perf[0, 1] == r["F_start"]
+    assert np.array_equal(perf[1:, 0], r["F_trace"])
+    best = r["F_start"]
+    for i in range(1, n + 1):
+        assert perf[i, 1] == best
+        assert dec[i] == (1 if perf[i, 0] < best else 0)
+        best = min(best, perf[i, 0])
+    assert np.array_equal(tile[r["acc_eval"] + 1], r["acc_tiles"])
+    assert np.array_equal(np.nonzero(dec[1:] == 1)[0], r["acc_eval"])
+    s = perf[:, 2::3] + 1j * perf[:, 3::3]
+    D = perf[:, 4::3]
+    if mode == O.OBJ_S11:
+        assert np.all(D == 0.0)
+        F = np.mean(np.abs(s) ** 2, axis=1)
+    else:
+        F = np.sum(200 * np.abs(s) ** 2 + (15 - D) ** 2, axis=1)
+    assert np.max(np.abs(F - perf[:, 0]) / np.abs(perf[:, 0])) < 1e-12
