@@ -179,36 +179,31 @@ __device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, c
 // Rows stay in place: perm[i] is the storage row of logical row i, the order that zgetf2's
 // row interchanges produce (pivot = first remaining logical row of max |Re|+|Im|).  Step j
 // removes column j from every other row and writes only columns > j (column j of the other
-// rows is not read again), so a step needs two barriers.  On return storage row perm[i] holds
-// d_i in column i and d_i times the solution in columns k..ncol-1.  Returns LAPACK-style info
-// (0, or j+1 for the first zero pivot, whose elimination step is skipped).  All threads of the
-// block call it; blockDim.x is a multiple of 32 and at least 64.
+// rows is not read again).  On return storage row perm[i] holds d_i in column i and d_i times
+// the solution in columns k..ncol-1.  Returns LAPACK-style info (0, or j+1 for the first zero
+// pivot, whose elimination step is skipped).  All threads of the block call it, after a barrier
+// that follows the writes of a; blockDim.x is a multiple of 64.
 __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
-    __shared__ int s_info;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int ri = tid & 31, cg = tid >> 5, ncg = blockDim.x >> 5;
-    if (tid < k) perm[tid] = tid;
-    if (tid == 0) s_info = 0;
-    __syncthreads();
+    // every wave keeps its own copy of the row permutation (lane i: position i) and repeats the
+    // pivot search, so one barrier per column suffices: the search for column j only reads
+    // column j, which the elimination of column j does not write
+    int pm = lane;
+    int info = 0;
     for (int j = 0; j < k; j++) {
-        if (tid < 64) {
-            const int i = j + tid;
-            double best = i < k ? zabs1(a[perm[i] * lda + j]) : -1.0;
-            int bi = i;
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-            }
-            if (tid == 0) {
-                const int pj = perm[j];
-                perm[j] = perm[bi];
-                perm[bi] = pj;
-                if (best == 0.0 && s_info == 0) s_info = j + 1;
-            }
+        double best = lane >= j && lane < k ? zabs1(a[pm * lda + j]) : -1.0;
+        int bi = lane;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        __syncthreads();
-        const int pr = perm[j];
+        const int pj = __shfl(pm, j, 64), pb = __shfl(pm, bi, 64);
+        if (lane == j) pm = pb;
+        else if (lane == bi) pm = pj;
+        if (best == 0.0 && info == 0) info = j + 1;
+        const int pr = pb;
         const zd pv = a[pr * lda + j];
         if ((pv.x != 0.0 || pv.y != 0.0) && ri < k && ri != pr) {
             const zd l = zmul(a[ri * lda + j], zrecip(pv));
@@ -216,7 +211,9 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
         }
         __syncthreads();
     }
-    return s_info;
+    if (tid < k) perm[tid] = pm;
+    __syncthreads();
+    return info;
 }
 
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
