@@ -69,7 +69,7 @@ struct LoopCtl {
 void launch_loop_batch(hipStream_t st, LoopState* ls, const int32_t* cl, int32_t* batch);
 void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32_t* tile, const int8_t* P,
                          const int32_t* owners, int32_t* batch, const zd* eobj, const int32_t* info,
-                         const LoopCtl& c);
+                         const LoopCtl& c, int32_t* iterlog = nullptr);
 
 // ---- eval
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,

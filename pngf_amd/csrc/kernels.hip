@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(256) loop_control_kernel(Dims d, GroupTab gt, 
                                                            const int8_t* __restrict__ P,
                                                            const int32_t* __restrict__ owners, int32_t* batch,
                                                            const zd* __restrict__ eobj,
-                                                           const int32_t* __restrict__ info, LoopCtl c) {
+                                                           const int32_t* __restrict__ info, LoopCtl c,
+                                                           int32_t* iterlog) {
     __shared__ int s_red[256];
     __shared__ double s_F;
     __shared__ int s_B, s_pos, s_ncl;
@@ -376,12 +377,19 @@ __global__ void __launch_bounds__(256) loop_control_kernel(Dims d, GroupTab gt, 
     if (tid == 0) loop_next_batch(ls, &s_B, &s_pos, &s_ncl);
     __syncthreads();
     for (int i = tid; i < s_B; i += nth) batch[i] = c.cl[(s_pos + i) % s_ncl];
+    // this iteration's accept as the timing of its column strip needs it: pending, k, K
+    if (iterlog && tid == 0) {
+        iterlog[0] = ls->dyn[kDynActive];
+        iterlog[1] = ls->dyn[kDynK];
+        iterlog[2] = ls->dyn[kDynKK];
+    }
 }
 
 void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32_t* tile, const int8_t* P,
                          const int32_t* owners, int32_t* batch, const zd* eobj, const int32_t* info,
-                         const LoopCtl& c) {
-    hipLaunchKernelGGL(loop_control_kernel, dim3(1), dim3(256), 0, st, d, gt, tile, P, owners, batch, eobj, info, c);
+                         const LoopCtl& c, int32_t* iterlog) {
+    hipLaunchKernelGGL(loop_control_kernel, dim3(1), dim3(256), 0, st, d, gt, tile, P, owners, batch, eobj, info, c,
+                       iterlog);
 }
 
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,

@@ -27,6 +27,9 @@ namespace {
 
 typedef rocblas_double_complex rbz;
 
+// iterations of the device-driven DBS loop per host synchronisation (and per captured graph)
+constexpr int kLoopChunkMax = 16;
+
 struct Timed {
     hipEvent_t a, b;
     int which;
@@ -136,6 +139,11 @@ struct pngf_handle {
     // disjoint; d_left = the components outside every group
     bool part = false;
     int32_t* d_left = nullptr;
+    // device-driven loop with profiling on: events around each iteration's column strip and the
+    // iteration's (pending, k, K) record, read after every chunk
+    hipEvent_t lev[2 * kLoopChunkMax] = {};
+    int32_t* d_iterlog = nullptr;
+    int32_t* h_iterlog = nullptr;
     int nleft = 0;
     hipGraph_t loop_graph = nullptr;
     hipGraphExec_t loop_exec = nullptr;
@@ -921,6 +929,8 @@ int pngf_destroy(pngf_handle* h) {
     if (h->h_pin_i) (void)hipHostFree(h->h_pin_i);
     if (h->h_pin_d) (void)hipHostFree(h->h_pin_d);
     if (h->h_loop) (void)hipHostFree(h->h_loop);
+    if (h->h_iterlog) (void)hipHostFree(h->h_iterlog);
+    for (auto e : h->lev) if (e) (void)hipEventDestroy(e);
     if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
     if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
     if (h->rb) rocblas_destroy_handle(h->rb);
@@ -1421,13 +1431,13 @@ int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out) {
 // iterations, each of which accepts at most one flip, and synchronises once per chunk; the
 // factor capacity is checked (and folded) between chunks.  Same batches, same selection and
 // the same kernels as the host loop, so the accepted sequence is identical.
-constexpr int kLoopChunk = 16;
+constexpr int kLoopChunk = kLoopChunkMax;
 // host threads of pngf_dbs_run_many
 constexpr int kDefaultWorkers = 4;
 
 bool device_loop_ok(const pngf_handle* h, const pngf_dbs_params* p) {
     return p->loop == PNGF_LOOP_AUTO && h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_FIRST &&
-           h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !hist_on(p) && !h->prof && h->d.gmax <= kLdsGmax;
+           h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !hist_on(p) && h->d.gmax <= kLdsGmax;
 }
 
 int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best, int bmin, int bmax) {
@@ -1457,6 +1467,11 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
         CK(dalloc_t(h, &h->d_accF, cap));
         h->acc_alloc = cap;
     }
+    if (h->prof && !h->d_iterlog) {
+        CK(dalloc_t(h, &h->d_iterlog, 3 * kLoopChunk));
+        HC(h, hipHostMalloc((void**)&h->h_iterlog, 3 * kLoopChunk * sizeof(int32_t), hipHostMallocDefault));
+        for (auto& e : h->lev) HC(h, hipEventCreate(&e));
+    }
     LoopState& ls = *h->h_loop;
     std::memset(&ls, 0, sizeof(ls));
     ls.max_evals = p->max_evals;
@@ -1476,8 +1491,11 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
     return PNGF_OK;
 }
 
-void device_loop_iteration(pngf_handle* h) {
+// Iteration i of a chunk.  With profiling on, events bracket the column strip and the control
+// kernel records (pending, k, K) for device_loop_collect.
+void device_loop_iteration(pngf_handle* h, int i) {
     hipStream_t st = h->st;
+    const bool tm = h->prof && h->d_iterlog;
     const Dims& d = h->d;
     LoopState* ls = h->d_loop;
     const int32_t* dyn = ls->dyn;
@@ -1501,7 +1519,8 @@ void device_loop_iteration(pngf_handle* h) {
     c.acc_F = h->d_accF;
     launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, ncl, h->d_eobj_out,
                    h->d_info, h->d_lu, &ls->nb);
-    launch_loop_control(st, d, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_eobj_out, h->d_info, c);
+    launch_loop_control(st, d, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_eobj_out, h->d_info, c,
+                        tm ? h->d_iterlog + 3 * i : nullptr);
     (void)hipEventRecord(h->ev_fork, st);
     (void)hipStreamWaitEvent(h->st2, h->ev_fork, 0);
     launch_lr_accept_prep(h->st2, d, h->vecs, 0, dt, ds, dlp, d.gmax, h->d_acc, h->acc_stride, h->d_info, h->d_lu, dyn,
@@ -1509,7 +1528,9 @@ void device_loop_iteration(pngf_handle* h) {
     (void)hipEventRecord(h->ev_join, h->st2);
     // host k = gmax selects the column-tile count; the kernels read the actual k and K
     launch_lr_gather(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride, dyn);
+    if (tm) (void)hipEventRecord(h->lev[2 * i], st);
     launch_lr_colblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, (int)h->small_stride, h->skv, dyn);
+    if (tm) (void)hipEventRecord(h->lev[2 * i + 1], st);
     if (!h->sym)
         launch_lr_rowblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv,
                          dyn);
@@ -1523,11 +1544,11 @@ void device_loop_iteration(pngf_handle* h) {
 // the runtime refuses falls back to direct launches.  Relaxed capture mode: other threads
 // driving other handles keep calling the runtime meanwhile.
 void device_loop_capture(pngf_handle* h) {
-    if (h->loop_exec || h->graph_failed) return;
+    if (h->loop_exec || h->graph_failed || (h->prof && h->d_iterlog)) return;
     hipStream_t st = h->st;
     bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
     if (ok) {
-        for (int i = 0; i < kLoopChunk; i++) device_loop_iteration(h);
+        for (int i = 0; i < kLoopChunk; i++) device_loop_iteration(h, i);
         ok = hipStreamEndCapture(st, &h->loop_graph) == hipSuccess && h->loop_graph;
         ok = ok && hipGraphInstantiate(&h->loop_exec, h->loop_graph, nullptr, nullptr, 0) == hipSuccess;
     }
@@ -1561,14 +1582,34 @@ int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_coun
             S = h->Kmax / gmax;
         }
         S = std::min(S, kLoopChunk);
-        if (S == kLoopChunk) device_loop_capture(h);
-        if (S == kLoopChunk && h->loop_exec)
+        // with profiling on, the iterations are launched directly: elapsed times of events that a
+        // graph replay recorded are not available
+        const bool tm = h->prof && h->d_iterlog;
+        if (S == kLoopChunk && !tm) device_loop_capture(h);
+        if (S == kLoopChunk && !tm && h->loop_exec)
             HC(h, hipGraphLaunch(h->loop_exec, st));
         else
-            for (int i = 0; i < S; i++) device_loop_iteration(h);
+            for (int i = 0; i < S; i++) device_loop_iteration(h, i);
         HC(h, hipGetLastError());
+        if (tm)
+            HC(h, hipMemcpyAsync(h->h_iterlog, h->d_iterlog, 3 * S * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HC(h, hipMemcpyAsync(&ls, h->d_loop, sizeof(LoopState), hipMemcpyDeviceToHost, st));
         HC(h, hipStreamSynchronize(st));
+        if (tm) {
+            // event time of every column strip that ran (iterations with an accept)
+            const Dims& d = h->d;
+            auto& ks = h->kst[PNGF_K_COLBLK];
+            for (int i = 0; i < S; i++) {
+                const int32_t* r = h->h_iterlog + 3 * i;
+                float ms = 0.f;
+                if (!r[0] || hipEventElapsedTime(&ms, h->lev[2 * i], h->lev[2 * i + 1]) != hipSuccess) continue;
+                const double N = d.N, kd = r[1], Kd = r[2];
+                ks.launches++;
+                ks.total_ms += ms;
+                ks.bytes += d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd);
+                ks.flops += d.nf * 8.0 * N * Kd * kd;
+            }
+        }
         Kknown = ls.dyn[kDynKK] + (ls.dyn[kDynActive] ? ls.dyn[kDynK] : 0);
         if (ls.done) break;
     }
