@@ -8,9 +8,16 @@ namespace pngf {
 struct DevTabs {
     zd** W0;     // LOWRANK: N x ldN row-major
     zd** Aall;   // LOWRANK: N x Kmax row-major
-    zd** Mall;   // LOWRANK: Kmax x ldN row-major
+    zd** Mall;   // LOWRANK: Kmax x ldN row-major (null when the factors are kept as Lam)
     zd** Grow;   // PLAIN: G row-major
     zd** Cinv;   // PLAIN: C^-1 column-major
+    // LOWRANK, symmetric W, groups of <= kLdsGmax components: M is not stored.  Row K' of M
+    // comes from the accept that appended A column K' (columns fstart[K'] .. + flen[K'] - 1):
+    //   M[K'][n] = sum_l Lam[K'][l] A[n][fstart[K'] + l],  Lam[K'] = s_c cap^-1[c][:]
+    // (c = K' - fstart[K']), which is M_new = diag(s) cap^-1 W[t,:] with W[t,:] = A_new^T.
+    zd** Lam;          // [Kmax][gmax] per frequency
+    int32_t* fstart;   // [Kmax]
+    int32_t* flen;     // [Kmax]
 };
 
 // Contiguous per-frequency small arrays.
@@ -30,6 +37,8 @@ struct Dims {
     // LOWRANK with symmetric W: W0 and every cap^-1-derived factor are kept exactly symmetric
     // and the diagonal blocks are read as (D + D^T) / 2
     int sym;
+    // factors kept as A and Lam (no M; symmetric W, groups of <= kLdsGmax components)
+    int lam;
 };
 
 // gmax above which the k x (k+1) capacitance matrix moves from LDS to global scratch
@@ -94,6 +103,9 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
                       zd* Mc, zd* Ar, int small_stride, const int32_t* dyn = nullptr);
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Mc, int small_stride, int variant, const int32_t* dyn = nullptr);
+// Rows kk0 .. kk0 + nrows - 1 of M (as ld-strided rows of out, all N columns) from Lam and A
+// (the factors kept without M); one frequency f.
+void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, int kk0, int nrows, zd* out, int ld);
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant,
                       const int32_t* dyn = nullptr);

@@ -775,6 +775,18 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int 
                        acc_stride, info, gscratch, dyn, g_dev);
 }
 
+// M[kk][n]: stored, or from Lam and A (see DevTabs); the sum runs in the order the accept's
+// epilogue used when it stored M rows, with the sign folded into Lam.
+__device__ inline zd m_entry(const DevTabs& tb, const Dims& d, int f, int kk, int n) {
+    if (!d.lam) return tb.Mall[f][(size_t)kk * d.ldN + n];
+    const zd* L = tb.Lam[f] + (size_t)kk * d.gmax;
+    const zd* A = tb.Aall[f] + (size_t)n * d.Kmax + tb.fstart[kk];
+    const int len = tb.flen[kk];
+    zd s = zmk(0.0, 0.0);
+    for (int l = 0; l < len; l++) s = zfma(L[l], A[l], s);
+    return s;
+}
+
 // Mc[K'][c] = M_all[K'][t_c] (K x k) and, unless Ar is null, Ar[c][K'] = A_all[t_c][K'] (k x K).
 __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, int K, zd* Mc,
                                  zd* Ar, int small_stride, const int32_t* dyn) {
@@ -784,26 +796,91 @@ __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__
         k = dyn[kDynK];
         K = dyn[kDynKK];
     }
-    const zd* M = tb.Mall[f];
     const zd* A = tb.Aall[f];
     zd* mc = Mc + (size_t)f * small_stride;
     zd* ar = Ar ? Ar + (size_t)f * small_stride : nullptr;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * k; e += gridDim.x * blockDim.x) {
         const int kk = e / k, c = e - kk * k;
-        mc[e] = M[(size_t)kk * d.ldN + t[c]];
+        mc[e] = m_entry(tb, d, f, kk, t[c]);
         if (!Ar) continue;
         const int c2 = e / K, k2 = e - c2 * K;
         ar[e] = A[(size_t)t[c2] * d.Kmax + k2];
     }
 }
 
+// Mc[K'][c] = M[K'][t_c] with M rebuilt from Lam and A (factors kept without M): a workgroup
+// takes 32 factor rows of one frequency and, one accept (common fstart) at a time, stages that
+// accept's A columns of the rows t in LDS; the same sums as m_entry.
+constexpr int kGatherRows = 32;
+__global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
+                                                           int K, zd* Mc, int small_stride, const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
+    extern __shared__ zd sml[];
+    const int f = blockIdx.y, tid = threadIdx.x, gmax = d.gmax;
+    zd* Ls = sml;                        // [kGatherRows][gmax] Lam rows
+    zd* As = sml + kGatherRows * gmax;   // [k][gmax] A[t_c][fs .. fs + len)
+    __shared__ int s_fs[kGatherRows], s_len[kGatherRows];
+    const zd* A = tb.Aall[f];
+    const zd* Lam = tb.Lam[f];
+    zd* mc = Mc + (size_t)f * small_stride;
+    for (int r0 = blockIdx.x * kGatherRows; r0 < K; r0 += gridDim.x * kGatherRows) {
+        const int nr = min(kGatherRows, K - r0);
+        __syncthreads();
+        for (int e = tid; e < nr * gmax; e += blockDim.x) Ls[e] = Lam[(size_t)r0 * gmax + e];
+        if (tid < nr) {
+            s_fs[tid] = tb.fstart[r0 + tid];
+            s_len[tid] = tb.flen[r0 + tid];
+        }
+        __syncthreads();
+        for (int rr = 0; rr < nr;) {
+            const int fs = s_fs[rr], len = s_len[rr];
+            int rr2 = rr + 1;
+            while (rr2 < nr && s_fs[rr2] == fs) rr2++;
+            for (int e = tid; e < k * len; e += blockDim.x) {
+                const int c = e / len, l = e - c * len;
+                As[c * gmax + l] = A[(size_t)t[c] * d.Kmax + fs + l];
+            }
+            __syncthreads();
+            for (int e = tid; e < (rr2 - rr) * k; e += blockDim.x) {
+                const int r = rr + e / k, c = e - (r - rr) * k;
+                zd acc = zmk(0.0, 0.0);
+                for (int l = 0; l < len; l++) acc = zfma(Ls[r * gmax + l], As[c * gmax + l], acc);
+                mc[(size_t)(r0 + r) * k + c] = acc;
+            }
+            __syncthreads();
+            rr = rr2;
+        }
+    }
+}
+
 void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       zd* Mc, zd* Ar, int small_stride, const int32_t* dyn) {
     if (!dyn && K == 0) return;
+    if (d.lam) {
+        const int rows = dyn ? d.Kmax : K;
+        const int blocks = std::min(1024, (rows + kGatherRows - 1) / kGatherRows);
+        const size_t sm = (size_t)(kGatherRows + d.gmax) * d.gmax * sizeof(zd);
+        hipLaunchKernelGGL(lr_gather_lam_kernel, dim3(blocks, d.nf), dim3(256), sm, st, d, tb, t, k, K, Mc,
+                           small_stride, dyn);
+        return;
+    }
     // grid-stride loop: with a device-resident K the grid is the largest one
     int blocks = dyn ? 1024 : min(1024, (K * k + 255) / 256);
     hipLaunchKernelGGL(lr_gather_kernel, dim3(blocks, d.nf), dim3(256), 0, st, d, tb, t, k, K, Mc, Ar,
                        small_stride, dyn);
+}
+
+__global__ void lr_build_m_kernel(Dims d, DevTabs tb, int f, int kk0, zd* out, int ld) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (n < d.N) out[(size_t)r * ld + n] = m_entry(tb, d, f, kk0 + r, n);
+}
+
+void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, int kk0, int nrows, zd* out, int ld) {
+    hipLaunchKernelGGL(lr_build_m_kernel, dim3((d.N + 255) / 256, nrows), dim3(256), 0, st, d, tb, f, kk0, out, ld);
 }
 
 // Skinny products over the deferred factors (k <= 96 columns), on FP64 MFMA
@@ -1204,8 +1281,21 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
             nrow = min(G, d.N - r0);
         }
     }
-    if (nrow <= 0) return;
     const zd* stf = acc + (size_t)f * acc_stride;
+    if (d.lam && bx == 0) {
+        // the accept's M rows, kept as Lam[K + c] = s_c cap^-1[c][:] (see DevTabs)
+        zd* L = tb.Lam[f] + (size_t)K * d.gmax;
+        for (int e = tid; e < k * k; e += 256) {
+            const int c = e / k, l = e - c * k;
+            const zd x = stf[e];
+            L[(size_t)c * d.gmax + l] = s[c] < 0 ? zmk(-x.x, -x.y) : x;
+        }
+        if (f == 0 && tid < k) {
+            tb.fstart[K + tid] = K;
+            tb.flen[K + tid] = k;
+        }
+    }
+    if (nrow <= 0) return;
     for (int e = tid; e < k * k; e += 256) {
         const int r = e / k, c = e - r * k;
         Xs[r][c] = stf[e];
@@ -1244,10 +1334,12 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     }
     __syncthreads();
     if (do_rows) {
-        zd* M = tb.Mall[f];
-        for (int e = tid; e < k * nrow; e += 256) {
-            const int c = e / nrow, rr = e - c * nrow;
-            M[(size_t)(K + c) * d.ldN + rows[rr]] = Ss[c][rr];
+        if (!d.lam) {
+            zd* M = tb.Mall[f];
+            for (int e = tid; e < k * nrow; e += 256) {
+                const int c = e / nrow, rr = e - c * nrow;
+                M[(size_t)(K + c) * d.ldN + rows[rr]] = Ss[c][rr];
+            }
         }
         const zd* w = stf + 2 * k * k + k;
         const zd* Rt = stf + 2 * k * k + 2 * k + 4;
@@ -1786,12 +1878,11 @@ __global__ void lr_block_current_kernel(Dims d, DevTabs tb, GroupTab gt, int g, 
     const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
     const zd* W0 = tb.W0[f];
     const zd* A = tb.Aall[f];
-    const zd* M = tb.Mall[f];
     zd* D = Dblk + ((size_t)f * d.ngroups + g) * gt.gmax * gt.gmax;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nc * nc; e += gridDim.x * blockDim.x) {
         const int a = e / nc, b = e - a * nc;
         zd s = W0[(size_t)cl[a] * d.ldN + cl[b]];
-        for (int kk = 0; kk < K; kk++) s = zfms(A[(size_t)cl[a] * d.Kmax + kk], M[(size_t)kk * d.ldN + cl[b]], s);
+        for (int kk = 0; kk < K; kk++) s = zfms(A[(size_t)cl[a] * d.Kmax + kk], m_entry(tb, d, f, kk, cl[b]), s);
         D[a * gt.gmax + b] = s;
     }
 }

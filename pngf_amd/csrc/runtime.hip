@@ -29,6 +29,8 @@ typedef rocblas_double_complex rbz;
 
 // iterations of the device-driven DBS loop per host synchronisation (and per captured graph)
 constexpr int kLoopChunkMax = 16;
+// rows of M rebuilt per zgemm when the factors are kept as A and Lam and are folded
+constexpr int kMChunk = 768;
 
 struct Timed {
     hipEvent_t a, b;
@@ -102,7 +104,13 @@ struct pngf_handle {
     zd* d_pe = nullptr;
     // per frequency big buffers (host arrays of device pointers)
     std::vector<zd*> Gcm, Grow, Cinv, W0, Aall, Mall;
-    zd** d_tabs = nullptr;  // device copy: [5][nf]
+    zd** d_tabs = nullptr;  // device copy: [6][nf]
+    // factors kept as A and Lam (no M): symmetric W and groups of <= kLdsGmax components
+    bool lam = false;
+    std::vector<zd*> Lam;
+    int32_t* d_fstart = nullptr;
+    int32_t* d_flen = nullptr;
+    zd* d_mchunk = nullptr;  // rows of M rebuilt for a fold (kMChunk x ldN)
     DevTabs tabs{};
     DevVecs vecs{};
     // scratch
@@ -399,14 +407,18 @@ int check_ready(pngf_handle* h) {
 
 int upload_tabs(pngf_handle* h) {
     const int nf = h->d.nf;
-    std::vector<zd*> all(5 * nf, nullptr);
+    std::vector<zd*> all(6 * nf, nullptr);
     for (int f = 0; f < nf; f++) {
         all[0 * nf + f] = h->W0[f]; all[1 * nf + f] = h->Aall[f]; all[2 * nf + f] = h->Mall[f];
         all[3 * nf + f] = h->Grow[f]; all[4 * nf + f] = h->Cinv[f];
+        all[5 * nf + f] = h->Lam.empty() ? nullptr : h->Lam[f];
     }
     HC(h, hipMemcpy(h->d_tabs, all.data(), all.size() * sizeof(zd*), hipMemcpyHostToDevice));
     h->tabs.W0 = h->d_tabs; h->tabs.Aall = h->d_tabs + nf; h->tabs.Mall = h->d_tabs + 2 * nf;
     h->tabs.Grow = h->d_tabs + 3 * nf; h->tabs.Cinv = h->d_tabs + 4 * nf;
+    h->tabs.Lam = h->d_tabs + 5 * nf;
+    h->tabs.fstart = h->d_fstart;
+    h->tabs.flen = h->d_flen;
     return PNGF_OK;
 }
 
@@ -584,15 +596,32 @@ int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
 }
 
 // ---------------------------------------------------------------- accept ---
+// out -= A_all[:, :K] M[:K, :] for frequency f (out: N x ldN row-major, W0 or a copy of it);
+// with the factors kept as A and Lam, M is rebuilt kMChunk rows at a time.
+int fold_into(pngf_handle* h, int f, zd* out) {
+    const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
+    const Dims& d = h->d;
+    if (!h->lam) {
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
+                            (rbz*)h->Mall[f], d.ldN, (rbz*)h->Aall[f], d.Kmax, &one, (rbz*)out, d.ldN));
+        return PNGF_OK;
+    }
+    for (int kk0 = 0; kk0 < h->K; kk0 += kMChunk) {
+        const int nr = std::min(kMChunk, h->K - kk0);
+        launch_lr_build_m(h->st, d, h->tabs, f, kk0, nr, h->d_mchunk, d.ldN);
+        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, nr, &mone,
+                            (rbz*)h->d_mchunk, d.ldN, (rbz*)(h->Aall[f] + kk0), d.Kmax, &one, (rbz*)out, d.ldN));
+    }
+    return PNGF_OK;
+}
+
 int flush_factors(pngf_handle* h) {
     if (h->K == 0) return PNGF_OK;
-    const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
     const Dims& d = h->d;
     Timed tm{};
     prof_begin(h, tm);
     for (int f = 0; f < d.nf; f++) {
-        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
-                            (rbz*)h->Mall[f], d.ldN, (rbz*)h->Aall[f], d.Kmax, &one, (rbz*)h->W0[f], d.ldN));
+        CK(fold_into(h, f, h->W0[f]));
         if (h->sym) launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
     }
     prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (32.0 * d.N * (double)d.N + 32.0 * d.N * (double)h->K),
@@ -903,7 +932,7 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     CK(dalloc_t(h, &h->d_Fbest, 4));
     CK(dalloc_t(h, &h->d_Fsel, 4));
     CK(dalloc_t(h, &h->d_pe, ld));
-    CK(dalloc_t(h, &h->d_tabs, 5 * (size_t)nf));
+    CK(dalloc_t(h, &h->d_tabs, 6 * (size_t)nf));
     CK(dalloc_t(h, &h->vecs.q, nf * ld));
     CK(dalloc_t(h, &h->vecs.S, nf * ld));
     CK(dalloc_t(h, &h->vecs.R, nf * ne * ld));
@@ -1155,13 +1184,18 @@ int pngf_init_state(pngf_handle* h) {
         size_t freeb = 0, totb = 0;
         HC(h, hipMemGetInfo(&freeb, &totb));
         // keep room for the scratch strips and for one checkpoint copy of W0
-        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (size_t)nf * ld * N * sizeof(zd) + (64u << 20);
+        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (size_t)nf * ld * N * sizeof(zd) +
+                               (size_t)kMChunk * ld * sizeof(zd) + (64u << 20);
         size_t avail = freeb > scratch ? freeb - scratch : 0;
         if (h->cfg.mem_limit > 0) {
             size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
             avail = std::min(avail, lim);
         }
-        const size_t per_col = (size_t)nf * 2 * ld * sizeof(zd);  // A_all column + M_all row
+        // symmetric W with groups held in LDS: M rows are rebuilt from A and the k x k Lam of
+        // their accept, so a factor column costs one A column (twice the capacity)
+        h->lam = h->sym && gmax <= kLdsGmax;
+        h->d.lam = h->lam ? 1 : 0;
+        const size_t per_col = (size_t)nf * (h->lam ? ld + gmax : 2 * ld) * sizeof(zd);  // A column + M row or Lam row
         // Folding K deferred columns into W0 costs 8 N^2 K flop; not folding costs 8 N K k more per
         // later accept, so folding pays only when more than N / k accepts remain: by default the
         // factors hold N / gmax accepts (memory permitting) and a typical run never folds.
@@ -1171,9 +1205,18 @@ int pngf_init_state(pngf_handle* h) {
         if (cols < (size_t)gmax) return fail(h, PNGF_ERR_NOMEM, "no device memory for the deferred factors");
         h->Kmax = (int)cols;
         h->d.Kmax = h->Kmax;
+        if (h->lam) h->Lam.assign(nf, nullptr);
         for (int f = 0; f < nf; f++) {
             CK(dalloc_t(h, &h->Aall[f], (size_t)N * h->Kmax));
-            CK(dalloc_t(h, &h->Mall[f], (size_t)h->Kmax * ld));
+            if (h->lam) CK(dalloc_t(h, &h->Lam[f], (size_t)h->Kmax * gmax));
+            else CK(dalloc_t(h, &h->Mall[f], (size_t)h->Kmax * ld));
+        }
+        if (h->lam) {
+            CK(dalloc_t(h, &h->d_fstart, h->Kmax));
+            CK(dalloc_t(h, &h->d_flen, h->Kmax));
+            CK(dalloc_t(h, &h->d_mchunk, (size_t)kMChunk * ld));
+            HC(h, hipMemset(h->d_fstart, 0, sizeof(int32_t) * h->Kmax));
+            HC(h, hipMemset(h->d_flen, 0, sizeof(int32_t) * h->Kmax));
         }
         h->small_stride = (size_t)h->Kmax * gmax;
         CK(dalloc_t(h, &h->d_Mc, nf * h->small_stride));
@@ -1182,7 +1225,8 @@ int pngf_init_state(pngf_handle* h) {
         CK(dalloc_t(h, &h->d_Bt, nf * h->bt_stride));
         for (int f = 0; f < nf; f++) {
             HC(h, hipMemset(h->Aall[f], 0, sizeof(zd) * (size_t)N * h->Kmax));
-            HC(h, hipMemset(h->Mall[f], 0, sizeof(zd) * (size_t)h->Kmax * ld));
+            if (h->lam) HC(h, hipMemset(h->Lam[f], 0, sizeof(zd) * (size_t)h->Kmax * gmax));
+            else HC(h, hipMemset(h->Mall[f], 0, sizeof(zd) * (size_t)h->Kmax * ld));
         }
         HC(h, hipMemset(h->d_Mc, 0, sizeof(zd) * nf * h->small_stride));
         HC(h, hipMemset(h->d_Ar, 0, sizeof(zd) * nf * h->small_stride));
@@ -1325,11 +1369,7 @@ int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out) {
     zd* tmp = nullptr;
     CK(dalloc_t(h, &tmp, ld * N));
     HC(h, hipMemcpyAsync(tmp, h->W0[f], ld * N * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
-    if (h->K > 0) {
-        const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
-        RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, N, N, h->K, &mone,
-                            (rbz*)h->Mall[f], ld, (rbz*)h->Aall[f], h->Kmax, &one, (rbz*)tmp, ld));
-    }
+    if (h->K > 0) CK(fold_into(h, f, tmp));
     HC(h, hipStreamSynchronize(h->st));
     HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
     dfree(h, tmp);

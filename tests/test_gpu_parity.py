@@ -422,6 +422,27 @@ def test_device_loop_matches_host_loop(layout, max_evals):
     gpu.close()
 
 
+def test_device_loop_profiled_run():
+    """With profiling on, the device-driven loop times every column strip that ran (one per
+    accept) with HIP events and counts its algorithmic flops (8 N K k per frequency, K = 24 j for
+    the j-th accept of 24-component flips); the run itself is bitwise the one without events."""
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK)
+    gpu.checkpoint()
+    gpu.profile(True)
+    a = gpu.dbs_run()
+    st = gpu.kernel_stats(1)
+    gpu.profile(False)
+    gpu.restore()
+    b = gpu.dbs_run()
+    assert a["accepts"] > 0 and st["launches"] == a["accepts"] and st["total_ms"] > 0
+    N, nf, J = gpu.N, len(freqs), a["accepts"]
+    assert st["flops"] == sum(8.0 * N * (24 * j) * 24 * nf for j in range(J))
+    assert np.array_equal(a["acc_tiles"], b["acc_tiles"]) and a["acc_F"].tolist() == b["acc_F"].tolist()
+    gpu.close()
+
+
 def test_device_loop_folds_factors():
     """Factor capacity of 3 accepts: the device-driven loop folds between its chunks and still
     follows the oracle's run."""
