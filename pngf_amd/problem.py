@@ -75,6 +75,6 @@ WORKLOADS = {
 
 # 16 concurrent DBS instances: 4 seeds at each of 11, 15, 21 and 25 tiles, 5 frequencies.
 # Factor capacity per size, in accepts (runs from these starts accept up to ~50, ~100, ~210 and
-# ~230 flips): no instance folds.  W0 of all 16 instances takes 120 GB, the factors ~115 GB.
+# ~230 flips): no instance folds.  W0 of all 16 instances takes 120 GB, the factors ~57 GB.
 MULTI16 = dict(sizes=[11, 15, 21, 25], seeds=[0, 1, 2, 3], freqs=freq_list(5, 28e9, 32e9), layout=1,
                max_deferred={11: 64, 15: 128, 21: 256, 25: 256})
