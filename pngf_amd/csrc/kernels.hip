@@ -236,7 +236,9 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     const int gmax = d.gmax;
     zd* dpe = smz;                                // [gmax]
     zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
-    int* t = (int*)(wdg + gmax);
+    zd* qt = wdg + gmax;                          // [gmax] q[t]
+    zd* rt = qt + gmax;                           // [ne][gmax] R[:, t]
+    int* t = (int*)(rt + 3 * gmax);
     int* sg = t + gmax;
     int* lp = sg + gmax;
     int* piv = lp + gmax;
@@ -258,13 +260,18 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
     const zd* q = v.q + (size_t)f * d.ldN;
     const zd* einc = v.Einc + (size_t)f * d.ldN;
+    // every global read of the candidate in one phase: dPE, q_t, R[:, t] and the block (the
+    // block gather: independent loads, several in flight per thread; the diagonal of W is kept
+    // apart (wdg) so that the rhs below reads W exactly from LDS)
     for (int c = tid; c < k; c += nth) {
         zd e = einc[t[c]];
         dpe[c] = sg[c] > 0 ? zmk(-e.x, -e.y) : e;
+        qt[c] = q[t[c]];
     }
-    __syncthreads();
-    // block gather (independent loads, several in flight per thread); the diagonal of W
-    // is kept apart (wdg) so the rhs below reads W exactly from LDS
+    for (int e = tid; e < d.ne * k; e += nth) {
+        const int r = e / k, c = e - r * k;
+        rt[r * gmax + c] = v.R[((size_t)f * d.ne + r) * d.ldN + t[c]];
+    }
 #pragma unroll 4
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
@@ -279,7 +286,7 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     __syncthreads();
     // rhs_r = q_t_r + sum_c W[t_r][t_c] dPE_c, with W[t_r][t_c] = s_c a[r][c] off the diagonal
     for (int r = tid; r < k; r += nth) {
-        zd acc = q[t[r]];
+        zd acc = qt[r];
         for (int c = 0; c < k; c++) {
             zd w = wdg[r];
             if (c != r) {
@@ -307,12 +314,11 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     }
     if (tid < d.ne) {
         const int r = tid;
-        const zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
         zd acc = v.Eobj[f * d.ne + r];
         for (int c = 0; c < k; c++) {
             zd y = a[c * lda + k];
             zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
-            acc = zfma(R[t[c]], w, acc);
+            acc = zfma(rt[r * gmax + c], w, acc);
         }
         out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
     }
@@ -397,7 +403,7 @@ void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt
                     int32_t* out_info, zd* gscratch, const int32_t* nb_dev) {
     dim3 grid(nb, d.nf);
     int nth = d.gmax <= 32 ? 128 : 256;
-    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)2 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
+    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
                        out_info, gscratch, nb_dev);
 }
