@@ -256,6 +256,22 @@ def main():
                 accepts=rb["accepts"], F_final=rb["F_final"])
     log("[bench] best-improvement: %d evals %d accepts %.3f s F %.6g" % (rb["flip_count"], rb["accepts"],
                                                                           rb["seconds"], rb["F_final"]))
+    # sweep time: one scoring launch over every candidate at every frequency, HIP-event timed
+    # (each step of the best-improvement run is one such launch)
+    sweep = None
+    if profile:
+        opt.restore()
+        opt.profile(True)
+        opt.dbs_run(max_evals=wl["max_evals"], order=P.ORDER_BEST)
+        ev = opt.kernel_stats(0)
+        opt.profile(False)
+        if ev["launches"]:
+            ncand = rb["flip_count"] // max(1, rb["accepts"] + 1)
+            sweep_ms = ev["total_ms"] / ev["launches"]
+            sweep = dict(candidates=ncand, freqs=len(wl["freqs"]), ms=sweep_ms, launches=ev["launches"],
+                         flips_per_s=ncand / (sweep_ms * 1e-3))
+            log("[bench] sweep of %d candidates x %d frequencies: %.1f us (HIP events)" % (
+                ncand, len(wl["freqs"]), sweep_ms * 1e3))
     opt.close()
     del opt
 
@@ -314,6 +330,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "F_final_vs_plain": (last["F_final"] / plain["F_final"]) if plain else None,
         "best_improvement_variant": best,
+        "sweep": sweep,
         "time_to_solution_s": {"first_improvement": last["seconds"], "best_improvement": rb["seconds"],
                                "plain_fp64_gpu": plain["seconds"] if plain else None},
         "kernels": allk,
