@@ -899,7 +899,7 @@ void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, 
 // B is staged in LDS already in that operand form: Bp[kk][j] holds (B_real[2kk][j],
 // B_real[2kk+1][j]) for real column j, so one 16-byte LDS read feeds the two MFMAs of a
 // complex inner index with no VALU work.
-// One wave owns 16 output rows and all CT 16-column tiles.  Lane l feeds inner index
+// One wave owns RT x 16 output rows and all CT 16-column tiles.  Lane l feeds inner index
 // 2*(l/16)+j of each 8-entry sub-chunk (two consecutive complex A entries per lane).
 // Fragment layouts (16x16x4 f64, measured on gfx950 with a lane probe): A[i][k] in lane
 // i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4.
@@ -921,7 +921,7 @@ __device__ inline zd ldg(const zd* p, size_t i) {
     return r;
 }
 
-template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB>
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
 __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
                                                                     int k, int K, const zd* __restrict__ small,
                                                                     int small_stride, zd* Bt, size_t bt_stride,
@@ -941,33 +941,46 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     const int f = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    const int n0 = (blockIdx.x * WAVES + wave) * 16;
-    const int nc = min(n0 + lr, d.N - 1);  // clamped row for loads
+    // RT row tiles of 16 per wave: each staged B operand feeds RT times as many MFMAs
+    const int n0 = (blockIdx.x * WAVES + wave) * 16 * RT;
+    int nc[RT];  // clamped rows for loads
+#pragma unroll
+    for (int r = 0; r < RT; r++) nc[r] = min(n0 + 16 * r + lr, d.N - 1);
     const zd* A = tb.Aall[f];
     const zd* M = tb.Mall[f];
     const zd* W0 = tb.W0[f];
     const zd* sm = small + (size_t)f * small_stride;
     const int Kc = K - 1;
-    d4v acc[CT];
+    d4v acc[RT][CT];
 #pragma unroll
-    for (int c = 0; c < CT; c++) acc[c] = d4v{0.0, 0.0, 0.0, 0.0};
-    auto ld = [&](int kb, zd (&a)[2]) {
+    for (int r = 0; r < RT; r++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int Kp = min(kb + 2 * lq + j, Kc);
-            a[j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc) : ldg(A, (size_t)nc * d.Kmax + Kp);
-        }
+        for (int c = 0; c < CT; c++) acc[r][c] = d4v{0.0, 0.0, 0.0, 0.0};
+    auto ld = [&](int kb, zd (&a)[RT][2]) {
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int Kp = min(kb + 2 * lq + j, Kc);
+                a[r][j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc[r]) : ldg(A, (size_t)nc[r] * d.Kmax + Kp);
+            }
     };
-    auto body = [&](const zd (&a)[2], int kloc, int bsel) {
+    auto body = [&](const zd (&a)[RT][2], int kloc, int bsel) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             zd m[CT];
 #pragma unroll
             for (int c = 0; c < CT; c++) m[c] = Bp[bsel][kloc + 2 * lq + j][c * 16 + lr];
 #pragma unroll
-            for (int c = 0; c < CT; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].x, m[c].x, acc[c], 0, 0, 0);
+            for (int r = 0; r < RT; r++)
 #pragma unroll
-            for (int c = 0; c < CT; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j].y, m[c].y, acc[c], 0, 0, 0);
+                for (int c = 0; c < CT; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][j].x, m[c].x, acc[r][c], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < CT; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][j].y, m[c].y, acc[r][c], 0, 0, 0);
         }
     };
     // B chunk k0..k0+KC-1 (rows past K and columns past k are zero) into registers, then
@@ -992,7 +1005,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             Bp[bsel][kk][2 * c + 1] = zmk(r[u].y, r[u].x);
         }
     };
-    zd buf[DEPTH][2];
+    zd buf[DEPTH][RT][2];
     if (K > 0) {
 #pragma unroll
         for (int q = 0; q < DEPTH; q++) ld(q * KS, buf[q]);
@@ -1039,14 +1052,16 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
         }
     }
 #pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
     for (int c = 0; c < CT; c++) {
         const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
         if (cc >= k) continue;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int row = n0 + lq + 4 * v;
+            const int row = n0 + 16 * r + lq + 4 * v;
             if (row >= d.N) continue;
-            const double a = acc[c][v];
+            const double a = acc[r][c][v];
             if (!ROW) {
                 const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
                 ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
@@ -1059,40 +1074,41 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 }
 
 // Tile-shape variants offered to the autotuner, per column-tile count CT: (KC inner entries
-// per staged chunk, DEPTH sub-chunks in flight, waves per workgroup, double-buffered stage).
+// per staged chunk, DEPTH sub-chunks in flight, waves per workgroup, double-buffered stage,
+// RT 16-row tiles per wave).
 // All of them issue the MFMAs of an output tile in the same inner-index order into one
 // accumulator, so every variant gives bitwise the same result and the autotuner's choice
 // cannot change a DBS run.
-template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB>
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
 static void launch_variant(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                            const zd* small, int small_stride, zd* Bt, size_t bt_stride, const int32_t* dyn) {
-    const int rows = 16 * WAVES;
+    const int rows = 16 * WAVES * RT;
     dim3 grid((d.N + rows - 1) / rows, d.nf);
-    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, DB>), grid, dim3(WAVES * 64), 0, st, d, tb,
-                       t, k, K, small, small_stride, Bt, bt_stride, dyn);
+    hipLaunchKernelGGL((lr_skinny_mfma_kernel<CT, ROW, KC, DEPTH, WAVES, DB, RT>), grid, dim3(WAVES * 64), 0, st, d,
+                       tb, t, k, K, small, small_stride, Bt, bt_stride, dyn);
 }
 
 template <bool ROW>
 static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                                const zd* small, int small_stride, zd* Bt, size_t bt_stride, int variant,
                                const int32_t* dyn) {
-#define PNGF_SKV(CT, KC, DEPTH, W, DB) \
-    launch_variant<CT, ROW, KC, DEPTH, W, DB>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
+#define PNGF_SKV(CT, KC, DEPTH, W, DB, RT) \
+    launch_variant<CT, ROW, KC, DEPTH, W, DB, RT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
     if (k <= 24) {
         switch (variant) {
-            case 1: PNGF_SKV(3, 32, 4, 8, true); break;
-            case 2: PNGF_SKV(3, 64, 2, 8, false); break;
-            case 3: PNGF_SKV(3, 32, 2, 8, false); break;
-            case 4: PNGF_SKV(3, 16, 2, 8, true); break;
-            case 5: PNGF_SKV(3, 32, 2, 4, true); break;
-            default: PNGF_SKV(3, 32, 2, 8, true); break;
+            case 1: PNGF_SKV(3, 32, 2, 4, true, 2); break;
+            case 2: PNGF_SKV(3, 64, 2, 8, false, 1); break;
+            case 3: PNGF_SKV(3, 16, 2, 4, true, 2); break;
+            case 4: PNGF_SKV(3, 16, 2, 8, true, 1); break;
+            case 5: PNGF_SKV(3, 32, 2, 4, true, 1); break;
+            default: PNGF_SKV(3, 32, 2, 8, true, 1); break;
         }
     } else if (k <= 48) {
-        if (variant % 2) PNGF_SKV(6, 16, 2, 8, false);
-        else PNGF_SKV(6, 16, 2, 8, true);
+        if (variant % 2) PNGF_SKV(6, 16, 2, 8, false, 1);
+        else PNGF_SKV(6, 16, 2, 8, true, 1);
     } else {
-        if (variant % 2) PNGF_SKV(12, 16, 2, 4, false);
-        else PNGF_SKV(12, 16, 2, 8, false);
+        if (variant % 2) PNGF_SKV(12, 16, 2, 4, false, 1);
+        else PNGF_SKV(12, 16, 2, 8, false, 1);
     }
 #undef PNGF_SKV
 }
