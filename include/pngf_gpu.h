@@ -62,10 +62,18 @@ typedef struct { double re, im; } pngf_c128;
 #define PNGF_SYM_AUTO 0 /* use it when every G is symmetric to 1e-14 of its largest entry */
 #define PNGF_SYM_OFF  1 /* always form both strips */
 
+/* LOWRANK deferred factors W = W0 - A M: how the M side is held */
+#define PNGF_FACTORS_AUTO    0 /* stored M, unless device memory caps its capacity below the wanted
+                                  number of accepts and PNGF_FACTORS_REBUILT applies */
+#define PNGF_FACTORS_STORED  1 /* M stored: one A column and one M row per factor column */
+#define PNGF_FACTORS_REBUILT 2 /* symmetric W, flips of at most 32 components: M rows rebuilt from A
+                                  and the k x k Lam' of their accept (M = Lam' A^T), so a factor
+                                  column costs one A column: twice the capacity, a slower gather */
+
 /* who drives the raster first-improvement loop (same accepted sequence either way) */
 #define PNGF_LOOP_AUTO 0 /* the device: batch, selection and accept bookkeeping run in kernels and the
                             host synchronises once per 16 iterations (LOWRANK, no training log, no
-                            profiling, flips of at most 32 components); otherwise the host */
+                            history, flips of at most 32 components); otherwise the host */
 #define PNGF_LOOP_HOST 1 /* the host: one synchronisation per scored batch */
 
 /* DBS visit order */
@@ -96,7 +104,7 @@ typedef struct {
     double synth_soft;    /* synthetic G only: softening radius in cells (0 = 1.0); smaller values
                              give better-conditioned C */
     int32_t symmetry;     /* LOWRANK: PNGF_SYM_AUTO or PNGF_SYM_OFF */
-    int32_t reserved;
+    int32_t factors;      /* LOWRANK: PNGF_FACTORS_* */
 } pngf_config;
 
 /* Creates a handle; selects the device and allocates nothing large yet.
@@ -303,6 +311,10 @@ int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out);
    symmetric), *rel_asym = max over frequencies of max|G_ij - G_ji| / max|G_ij| (|z| =
    |Re z| + |Im z|).  Either pointer may be null. */
 int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym);
+/* After pngf_init_state (LOWRANK): *columns = factor columns the handle holds before it folds them
+   into W0, *rebuilt = 1 if M rows are rebuilt from A and Lam' (PNGF_FACTORS_REBUILT in effect).
+   Either pointer may be null.  No counterpart in the reference, which folds every accept. */
+int pngf_factor_capacity(pngf_handle* h, int32_t* columns, int32_t* rebuilt);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,7 @@ METHOD_PLAIN, METHOD_LOWRANK = 0, 1
 DRIVER_REF, DRIVER_RASTER = 0, 1
 ORDER_FIRST, ORDER_BEST = 0, 1
 SYM_AUTO, SYM_OFF = 0, 1
+FACTORS_AUTO, FACTORS_STORED, FACTORS_REBUILT = 0, 1, 2
 LOOP_AUTO, LOOP_HOST = 0, 1
 K_EVAL, K_COLBLK, K_ROWBLK, K_ACC_SMALL, K_VCINV, K_RANKK = range(6)
 K_NAMES = ["eval", "colblk", "rowblk", "acc_small", "vcinv", "rankk"]
@@ -37,7 +38,7 @@ EXPORTS = [
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many", "pngf_dbs_run_pool",
-    "pngf_autotune_result", "pngf_symmetry",
+    "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity",
 ]
 
 
@@ -55,7 +56,7 @@ class Config(ct.Structure):
                 ("device", ct.c_int32), ("tile_variant", ct.c_int32), ("dx", ct.c_double),
                 ("dy", ct.c_double), ("dz", ct.c_double), ("z0", ct.c_double),
                 ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double), ("symmetry", ct.c_int32),
-                ("reserved", ct.c_int32)]
+                ("factors", ct.c_int32)]
 
 
 class DbsParams(ct.Structure):
@@ -138,6 +139,7 @@ def lib():
         "pngf_dbs_run_pool": (I, [P, I, P, P, I]),
         "pngf_autotune_result": (I, [P, ct.POINTER(I), ct.POINTER(I), P]),
         "pngf_symmetry": (I, [P, ct.POINTER(I), ct.POINTER(ct.c_double)]),
+        "pngf_factor_capacity": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
         "pngf_features": (I, [P, P, I, P]),
         "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
                                     ct.POINTER(DbsResult)]),
@@ -189,10 +191,11 @@ class Optimizer:
 
     def __init__(self, layout, tiles, num_freq=1, num_eobj=1, max_flips=1, method=METHOD_LOWRANK,
                  objective=OBJ_S11, driver=DRIVER_RASTER, max_deferred=0, device=0, mem_limit=0,
-                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0, symmetry=0):
+                 dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0, symmetry=0,
+                 factors=0):
         self.cfg = Config(layout, tiles, tiles, cells, cells, num_eobj, max_flips, num_freq, method,
                           objective, driver, max_deferred, device, tile_variant, dx, dy, dz, z0, mem_limit,
-                          synth_soft, symmetry, 0)
+                          synth_soft, symmetry, factors)
         self.h = ct.c_void_p()
         rc = lib().pngf_create(ct.byref(self.cfg), ct.byref(self.h))
         if rc != 0:
@@ -393,6 +396,13 @@ class Optimizer:
         u, a = ct.c_int32(), ct.c_double()
         self._ck(lib().pngf_symmetry(self.h, ct.byref(u), ct.byref(a)), "symmetry")
         return bool(u.value), a.value
+
+    def factor_capacity(self):
+        """(columns, rebuilt): deferred factor columns held before a fold, and whether M rows are
+        rebuilt from A and Lam' instead of stored."""
+        c, r = ct.c_int32(), ct.c_int32()
+        self._ck(lib().pngf_factor_capacity(self.h, ct.byref(c), ct.byref(r)), "factor_capacity")
+        return c.value, bool(r.value)
 
     def profile(self, enable=True):
         self._ck(lib().pngf_profile(self.h, int(enable)), "profile")

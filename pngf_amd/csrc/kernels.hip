@@ -815,8 +815,9 @@ __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__
 }
 
 // Mc[K'][c] = M[K'][t_c] with M rebuilt from Lam and A (factors kept without M): a workgroup
-// takes 32 factor rows of one frequency and, one accept (common fstart) at a time, stages that
-// accept's A columns of the rows t in LDS; the same sums as m_entry.
+// takes 32 factor rows of one frequency; the A columns those rows use lie within gmax - 1 of
+// the rows' range, so they are staged in LDS for every t_c in the same phase as the Lam rows.
+// Same sums as m_entry.
 constexpr int kGatherRows = 32;
 __global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
                                                            int K, zd* Mc, int small_stride, const int32_t* dyn) {
@@ -827,38 +828,34 @@ __global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabs tb, 
     }
     extern __shared__ zd sml[];
     const int f = blockIdx.y, tid = threadIdx.x, gmax = d.gmax;
-    zd* Ls = sml;                        // [kGatherRows][gmax] Lam rows
-    zd* As = sml + kGatherRows * gmax;   // [k][gmax] A[t_c][fs .. fs + len)
+    const int span = kGatherRows + 2 * (gmax - 1);  // A columns a workgroup can need
+    zd* Ls = sml;                                    // [kGatherRows][gmax] Lam rows
+    zd* As = sml + kGatherRows * gmax;               // [k][span] A[t_c][lo ..]
     __shared__ int s_fs[kGatherRows], s_len[kGatherRows];
     const zd* A = tb.Aall[f];
     const zd* Lam = tb.Lam[f];
     zd* mc = Mc + (size_t)f * small_stride;
     for (int r0 = blockIdx.x * kGatherRows; r0 < K; r0 += gridDim.x * kGatherRows) {
         const int nr = min(kGatherRows, K - r0);
+        const int lo = max(0, r0 - (gmax - 1)), hi = min(K, r0 + nr + gmax - 1);
         __syncthreads();
         for (int e = tid; e < nr * gmax; e += blockDim.x) Ls[e] = Lam[(size_t)r0 * gmax + e];
         if (tid < nr) {
             s_fs[tid] = tb.fstart[r0 + tid];
             s_len[tid] = tb.flen[r0 + tid];
         }
+        const int w = hi - lo;
+        for (int e = tid; e < k * w; e += blockDim.x) {
+            const int c = e / w, l = e - c * w;
+            As[c * span + l] = A[(size_t)t[c] * d.Kmax + lo + l];
+        }
         __syncthreads();
-        for (int rr = 0; rr < nr;) {
-            const int fs = s_fs[rr], len = s_len[rr];
-            int rr2 = rr + 1;
-            while (rr2 < nr && s_fs[rr2] == fs) rr2++;
-            for (int e = tid; e < k * len; e += blockDim.x) {
-                const int c = e / len, l = e - c * len;
-                As[c * gmax + l] = A[(size_t)t[c] * d.Kmax + fs + l];
-            }
-            __syncthreads();
-            for (int e = tid; e < (rr2 - rr) * k; e += blockDim.x) {
-                const int r = rr + e / k, c = e - (r - rr) * k;
-                zd acc = zmk(0.0, 0.0);
-                for (int l = 0; l < len; l++) acc = zfma(Ls[r * gmax + l], As[c * gmax + l], acc);
-                mc[(size_t)(r0 + r) * k + c] = acc;
-            }
-            __syncthreads();
-            rr = rr2;
+        for (int e = tid; e < nr * k; e += blockDim.x) {
+            const int r = e / k, c = e - r * k;
+            const int off = s_fs[r] - lo, len = s_len[r];
+            zd acc = zmk(0.0, 0.0);
+            for (int l = 0; l < len; l++) acc = zfma(Ls[r * gmax + l], As[c * span + off + l], acc);
+            mc[(size_t)(r0 + r) * k + c] = acc;
         }
     }
 }
@@ -869,7 +866,7 @@ void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const in
     if (d.lam) {
         const int rows = dyn ? d.Kmax : K;
         const int blocks = std::min(1024, (rows + kGatherRows - 1) / kGatherRows);
-        const size_t sm = (size_t)(kGatherRows + d.gmax) * d.gmax * sizeof(zd);
+        const size_t sm = ((size_t)kGatherRows * d.gmax + (size_t)d.gmax * (kGatherRows + 2 * (d.gmax - 1))) * sizeof(zd);
         hipLaunchKernelGGL(lr_gather_lam_kernel, dim3(blocks, d.nf), dim3(256), sm, st, d, tb, t, k, K, Mc,
                            small_stride, dyn);
         return;

@@ -878,7 +878,8 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     if (!(h->cfg.synth_soft > 0.0)) h->cfg.synth_soft = kSoftDefault;
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
     if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6 ||
-        (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF)) {
+        (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF) ||
+        c.factors < PNGF_FACTORS_AUTO || c.factors > PNGF_FACTORS_REBUILT) {
         delete h;
         return PNGF_ERR_ARG;
     }
@@ -1191,16 +1192,22 @@ int pngf_init_state(pngf_handle* h) {
             size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
             avail = std::min(avail, lim);
         }
-        // symmetric W with groups held in LDS: M rows are rebuilt from A and the k x k Lam of
-        // their accept, so a factor column costs one A column (twice the capacity)
-        h->lam = h->sym && gmax <= kLdsGmax;
-        h->d.lam = h->lam ? 1 : 0;
-        const size_t per_col = (size_t)nf * (h->lam ? ld + gmax : 2 * ld) * sizeof(zd);  // A column + M row or Lam row
         // Folding K deferred columns into W0 costs 8 N^2 K flop; not folding costs 8 N K k more per
         // later accept, so folding pays only when more than N / k accepts remain: by default the
         // factors hold N / gmax accepts (memory permitting) and a typical run never folds.
         int maxdef = h->cfg.max_deferred > 0 ? h->cfg.max_deferred : (N + gmax - 1) / gmax;
-        size_t cols = std::min<size_t>((size_t)maxdef * gmax, (size_t)(0.85 * avail) / per_col);
+        // A column + M row.  When memory caps that below the wanted capacity (or the caller asks)
+        // and W is symmetric with groups held in LDS, M rows are instead rebuilt from A and the
+        // k x k Lam of their accept: a factor column then costs one A column (twice the capacity;
+        // the gather that rebuilds M[:, t] costs ~15 us more per accept than reading stored M).
+        const size_t want = (size_t)maxdef * gmax;
+        const size_t per_col_m = (size_t)nf * 2 * ld * sizeof(zd);
+        const bool capped = (size_t)(0.85 * avail) / per_col_m < want;
+        h->lam = h->sym && gmax <= kLdsGmax && h->cfg.factors != PNGF_FACTORS_STORED &&
+                 (capped || h->cfg.factors == PNGF_FACTORS_REBUILT);
+        h->d.lam = h->lam ? 1 : 0;
+        const size_t per_col = h->lam ? (size_t)nf * (ld + gmax) * sizeof(zd) : per_col_m;
+        size_t cols = std::min<size_t>(want, (size_t)(0.85 * avail) / per_col);
         cols = cols / gmax * gmax;
         if (cols < (size_t)gmax) return fail(h, PNGF_ERR_NOMEM, "no device memory for the deferred factors");
         h->Kmax = (int)cols;
@@ -2046,6 +2053,13 @@ int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym) {
     if (!h) return PNGF_ERR_ARG;
     if (used) *used = h->sym ? 1 : 0;
     if (rel_asym) *rel_asym = h->asym;
+    return PNGF_OK;
+}
+
+int pngf_factor_capacity(pngf_handle* h, int32_t* columns, int32_t* rebuilt) {
+    if (!h) return PNGF_ERR_ARG;
+    if (columns) *columns = h->Kmax;
+    if (rebuilt) *rebuilt = h->lam ? 1 : 0;
     return PNGF_OK;
 }
 

@@ -33,7 +33,8 @@ def rel(a, b):
 
 
 def setup_pair(layout, T, freqs, method, ne=1, driver=P.DRIVER_RASTER, objective=P.OBJ_S11, max_flips=1,
-               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL, tile_variant=0, symmetry=P.SYM_AUTO, Gs=None):
+               max_deferred=0, seed=0, tm=None, soft=SOFT_WELL, tile_variant=0, symmetry=P.SYM_AUTO, Gs=None,
+               factors=P.FACTORS_AUTO):
     if Gs is None:
         Gs = [P.synth_g_host(layout, T, ne, hz, soft=soft) for hz in freqs]
     if tm is None:
@@ -41,7 +42,7 @@ def setup_pair(layout, T, freqs, method, ne=1, driver=P.DRIVER_RASTER, objective
     mid = (T - 1) // 2
     gpu = P.Optimizer(layout, T, num_freq=len(freqs), num_eobj=ne, max_flips=max_flips, method=method,
                       objective=objective, driver=driver, max_deferred=max_deferred, tile_variant=tile_variant,
-                      symmetry=symmetry)
+                      symmetry=symmetry, factors=factors)
     ors = []
     for f, (hz, G) in enumerate(zip(freqs, Gs)):
         gpu.set_frequency(f, hz)
@@ -135,14 +136,18 @@ def test_dbs_raster_identical_sequence(layout, method):
     assert g["converged"]
 
 
-@pytest.mark.parametrize("symmetry", [P.SYM_AUTO, P.SYM_OFF])
-def test_dbs_multifreq_and_flush(symmetry):
+@pytest.mark.parametrize("symmetry,factors", [(P.SYM_AUTO, P.FACTORS_AUTO), (P.SYM_AUTO, P.FACTORS_REBUILT),
+                                              (P.SYM_OFF, P.FACTORS_AUTO)])
+def test_dbs_multifreq_and_flush(symmetry, factors):
     """3 frequencies, deferred factors folded into W every 3 accepts; the accept with the
-    row strip taken from the column strip (symmetric G) and with both strips formed."""
+    row strip taken from the column strip (symmetric G, M stored or rebuilt from A and Lam')
+    and with both strips formed."""
     T = 7
     freqs = freq_list(3, 28e9, 32e9)
-    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3, symmetry=symmetry)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3, symmetry=symmetry,
+                              factors=factors)
     assert gpu.symmetry()[0] == (symmetry == P.SYM_AUTO)
+    assert gpu.factor_capacity() == (3 * 24, factors == P.FACTORS_REBUILT)
     g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
     assert g["flushes"] > 0
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
@@ -328,12 +333,14 @@ def test_asymmetric_g_forms_both_strips():
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
-@pytest.mark.parametrize("asym", [False, True])
-def test_get_w_after_folds_and_pending_factors(asym):
+@pytest.mark.parametrize("asym,factors", [(False, P.FACTORS_AUTO), (False, P.FACTORS_REBUILT),
+                                          (True, P.FACTORS_REBUILT)])
+def test_get_w_after_folds_and_pending_factors(asym, factors):
     """W = (G - I) C^-1 read back after accepts that folded the deferred factors into W0 (the
     in-place fold) and left some pending (the fold into a copy): equal to the oracle's
-    (G - I) C^-1, row-major, for a symmetric and a non-symmetric G.  (Capacity 2 accepts of 24
-    components: the 5 accepts fold twice and leave one pending.)"""
+    (G - I) C^-1, row-major, for a symmetric G (M stored or rebuilt) and a non-symmetric one (M
+    stored, whatever the request).  (Capacity 2 accepts of 24 components: the 5 accepts fold
+    twice and leave one pending.)"""
     T = 7
     freqs = freq_list(2, 29e9, 31e9)
     rng = np.random.default_rng(11)
@@ -344,8 +351,10 @@ def test_get_w_after_folds_and_pending_factors(asym):
             N = G.shape[0]
             G = G + 1e-3 * np.abs(G).max() * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N
         Gs.append(np.asfortranarray(G))
-    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=2, Gs=Gs)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=2, Gs=Gs,
+                              factors=factors)
     assert gpu.symmetry()[0] == (not asym)
+    assert gpu.factor_capacity()[1] == (not asym and factors == P.FACTORS_REBUILT)
     for t in [3, 17, 30, 44, 8]:
         for o in ors:
             o.compute_updated_Eobj([t])
@@ -398,14 +407,18 @@ def test_full_size_state_matches_fresh_init():
     a.close()
 
 
-@pytest.mark.parametrize("layout,max_evals", [(P.LAYOUT_DISJOINT, 0), (P.LAYOUT_SHARED, 0), (P.LAYOUT_DISJOINT, 60)])
-def test_device_loop_matches_host_loop(layout, max_evals):
+@pytest.mark.parametrize("layout,max_evals,factors", [(P.LAYOUT_DISJOINT, 0, P.FACTORS_AUTO),
+                                                      (P.LAYOUT_SHARED, 0, P.FACTORS_AUTO),
+                                                      (P.LAYOUT_DISJOINT, 60, P.FACTORS_AUTO),
+                                                      (P.LAYOUT_DISJOINT, 0, P.FACTORS_REBUILT)])
+def test_device_loop_matches_host_loop(layout, max_evals, factors):
     """The device-driven raster loop (batches, selection and accept bookkeeping in kernels) and
     the host-driven one give bitwise the same run, on the ill-conditioned G where any
     difference in rounding would show up as a different accepted sequence."""
     T = 7
     freqs = freq_list(3, 28e9, 32e9)
-    gpu, ors, tm = setup_pair(layout, T, freqs, P.METHOD_LOWRANK, soft=0.0)
+    gpu, ors, tm = setup_pair(layout, T, freqs, P.METHOD_LOWRANK, soft=0.0, factors=factors)
+    assert gpu.factor_capacity()[1] == (factors == P.FACTORS_REBUILT)
     gpu.checkpoint()
     a = gpu.dbs_run(max_evals=max_evals, loop=P.LOOP_HOST)
     map_a, st_a = gpu.tile_map(), gpu.state()
@@ -443,12 +456,13 @@ def test_device_loop_profiled_run():
     gpu.close()
 
 
-def test_device_loop_folds_factors():
+@pytest.mark.parametrize("factors", [P.FACTORS_AUTO, P.FACTORS_REBUILT])
+def test_device_loop_folds_factors(factors):
     """Factor capacity of 3 accepts: the device-driven loop folds between its chunks and still
-    follows the oracle's run."""
+    follows the oracle's run (M stored, or rebuilt from A and Lam')."""
     T = 7
     freqs = freq_list(3, 28e9, 32e9)
-    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, max_deferred=3, factors=factors)
     g = gpu.dbs_run(loop=P.LOOP_AUTO)
     o = O.dbs_run(ors, O.OBJ_S11, freqs, DX, DY, DZ)
     assert g["flushes"] > 0
