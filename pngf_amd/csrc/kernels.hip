@@ -1412,10 +1412,12 @@ __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, c
         k = dyn[kDynK];
         ntiles = dyn[kDynNt];
     }
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        for (int c = 0; c < k; c++) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
-        for (int i = 0; i < ntiles; i++) tile[gtiles[i]] ^= 1;
-        if (dyn) {
+    // the components of a flip are distinct, and so are its tiles: one thread each
+    for (int c = threadIdx.x; c < k; c += blockDim.x) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) tile[gtiles[i]] ^= 1;
+    if (dyn) {
+        __syncthreads();  // every thread has read dyn before it changes
+        if (threadIdx.x == 0) {
             dyn[kDynKK] += k;
             dyn[kDynActive] = 0;
         }
@@ -1424,7 +1426,7 @@ __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, c
 
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
                         const int32_t* gtiles, int ntiles, int32_t* dyn) {
-    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(64), 0, st, P, tile, t, s, k, gtiles, ntiles, dyn);
+    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(128), 0, st, P, tile, t, s, k, gtiles, ntiles, dyn);
 }
 
 // ---------------------------------------------------- device-driven loop -----
