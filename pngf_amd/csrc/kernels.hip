@@ -183,6 +183,8 @@ __device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, c
 // the solution in columns k..ncol-1.  Returns LAPACK-style info (0, or j+1 for the first zero
 // pivot, whose elimination step is skipped).  All threads of the block call it, after a barrier
 // that follows the writes of a; blockDim.x is a multiple of 64.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+
 __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int ri = tid & 31, cg = tid >> 5, ncg = blockDim.x >> 5;
@@ -192,22 +194,39 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
     int pm = lane;
     int info = 0;
     for (int j = 0; j < k; j++) {
-        double best = lane >= j && lane < k ? zabs1(a[pm * lda + j]) : -1.0;
-        int bi = lane;
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        const int pj = __shfl(pm, j, 64), pb = __shfl(pm, bi, 64);
+        // pivot search as one wave reduction: |Re|+|Im| >= 0, so its bit pattern orders like
+        // the value; + 1 puts every candidate above the lanes outside [j, k) (key 0), and the
+        // first lane holding the maximum key is the first logical row of the maximum
+        const unsigned long long key =
+            lane >= j && lane < k ? (unsigned long long)__double_as_longlong(zabs1(a[pm * lda + j])) + 1ull : 0ull;
+        const unsigned long long mk = __ockl_wfred_max_u64(key);
+        const int bi = __ffsll((unsigned long long)__ballot(key == mk)) - 1;
+        const int pj = __builtin_amdgcn_readlane(pm, j), pb = __builtin_amdgcn_readlane(pm, bi);
         if (lane == j) pm = pb;
         else if (lane == bi) pm = pj;
-        if (best == 0.0 && info == 0) info = j + 1;
+        if (mk == 1ull && info == 0) info = j + 1;  // the maximum is 0
         const int pr = pb;
         const zd pv = a[pr * lda + j];
         if ((pv.x != 0.0 || pv.y != 0.0) && ri < k && ri != pr) {
+            // all reads of this thread's columns first, then the updates (rows pr and ri differ,
+            // and only this thread writes row ri's columns j+1+cg, j+1+cg+ncg, ...)
+            constexpr int U = 9;
+            zd pvv[U], rvv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = j + 1 + cg + u * ncg;
+                if (c < ncol) {
+                    pvv[u] = a[pr * lda + c];
+                    rvv[u] = a[ri * lda + c];
+                }
+            }
             const zd l = zmul(a[ri * lda + j], zrecip(pv));
-            for (int c = j + 1 + cg; c < ncol; c += ncg) a[ri * lda + c] = zfms(l, a[pr * lda + c], a[ri * lda + c]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = j + 1 + cg + u * ncg;
+                if (c < ncol) a[ri * lda + c] = zfms(l, pvv[u], rvv[u]);
+            }
+            for (int c = j + 1 + cg + U * ncg; c < ncol; c += ncg) a[ri * lda + c] = zfms(l, a[pr * lda + c], a[ri * lda + c]);
         }
         __syncthreads();
     }
@@ -242,8 +261,6 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     int* sg = t + gmax;
     int* lp = sg + gmax;
     int* piv = lp + gmax;
-    // [k][k+1] capacitance | rhs: LDS when it fits, else this block's global slot
-    zd* a = gscratch ? gscratch + ((size_t)f * gridDim.x + b) * gmax * (2 * gmax + 1) : (zd*)(piv + gmax);
     __shared__ int s_k;
     const int g = batch[b];
     if (!gscratch) {
@@ -256,6 +273,10 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     }
     __syncthreads();
     const int k = s_k;
+    // [k][k+1] capacitance | rhs: LDS when it fits, else this block's global slot.  The rest of
+    // the candidate is written once for a matrix pointer and expanded for each case, so that
+    // the LDS case compiles to LDS instructions (a pointer that may be either is a flat one).
+    auto rest = [&](zd* a) __attribute__((always_inline)) {
     const int lda = k + 1;
     const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
     const zd* q = v.q + (size_t)f * d.ldN;
@@ -323,6 +344,11 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
         out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
     }
     if (tid == 0 && out_info) out_info[b * d.nf + f] = info;
+    };
+    if (!gscratch)
+        rest((zd*)(piv + gmax));
+    else
+        rest(gscratch + ((size_t)f * gridDim.x + b) * gmax * (2 * gmax + 1));
 }
 
 __device__ void loop_after_select_wave(const Dims& d, LoopState* ls, const GroupTab& gt, const int32_t* tile,
@@ -686,7 +712,9 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     int* piv = (int*)smem;
     zd* dpe = (zd*)(piv + 100);
     zd* tmp = dpe + 96;
-    zd* a = gscratch ? gscratch + (size_t)f * gmax * (2 * gmax + 1) : tmp + 96;
+    // the matrix: LDS for k <= 32, else this frequency's global slot (the body is expanded for
+    // each, so that the LDS case compiles to LDS instructions)
+    auto rest = [&](zd* a) __attribute__((always_inline)) {
     // k <= 32 (LDS): Gauss-Jordan on [cap | rhs | I]; larger k: LU, back substitution, inverse
     const bool gj = gscratch == nullptr;
     const int lda = gj ? 2 * k + 1 : k + 1;
@@ -771,6 +799,11 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         st[e] = x;
     }
     if (tid == 0 && info_out) info_out[f] = info;
+    };
+    if (!gscratch)
+        rest(tmp + 96);
+    else
+        rest(gscratch + (size_t)f * gmax * (2 * gmax + 1));
 }
 
 void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
