@@ -121,16 +121,23 @@ void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const Dev
 // left != null: partition layout (groups [0, ngroups_part) have disjoint component lists; `left`
 // holds the nleft components outside every group); null: every group's block plus 32-component
 // chunks of all N
+// The device-driven loop's accepted flip, applied to the component mask and the tile map by the
+// finish launch (no workgroup of that kernel reads them): P[t[c]] += s[c] for c < k and
+// tile[gtiles[i]] ^= 1 for i < dyn[kDynNt].
+struct FlipUpdate {
+    int8_t* P = nullptr;
+    int32_t* tile = nullptr;
+    const int32_t* t = nullptr;
+    const int32_t* gtiles = nullptr;
+};
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
                       int K, const int32_t* dyn = nullptr, const int32_t* left = nullptr, int nleft = 0,
-                      int ngroups_part = 0);
+                      int ngroups_part = 0, FlipUpdate fu = FlipUpdate());
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       int k, int K);
-// With dyn (device-driven loop) it is the accept's last kernel: it also commits the accept to
-// the factor count (dyn[K] += dyn[k]) and clears the pending flag.
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles, int32_t* dyn = nullptr);
+                        const int32_t* gtiles, int ntiles);
 
 // ---- PLAIN accept
 void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, zd* V,

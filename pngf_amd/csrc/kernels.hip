@@ -942,6 +942,7 @@ void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, 
 // (its L2 latency hidden behind the MFMAs) and need one barrier per chunk instead of two.
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const double gdbl;
+typedef __attribute__((address_space(1))) double gdbl_w;
 
 __device__ inline zd ldg(const zd* p, size_t i) {
     gdbl* q = (gdbl*)(p + i);
@@ -1092,12 +1093,13 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             const int row = n0 + 16 * r + lq + 4 * v;
             if (row >= d.N) continue;
             const double a = acc[r][c][v];
+            // global-space pointers: the tables' entries would otherwise be flat accesses
             if (!ROW) {
-                const double* w = (const double*)&W0[(size_t)row * d.ldN + t[cc]];
-                ((double*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
+                gdbl* w = (gdbl*)(W0 + (size_t)row * d.ldN + t[cc]);
+                ((gdbl_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
             } else {
-                const double* w = (const double*)&W0[(size_t)t[cc] * d.ldN + row];
-                ((double*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
+                gdbl* w = (gdbl*)(W0 + (size_t)t[cc] * d.ldN + row);
+                ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
             }
         }
     }
@@ -1299,11 +1301,16 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
                                                         const zd* __restrict__ Bt, size_t bt_stride,
                                                         const int32_t* __restrict__ s, int k, int K, int nchunk,
                                                         const int32_t* dyn, const int32_t* __restrict__ left,
-                                                        int nleft, int ngroups_part) {
+                                                        int nleft, int ngroups_part, FlipUpdate fu) {
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
         K = dyn[kDynKK];
+    }
+    if (fu.P && blockIdx.x == 0 && blockIdx.y == 0) {
+        // the flip's components are distinct, and so are its tiles: one thread each
+        for (int c = threadIdx.x; c < k; c += 256) fu.P[fu.t[c]] = (int8_t)(fu.P[fu.t[c]] + s[c]);
+        for (int i = threadIdx.x; i < dyn[kDynNt]; i += 256) fu.tile[fu.gtiles[i]] ^= 1;
     }
     __shared__ zd Xs[G][G + 1];
     __shared__ zd As[G][G + 1];
@@ -1423,50 +1430,39 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
 
 void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
                       const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
-                      int K, const int32_t* dyn, const int32_t* left, int nleft, int ngroups_part) {
+                      int K, const int32_t* dyn, const int32_t* left, int nleft, int ngroups_part, FlipUpdate fu) {
     // G: components per workgroup (the group size bound); k <= 24 halves the LDS footprint
     if (d.gmax <= 24) {
         const int nchunk = (d.N + 23) / 24;
         const int nblk = left ? ngroups_part + (nleft + 23) / 24 : nchunk + d.ngroups;
         hipLaunchKernelGGL(lr_finish_kernel<24>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
-                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
+                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     } else {
         const int nchunk = (d.N + 31) / 32;
         const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
         hipLaunchKernelGGL(lr_finish_kernel<32>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
-                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part);
+                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     }
 }
 
 __global__ void mask_update_kernel(int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                                   const int32_t* gtiles, int ntiles, int32_t* dyn) {
-    if (dyn) {
-        if (!dyn[kDynActive]) return;
-        k = dyn[kDynK];
-        ntiles = dyn[kDynNt];
-    }
+                                   const int32_t* gtiles, int ntiles) {
     // the components of a flip are distinct, and so are its tiles: one thread each
     for (int c = threadIdx.x; c < k; c += blockDim.x) P[t[c]] = (int8_t)(P[t[c]] + s[c]);
     for (int i = threadIdx.x; i < ntiles; i += blockDim.x) tile[gtiles[i]] ^= 1;
-    if (dyn) {
-        __syncthreads();  // every thread has read dyn before it changes
-        if (threadIdx.x == 0) {
-            dyn[kDynKK] += k;
-            dyn[kDynActive] = 0;
-        }
-    }
 }
 
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
-                        const int32_t* gtiles, int ntiles, int32_t* dyn) {
-    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(128), 0, st, P, tile, t, s, k, gtiles, ntiles, dyn);
+                        const int32_t* gtiles, int ntiles) {
+    hipLaunchKernelGGL(mask_update_kernel, dim3(1), dim3(128), 0, st, P, tile, t, s, k, gtiles, ntiles);
 }
 
 // ---------------------------------------------------- device-driven loop -----
 // Raster first-improvement iterations without a host round trip.  Per iteration: the scoring
 // kernel (lr_eval_kernel) reads the batch size from ls->nb, and its last workgroup selects,
 // writes the accept record and the next batch; the accept kernels read k, K and "accept
-// pending" from ls->dyn; mask_update_kernel commits the accept to the factor count.
+// pending" from ls->dyn; the finish kernel applies the flip to the mask and the tile map, and
+// the next iteration's loop control adds its k to the factor count.
 // Size and start of the next batch: the candidates after the current position, cyclically,
 // at most bs of them (doubling after each non-improving batch), none once the run is done.
 __device__ void loop_next_batch(LoopState* ls, int* B_out, int* pos_out, int* ncl_out) {
@@ -1502,10 +1498,12 @@ __device__ void loop_after_select_wave(const Dims& d, LoopState* ls, const Group
                                        const int8_t* P, const int32_t* owners, const int32_t* batch,
                                        const int32_t* info, const LoopCtl& c) {
     const int tid = threadIdx.x;
-    if (ls->done) {
-        if (tid == 0) ls->dyn[kDynActive] = 0;
-        return;
+    // the previous iteration's accept (its kernels have all run) joins the factor count
+    if (tid == 0 && ls->dyn[kDynActive]) {
+        ls->dyn[kDynKK] += ls->dyn[kDynK];
+        ls->dyn[kDynActive] = 0;
     }
+    if (ls->done) return;
     const int nb = ls->nb, sel = ls->sel;
     // a singular capacitance matrix in the batch ends the run with an error, as in the host loop
     bool bad = false;

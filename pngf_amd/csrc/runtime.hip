@@ -1582,9 +1582,13 @@ void device_loop_iteration(pngf_handle* h, int i) {
         launch_lr_rowblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv,
                          dyn);
     (void)hipStreamWaitEvent(st, h->ev_join, 0);
+    FlipUpdate fu;
+    fu.P = h->d_P;
+    fu.tile = h->d_tile;
+    fu.t = dt;
+    fu.gtiles = dgt;
     launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, d.gmax, 0,
-                     dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
-    launch_mask_update(st, h->d_P, h->d_tile, dt, ds, 0, dgt, 0, ls->dyn);
+                     dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv, fu);
 }
 
 // Captures kLoopChunk iterations into h->loop_exec once (after device_loop_prepare); a capture
