@@ -81,6 +81,8 @@ void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32
                          const LoopCtl& c, int32_t* iterlog = nullptr);
 
 // ---- eval
+// batches of at least this many (candidate, frequency) pairs are scored one pair per wave
+constexpr int kWaveEvalMin = 2048;
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
                     int32_t* out_info, zd* gscratch, const int32_t* nb_dev = nullptr);

@@ -424,10 +424,154 @@ void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32
                        iterlog);
 }
 
+// x of lane `src` in every lane (src uniform)
+__device__ inline zd readlane_z(zd x, int src) {
+    const unsigned long long rx = (unsigned long long)__double_as_longlong(x.x);
+    const unsigned long long ry = (unsigned long long)__double_as_longlong(x.y);
+    const unsigned long long ox = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)rx, src) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(rx >> 32), src) << 32);
+    const unsigned long long oy = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)ry, src) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(ry >> 32), src) << 32);
+    return zmk(__longlong_as_double((long long)ox), __longlong_as_double((long long)oy));
+}
+
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
+
+// LOWRANK scoring of one (candidate b, frequency f) per wave, for flips of at most G <= 32
+// components: lane r holds row r of [cap | rhs] in registers (cap = I + D[t,t] diag(s), D the
+// maintained block) and the Gauss-Jordan elimination runs with neither LDS traffic nor
+// barriers: the pivot search is a wave reduction and the pivot row is read across lanes.  The
+// rows stay in their lanes; pos is the lane's logical row, the order zgetf2's interchanges
+// produce.  The operations and their order are those of lr_eval_one with lds_gj, so the scores
+// are bitwise the same.
+template <int G>
+__global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, GroupTab gt,
+                                                          const int32_t* __restrict__ tile,
+                                                          const int8_t* __restrict__ P,
+                                                          const int32_t* __restrict__ owners,
+                                                          const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
+                                                          int32_t* __restrict__ out_info, const int32_t* nb_dev) {
+    const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+    if (nb_dev && b >= *nb_dev) return;
+    __shared__ zd dpe[G], qt[G], rt[3 * G], ys[G];
+    __shared__ int t[G], sg[G], lp[G];
+    const int gmax = d.gmax;
+    const int g = batch[b];
+    const int k = change_set_wave(gt, g, tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    if (lane < k) {
+        const zd e = v.Einc[(size_t)f * d.ldN + t[lane]];
+        dpe[lane] = sg[lane] > 0 ? zmk(-e.x, -e.y) : e;
+        qt[lane] = v.q[(size_t)f * d.ldN + t[lane]];
+    }
+    for (int e = lane; e < d.ne * k; e += 64) {
+        const int r = e / k, c = e - r * k;
+        rt[r * G + c] = v.R[((size_t)f * d.ne + r) * d.ldN + t[c]];
+    }
+    const bool row = lane < k;
+    const int lr = row ? lp[lane] : 0;
+    zd a[G];
+    zd wd = zmk(0.0, 0.0);  // W[t_r][t_r]
+#pragma unroll
+    for (int c = 0; c < G; c++) {
+        a[c] = zmk(0.0, 0.0);
+        if (row && c < k) {
+            const zd w = dblk(D, gmax, lr, lp[c], d.sym);
+            zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
+            if (c == lane) {
+                wd = w;
+                val.x += 1.0;
+            }
+            a[c] = val;
+        }
+    }
+    __syncthreads();
+    // rhs_r = q_t_r + sum_c W[t_r][t_c] dPE_c, with W[t_r][t_c] = s_c cap[r][c] off the diagonal
+    zd bv = zmk(0.0, 0.0);
+    if (row) {
+        bv = qt[lane];
+#pragma unroll
+        for (int c = 0; c < G; c++) {
+            if (c < k) {
+                zd w = wd;
+                if (c != lane) w = sg[c] > 0 ? a[c] : zmk(-a[c].x, -a[c].y);
+                bv = zfma(w, dpe[c], bv);
+            }
+        }
+    }
+    int pos = lane, info = 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        if (j < k) {
+            // pivot: the first logical row >= j of maximal |Re|+|Im| in column j (the bit
+            // pattern of a non-negative double orders like its value; + 1 keeps every row above
+            // the lanes that take no part)
+            const unsigned long long key =
+                row && pos >= j ? (unsigned long long)__double_as_longlong(zabs1(a[j])) + 1ull : 0ull;
+            const unsigned long long mk = __ockl_wfred_max_u64(key);
+            const unsigned int mp = __ockl_wfred_min_u32(key == mk ? (unsigned)pos : 0xffffffffu);
+            const int pr = __ffsll((unsigned long long)__ballot(row && pos == (int)mp)) - 1;
+            if (pos == j) pos = (int)mp;
+            else if (pos == (int)mp) pos = j;
+            if (mk == 1ull && info == 0) info = j + 1;  // the maximum is 0
+            const zd pv = readlane_z(a[j], pr);
+            if (pv.x != 0.0 || pv.y != 0.0) {
+                const bool upd = row && lane != pr;
+                const zd l = zmul(a[j], zrecip(pv));
+#pragma unroll
+                for (int c = j + 1; c < G; c++) {
+                    if (c < k) {
+                        const zd pc = readlane_z(a[c], pr);
+                        if (upd) a[c] = zfms(l, pc, a[c]);
+                    }
+                }
+                const zd pb = readlane_z(bv, pr);
+                if (upd) bv = zfms(l, pb, bv);
+            }
+        }
+    }
+    // this lane's storage row is logical row pos: d_pos in column pos, d_pos y_pos in the rhs
+    if (row) {
+        zd dd = a[0];
+#pragma unroll
+        for (int c = 1; c < G; c++)
+            if (c == pos) dd = a[c];
+        ys[pos] = zdiv(bv, dd);
+    }
+    __syncthreads();
+    if (lane < d.ne) {
+        zd acc = v.Eobj[f * d.ne + lane];
+        for (int c = 0; c < k; c++) {
+            const zd y = ys[c];
+            const zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+            acc = zfma(rt[lane * G + c], w, acc);
+        }
+        out_eobj[((size_t)b * d.nf + f) * d.ne + lane] = acc;
+    }
+    if (lane == 0 && out_info) out_info[b * d.nf + f] = info;
+}
+
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
                     int32_t* out_info, zd* gscratch, const int32_t* nb_dev) {
     dim3 grid(nb, d.nf);
+    // A wave per candidate takes longer per candidate than a workgroup of two (its column
+    // updates run on one wave) but issues far less work in all (no LDS round trips or barriers):
+    // it is the faster choice once the batch gives every SIMD a few candidates, as a full sweep
+    // does.  The device-driven loop's batches (32 candidates after an accept) stay on the
+    // workgroup kernel.  Both give bitwise the same scores.
+    const bool wave = !gscratch && d.gmax <= 32 && !nb_dev && (int64_t)nb * d.nf >= kWaveEvalMin;
+    if (wave && d.gmax <= 24) {
+        hipLaunchKernelGGL(lr_eval_wave_kernel<24>, grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch, out_eobj,
+                           out_info, nb_dev);
+        return;
+    }
+    if (wave) {
+        hipLaunchKernelGGL(lr_eval_wave_kernel<32>, grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch, out_eobj,
+                           out_info, nb_dev);
+        return;
+    }
     int nth = d.gmax <= 32 ? 128 : 256;
     size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,

@@ -296,6 +296,31 @@ def test_tile_variants_are_bitwise_identical():
         assert r["acc_F"].tolist() == runs[0]["acc_F"].tolist()
 
 
+@pytest.mark.parametrize("layout", [P.LAYOUT_DISJOINT, P.LAYOUT_SHARED])
+def test_wave_scoring_matches_workgroup_scoring(layout):
+    """All 225 candidates of 15x15 tiles at 10 frequencies in one batch (2250 pairs: scored one
+    pair per wave, the capacitance matrix in registers) and in two batches of 113 and 112
+    (1130 pairs or fewer: the two-wave workgroup kernel, matrix in LDS): bitwise the same
+    scores, on the default synthetic G (cond(C) ~ 1e5, where any change in rounding shows)."""
+    T, nf = 15, 10
+    gpu = P.Optimizer(layout, T, num_freq=nf, method=P.METHOD_LOWRANK)
+    for f, hz in enumerate(freq_list(nf, 26e9, 34e9)):
+        gpu.set_frequency(f, hz)
+        gpu.synth_g(f)
+    gpu.set_tile_map(initial_pattern(T, 3))
+    gpu.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+    gpu.init_state()
+    cands = [[t] for t in range(T * T) if t != (T - 1) // 2 * (T + 1)]
+    F1, e1 = gpu.eval_batch(cands)
+    F2a, e2a = gpu.eval_batch(cands[:113])
+    F2b, e2b = gpu.eval_batch(cands[113:])
+    assert len(cands) * nf >= 2048 and 113 * nf < 2048
+    assert np.array_equal(F1, np.concatenate([F2a, F2b]))
+    assert np.array_equal(e1, np.concatenate([e2a, e2b]))
+    assert np.all(np.isfinite(F1))
+    gpu.close()
+
+
 def test_autotune_reports_variants():
     T = 7
     gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, [30e9], P.METHOD_LOWRANK)
