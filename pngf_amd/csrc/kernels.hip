@@ -251,7 +251,7 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
                             const int32_t* batch, zd* __restrict__ out_eobj, int32_t* __restrict__ out_info,
                             zd* gscratch) {
     extern __shared__ zd smz[];
-    const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x, nth = blockDim.x;
+    const int b = blockIdx.y, f = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int gmax = d.gmax;
     zd* dpe = smz;                                // [gmax]
     zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
@@ -348,7 +348,7 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     if (!gscratch)
         rest((zd*)(piv + gmax));
     else
-        rest(gscratch + ((size_t)f * gridDim.x + b) * gmax * (2 * gmax + 1));
+        rest(gscratch + ((size_t)b * gridDim.x + f) * gmax * (2 * gmax + 1));
 }
 
 __device__ void loop_after_select_wave(const Dims& d, LoopState* ls, const GroupTab& gt, const int32_t* tile,
@@ -362,7 +362,7 @@ __global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __
                                const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
                                const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
                                int32_t* __restrict__ out_info, zd* gscratch, const int32_t* nb_dev) {
-    if (nb_dev && (int)blockIdx.x >= *nb_dev) return;
+    if (nb_dev && (int)blockIdx.y >= *nb_dev) return;
     lr_eval_one(d, v, gt, tile, P, owners, batch, out_eobj, out_info, gscratch);
 }
 
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, Gro
                                                           const int32_t* __restrict__ owners,
                                                           const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
                                                           int32_t* __restrict__ out_info, const int32_t* nb_dev) {
-    const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+    const int b = blockIdx.y, f = blockIdx.x, lane = threadIdx.x;
     if (nb_dev && b >= *nb_dev) return;
     __shared__ zd dpe[G], qt[G], rt[3 * G], ys[G];
     __shared__ int t[G], sg[G], lp[G];
@@ -555,7 +555,9 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, Gro
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
                     int32_t* out_info, zd* gscratch, const int32_t* nb_dev) {
-    dim3 grid(nb, d.nf);
+    // frequency fastest in dispatch order: with a device-resident batch size the workgroups of
+    // the batch's candidates are dispatched first, the ones past it (which return at once) after
+    dim3 grid(d.nf, nb);
     // A wave per candidate takes longer per candidate than a workgroup of two (its column
     // updates run on one wave) but issues far less work in all (no LDS round trips or barriers):
     // it is the faster choice once the batch gives every SIMD a few candidates, as a full sweep
