@@ -268,8 +268,11 @@ def main():
         if ev["launches"]:
             ncand = rb["flip_count"] // max(1, rb["accepts"] + 1)
             sweep_ms = ev["total_ms"] / ev["launches"]
+            # algorithmic traffic of a score: the k x k block and k-vectors, 16 (k^2 + k (ne + 2)) B
+            gbps = ev["bytes"] / (ev["total_ms"] * 1e-3) / 1e9
             sweep = dict(candidates=ncand, freqs=len(wl["freqs"]), ms=sweep_ms, launches=ev["launches"],
-                         flips_per_s=ncand / (sweep_ms * 1e-3))
+                         flips_per_s=ncand / (sweep_ms * 1e-3), algo_bytes=ev["bytes"] / ev["launches"],
+                         hbm_GBps=gbps, hbm_frac=gbps / PEAK_HBM_GBS)
             log("[bench] sweep of %d candidates x %d frequencies: %.1f us (HIP events)" % (
                 ncand, len(wl["freqs"]), sweep_ms * 1e3))
     opt.close()
