@@ -152,6 +152,14 @@ int pngf_eval(pngf_handle* h, const int32_t* tiles, int32_t ntiles, pngf_c128* n
    ntiles: [ncand].  F_out: [ncand]; eobj_out: [ncand][num_freq][num_eobj] (may be null). */
 int pngf_eval_batch(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand,
                     double* F_out, pngf_c128* eobj_out);
+/* First improvement over a visit sequence against the current state: the sequential loop of
+   main.cpp:338-459 without its accept (`if (F < F_best)`, strict, main.cpp:429), scored in
+   batches.  tiles/ntiles as in pngf_eval_batch.  *accepted_idx = index of the first flip with
+   F < F_best, or -1; *F_new = its objective (F_best if none; may be null); *evaluated = flips up
+   to and including it, or ncand (the reference's flip_count increments; may be null).  The flip
+   is not committed: pngf_accept does that. */
+int pngf_sweep(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand, double F_best,
+               int32_t* accepted_idx, double* F_new, int64_t* evaluated);
 /* Commits a flip: update_C_inv + compute_PEinc + compute_R_matrix(0) at every frequency
    (pngfopt.hpp:256-367, 542-552, 237-250).  The flip is named explicitly instead of
    relying on the last compute_updated_Eobj call. */

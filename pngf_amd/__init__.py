@@ -38,7 +38,7 @@ EXPORTS = [
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many", "pngf_dbs_run_pool",
-    "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity",
+    "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity", "pngf_sweep",
 ]
 
 
@@ -140,6 +140,8 @@ def lib():
         "pngf_autotune_result": (I, [P, ct.POINTER(I), ct.POINTER(I), P]),
         "pngf_symmetry": (I, [P, ct.POINTER(I), ct.POINTER(ct.c_double)]),
         "pngf_factor_capacity": (I, [P, ct.POINTER(I), ct.POINTER(I)]),
+        "pngf_sweep": (I, [P, ct.c_void_p, ct.c_void_p, I, ct.c_double, ct.POINTER(I), ct.POINTER(ct.c_double),
+                           ct.POINTER(ct.c_int64)]),
         "pngf_features": (I, [P, P, I, P]),
         "pngf_dbs_run_ranked": (I, [P, P, ct.POINTER(RankParams), ct.POINTER(DbsParams),
                                     ct.POINTER(DbsResult)]),
@@ -266,6 +268,19 @@ class Optimizer:
         e = np.zeros((n, self.nf, self.ne), dtype=np.complex128)
         self._ck(lib().pngf_eval_batch(self.h, _p(t), _p(nt), n, _p(F), _p(e)), "eval_batch")
         return F, e
+
+    def sweep(self, cands, F_best):
+        """First improving flip of a visit sequence (not committed): (index or -1, F, flips scored)."""
+        n = len(cands)
+        t = np.full((max(n, 1), self.max_flips), -1, dtype=np.int32)
+        nt = np.zeros(max(n, 1), dtype=np.int32)
+        for i, c in enumerate(cands):
+            t[i, :len(c)] = c
+            nt[i] = len(c)
+        idx, F, ev = ct.c_int32(), ct.c_double(), ct.c_int64()
+        self._ck(lib().pngf_sweep(self.h, _p(t), _p(nt), n, float(F_best), ct.byref(idx), ct.byref(F), ct.byref(ev)),
+                 "sweep")
+        return idx.value, F.value, ev.value
 
     def accept(self, tiles):
         """update_C_inv + compute_PEinc + compute_R_matrix(0) for the named flip."""

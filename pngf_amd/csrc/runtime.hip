@@ -1315,6 +1315,56 @@ int pngf_eval_batch(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles,
     return flush_batch();
 }
 
+int pngf_sweep(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand, double F_best,
+               int32_t* accepted_idx, double* F_new, int64_t* evaluated) {
+    if (!h || !tiles || !ntiles || ncand < 0 || !accepted_idx) return fail(h, PNGF_ERR_ARG, "bad sweep");
+    CK(check_ready(h));
+    const int mf = h->cfg.max_flips;
+    *accepted_idx = -1;
+    if (F_new) *F_new = F_best;
+    if (evaluated) *evaluated = ncand;
+    h->h_pin_d[8] = F_best;
+    HC(h, hipMemcpyAsync(h->d_Fbest, h->h_pin_d + 8, sizeof(double), hipMemcpyHostToDevice, h->st));
+    std::vector<int> gs, idx;
+    // scores the pending candidates as one batch; true when one of them improves
+    auto flush = [&](bool* found) -> int {
+        *found = false;
+        if (gs.empty()) return PNGF_OK;
+        CK(eval_launch(h, gs.data(), (int)gs.size()));
+        int sel;
+        double fs;
+        CK(sync_sel(h, (int)gs.size(), &sel, &fs, true));
+        if (sel >= 0) {
+            *found = true;
+            *accepted_idx = idx[sel];
+            if (F_new) *F_new = fs;
+            if (evaluated) *evaluated = (int64_t)idx[sel] + 1;
+        }
+        gs.clear();
+        idx.clear();
+        return PNGF_OK;
+    };
+    bool found = false;
+    for (int c = 0; c < ncand && !found; c++) {
+        int g;
+        CK(group_for(h, tiles + (size_t)c * mf, ntiles[c], &g));
+        if (g == h->adhoc) {
+            // the ad-hoc slot holds one flip at a time: score what is pending, then this one alone
+            CK(flush(&found));
+            if (found) break;
+            gs.push_back(g);
+            idx.push_back(c);
+            CK(flush(&found));
+        } else {
+            gs.push_back(g);
+            idx.push_back(c);
+            if ((int)gs.size() >= h->ngroups_drv) CK(flush(&found));
+        }
+    }
+    if (!found) CK(flush(&found));
+    return PNGF_OK;
+}
+
 int pngf_accept(pngf_handle* h, const int32_t* tiles, int32_t nt) {
     if (!h) return PNGF_ERR_ARG;
     CK(check_ready(h));

@@ -296,6 +296,33 @@ def test_tile_variants_are_bitwise_identical():
         assert r["acc_F"].tolist() == runs[0]["acc_F"].tolist()
 
 
+@pytest.mark.parametrize("driver", [P.DRIVER_RASTER, P.DRIVER_REF])
+def test_sweep_is_first_improvement_of_sequential_evals(driver):
+    """pngf_sweep over a shuffled visit sequence returns the first flip whose objective, scored
+    alone, is strictly below F_best (main.cpp:429), with that objective bitwise and
+    the number of flips scored; with F_best below every score it returns -1 after the whole
+    sequence.  Raster driver: single tiles, scored in batches; reference driver: single tiles
+    are ad-hoc flips (scored one at a time) between two of its mirrored pairs."""
+    T = 7
+    mid = (T - 1) // 2
+    freqs = freq_list(2, 29e9, 31e9)
+    mf = 1 if driver == P.DRIVER_RASTER else 4
+    gpu, ors, tm = setup_pair(P.LAYOUT_SHARED, T, freqs, P.METHOD_LOWRANK, driver=driver, max_flips=mf,
+                              objective=P.OBJ_S11)
+    rng = np.random.default_rng(5)
+    cands = [[int(t)] for t in rng.permutation(T * T) if t != mid * (T + 1)]
+    if driver == P.DRIVER_REF:
+        cands = cands[:12] + [[mid * T + 0, mid * T + T - 1], [mid * T + 1, mid * T + T - 2]] + cands[12:]
+    F = [gpu.eval_batch([c])[0][0] for c in cands]
+    F0 = gpu.state()["F"]
+    first = next(i for i, x in enumerate(F) if x < F0)
+    idx, Fs, ev = gpu.sweep(cands, F0)
+    assert (idx, ev) == (first, first + 1) and Fs == F[first]
+    idx, Fs, ev = gpu.sweep(cands, min(F) * (1 - 1e-12))
+    assert (idx, ev) == (-1, len(cands)) and Fs == min(F) * (1 - 1e-12)
+    gpu.close()
+
+
 @pytest.mark.parametrize("layout", [P.LAYOUT_DISJOINT, P.LAYOUT_SHARED])
 def test_wave_scoring_matches_workgroup_scoring(layout):
     """All 225 candidates of 15x15 tiles at 10 frequencies in one batch (2250 pairs: scored one
