@@ -174,6 +174,11 @@ __device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, c
     return k;
 }
 
+// Wave-wide maximum and minimum of the ROCm device library (the HIP headers declare the 64-bit
+// one only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES); every lane gets the result.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
+
 // Gauss-Jordan elimination with partial pivoting of the k x k leading block (k <= 32) of the
 // LDS matrix a (leading dimension lda), applying the row operations to columns k..ncol-1.
 // Rows stay in place: perm[i] is the storage row of logical row i, the order that zgetf2's
@@ -183,8 +188,6 @@ __device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, c
 // the solution in columns k..ncol-1.  Returns LAPACK-style info (0, or j+1 for the first zero
 // pivot, whose elimination step is skipped).  All threads of the block call it, after a barrier
 // that follows the writes of a; blockDim.x is a multiple of 64.
-extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
-
 __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int ri = tid & 31, cg = tid >> 5, ncg = blockDim.x >> 5;
@@ -434,8 +437,6 @@ __device__ inline zd readlane_z(zd x, int src) {
                                   ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(ry >> 32), src) << 32);
     return zmk(__longlong_as_double((long long)ox), __longlong_as_double((long long)oy));
 }
-
-extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
 
 // LOWRANK scoring of one (candidate b, frequency f) per wave, for flips of at most G <= 32
 // components: lane r holds row r of [cap | rhs] in registers (cap = I + D[t,t] diag(s), D the
