@@ -1,0 +1,68 @@
+"""bench.py's host-side contract, without a GPU: the one-GPU refusal and the roofline record it
+builds from the library's event-timed kernel statistics."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout + r.stderr
+    assert r.stdout.strip() == ""  # no JSON line
+
+
+class FakeHandle:
+    """kernel_stats as the library reports them: launches, summed event ms, algorithmic bytes
+    and flops per category (0 eval, 1 colblk, 2 rowblk, 3 acc_small, 4 vcinv, 5 rankk)."""
+
+    def __init__(self, N, stats):
+        self.N = N
+        self.stats = stats
+
+    def kernel_stats(self, k):
+        return self.stats.get(k, dict(launches=0, total_ms=0.0, bytes=0.0, flops=0.0))
+
+
+def test_roofline_of_the_column_strip():
+    # 183 launches of the 21x21x9 strip: 8 N K k flop and 16 (N K + 2 N k + K k) B per
+    # frequency at the average factor depth K = 2196, k = 24
+    N, K, k, nf, n = 10590, 2196, 24, 9, 183
+    flops = nf * 8.0 * N * K * k
+    byts = nf * 16.0 * (N * K + 2 * N * k + K * k)
+    h = FakeHandle(N, {0: dict(launches=220, total_ms=8.3, bytes=1e9, flops=1e9),
+                       1: dict(launches=n, total_ms=n * 0.72, bytes=n * byts, flops=n * flops)})
+    rec, allk = bench.roofline(h, gmax=k)
+    assert rec["kernel"] == "colblk" and rec["launches"] == n
+    # arithmetic intensity 11.7 flop/B is above the 9.8 ridge of 78.6 TFLOP/s over 8 TB/s
+    assert rec["bound"] == "mfma" and rec["unit"] == "TFLOP/s" and rec["peak"] == 78.6
+    assert rec["achieved"] == pytest.approx(flops / 0.72e-3 / 1e12)
+    assert rec["frac"] == pytest.approx(rec["achieved"] / 78.6)
+    # HBM bytes: the algorithmic bytes plus the staged block re-fetched by 7 more XCD L2s
+    assert rec["traffic"] == pytest.approx(byts * (1 + 7 * k / N))
+    assert set(allk) == {"eval", "colblk", "rowblk", "acc_small", "vcinv", "rankk"}
+
+
+def test_roofline_of_an_hbm_bound_kernel():
+    # the PLAIN score streams 16 N (2k + 1 + ne) B for 8 N (k^2 + 2k + ne) flop: 6.3 flop/B
+    N, k = 10590, 24
+    byts = 16.0 * N * (2 * k + 2)
+    flops = 8.0 * N * (k * k + 2 * k + 1)
+    h = FakeHandle(N, {0: dict(launches=10, total_ms=10 * 0.002, bytes=10 * byts, flops=10 * flops)})
+    rec, _ = bench.roofline(h, gmax=k)
+    assert rec["kernel"] == "eval" and rec["bound"] == "hbm" and rec["unit"] == "GB/s"
+    assert rec["achieved"] == pytest.approx(byts / 2e-6 / 1e9)
+    assert rec["traffic"] is None
+
+
+def test_no_kernel_timed():
+    rec, allk = bench.roofline(FakeHandle(100, {}))
+    assert rec is None and allk == {}
