@@ -320,6 +320,7 @@ def test_sweep_is_first_improvement_of_sequential_evals(driver):
     assert (idx, ev) == (first, first + 1) and Fs == F[first]
     idx, Fs, ev = gpu.sweep(cands, min(F) * (1 - 1e-12))
     assert (idx, ev) == (-1, len(cands)) and Fs == min(F) * (1 - 1e-12)
+    assert gpu.sweep([], F0) == (-1, F0, 0)
     gpu.close()
 
 
