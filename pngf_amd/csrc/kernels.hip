@@ -1490,11 +1490,13 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     const zd* stf = acc + (size_t)f * acc_stride;
     if (d.lam && bx == 0) {
         // the accept's M rows, kept as Lam[K + c] = s_c cap^-1[c][:] (see DevTabs)
-        zd* L = tb.Lam[f] + (size_t)K * d.gmax;
+        gdbl_w* L = (gdbl_w*)(tb.Lam[f] + (size_t)K * d.gmax);
         for (int e = tid; e < k * k; e += 256) {
             const int c = e / k, l = e - c * k;
             const zd x = stf[e];
-            L[(size_t)c * d.gmax + l] = s[c] < 0 ? zmk(-x.x, -x.y) : x;
+            const size_t o = ((size_t)c * d.gmax + l) * 2;
+            L[o] = s[c] < 0 ? -x.x : x.x;
+            L[o + 1] = s[c] < 0 ? -x.y : x.y;
         }
         if (f == 0 && tid < k) {
             tb.fstart[K + tid] = K;
@@ -1513,7 +1515,7 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     for (int e = tid; e < nrow * k; e += 256) {
         const int rr = e / k, c = e - rr * k;
         const int n = rows[rr];
-        As[rr][c] = A[(size_t)n * d.Kmax + K + c];
+        As[rr][c] = ldg(A, (size_t)n * d.Kmax + K + c);
         if (!d.sym) Ss[rr][c] = S[(size_t)n * d.gmax + c];
     }
     __syncthreads();
@@ -1541,10 +1543,15 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     __syncthreads();
     if (do_rows) {
         if (!d.lam) {
-            zd* M = tb.Mall[f];
+            // global-space stores (a table entry is otherwise a flat pointer, whose stores the
+            // LDS waits below would also wait for)
+            gdbl_w* M = (gdbl_w*)tb.Mall[f];
             for (int e = tid; e < k * nrow; e += 256) {
                 const int c = e / nrow, rr = e - c * nrow;
-                M[(size_t)(K + c) * d.ldN + rows[rr]] = Ss[c][rr];
+                const zd v = Ss[c][rr];
+                const size_t o = ((size_t)(K + c) * d.ldN + rows[rr]) * 2;
+                M[o] = v.x;
+                M[o + 1] = v.y;
             }
         }
         const zd* w = stf + 2 * k * k + k;
