@@ -141,3 +141,21 @@ def test_write_history_layout(tmp_path):
         "1 4 0 0.6 0.5 0.125 1e-05 0 \n"
         "2 9 1 0.25 0.5 -1.5 2 12.3457 \n"
         "..xx\n")
+
+
+def test_headers_compile_as_c_and_cpp():
+    """The C ABI header is plain C99 (a cgo or FFI binding includes it as C); the C++ view
+    compiles on top of it."""
+    import shutil
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    if shutil.which("gcc") is None or shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=c99", "-Wall", "-Werror", "-x", "c",
+                        os.path.join(inc, "pngf_gpu.h")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(["g++", "-fsyntax-only", "-std=c++17", "-Wall", "-I", inc, "-x", "c++", "-"],
+                       input='#include "pngf_optimizer.hpp"\nint main() { return 0; }\n', capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+
