@@ -92,9 +92,14 @@ def roofline(opt, gmax=24):
     return rec, allk
 
 
-def cpu_baseline(wl, g0, cinv0, budget_s):
+def cpu_baseline(wl, g0, cinv0, budget_s, gpu_run):
     """Reference-structure FP64 CPU path (the oracle) on a bounded sample: one frequency of the
-    same problem, same G and initial state, DBS loop timed for about budget_s seconds."""
+    same problem, same G and initial state, DBS loop timed for about budget_s seconds.  The
+    sample's early phase accepts far more often than a whole run does, and a CPU accept
+    (O(k N^2)) costs much more than an evaluation (O(k^2 N)), so the sample's own flips/s is not
+    the run's: the sample gives the time per evaluation and per accept at one frequency, and the
+    value is the GPU run's evaluation and accept counts priced at those times, at every
+    frequency (the reference scores frequencies one after another)."""
     import oracle as O
     T = wl["T"]
     mid = (T - 1) // 2
@@ -106,12 +111,17 @@ def cpu_baseline(wl, g0, cinv0, budget_s):
     o.compute_R_matrix(1)
     r = O.dbs_run([o], O.OBJ_S11, [wl["freqs"][0]], DX, DY, DZ, max_seconds=budget_s)
     nf = len(wl["freqs"])
-    per_freq = r["flip_count"] / r["loop_s"]
-    return dict(value=per_freq / nf, unit="flips/s", cores=O.num_threads(), kind="port",
-                sample="oracle (C++ restatement of pngfopt.hpp, FP64, OpenMP) on 1 of %d frequencies, "
-                       "same G and initial state, %d evaluations incl. %d accepts in %.1f s; "
-                       "value = single-frequency flips/s / %d (frequencies are serial in the reference)"
-                       % (nf, r["flip_count"], r["accepts"], r["loop_s"], nf))
+    t_eval = r["eval_s"] / max(1, r["flip_count"])
+    t_acc = (r["loop_s"] - r["eval_s"]) / max(1, r["accepts"])
+    run_s = nf * (gpu_run["flip_count"] * t_eval + gpu_run["accepts"] * t_acc)
+    return dict(value=gpu_run["flip_count"] / run_s, unit="flips/s", cores=O.num_threads(), kind="port",
+                projected_run_s=run_s, eval_ms=1e3 * t_eval, accept_ms=1e3 * t_acc,
+                sample="oracle (C++ restatement of pngfopt.hpp, FP64, OpenMP over %d threads, its own loops: "
+                       "no BLAS library) on 1 of %d frequencies, same G and initial state: %d evaluations and %d "
+                       "accepts in %.1f s give %.3f ms per evaluation and %.1f ms per accept; value = the GPU "
+                       "run's %d evaluations and %d accepts at those times x %d frequencies (%.0f s)"
+                       % (O.num_threads(), nf, r["flip_count"], r["accepts"], r["loop_s"], 1e3 * t_eval,
+                          1e3 * t_acc, gpu_run["flip_count"], gpu_run["accepts"], nf, run_s))
 
 
 def copy_probe():
@@ -298,7 +308,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (%.0f s budget)" % args.cpu_seconds)
-        cpu = cpu_baseline(wl, g0, cinv0, args.cpu_seconds)
+        cpu = cpu_baseline(wl, g0, cinv0, args.cpu_seconds, last)
         del g0, cinv0
 
     out = {

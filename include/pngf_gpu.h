@@ -199,7 +199,8 @@ typedef struct {
                               the default INIT_EMPTY) */
     int32_t order;         /* PNGF_ORDER_FIRST (reference semantics, main.cpp:429) or PNGF_ORDER_BEST */
     int32_t loop;          /* PNGF_LOOP_AUTO or PNGF_LOOP_HOST */
-    int32_t reserved;
+    int32_t resume;        /* 1: continue from the driver position in driver_state (first-improvement
+                              order) instead of the start of the visit order */
     /* Optional training log (LOWRANK): for every scored candidate, its ranker inputs
        (pngf_ranker_dims din floats) and the exact change of |S11_f|^2 at every frequency
        (dout floats), appended until log_cap rows. */
@@ -217,7 +218,17 @@ typedef struct {
     int32_t* hist_dec;
     double* hist_perf;
     int64_t hist_cap;
+    /* Optional driver position, PNGF_DRIVER_STATE_WORDS words (may be null): written when the run
+       returns (first-improvement order), read first when resume = 1.  Raster driver: the place in
+       the visit cycle and the count of non-improving flips since the last accept; reference
+       driver: the MagicHat generator (std::mt19937 state), its remaining draws and the last tile
+       tried (magichat.hpp:84-107, main.cpp:343-353).  With the tile map, this is all a run needs
+       to continue on a handle rebuilt by pngf_init_state; the reference keeps no checkpoint
+       (main.cpp:469-472). */
+    uint32_t* driver_state;
 } pngf_dbs_params;
+
+#define PNGF_DRIVER_STATE_WORDS 2048
 
 typedef struct {
     int64_t flip_count;        /* committed evaluations (main.cpp:458) */

@@ -63,10 +63,13 @@ class DbsParams(ct.Structure):
     _fields_ = [("max_evals", ct.c_int64), ("seed", ct.c_int32), ("max_accepts", ct.c_int32),
                 ("acc_tiles", ct.c_void_p), ("acc_eval", ct.c_void_p), ("acc_F", ct.c_void_p),
                 ("batch_min", ct.c_int32), ("batch_max", ct.c_int32), ("rng_discard", ct.c_int32),
-                ("order", ct.c_int32), ("loop", ct.c_int32), ("reserved", ct.c_int32),
+                ("order", ct.c_int32), ("loop", ct.c_int32), ("resume", ct.c_int32),
                 ("log_x", ct.c_void_p), ("log_y", ct.c_void_p), ("log_cap", ct.c_int64),
                 ("hist_tile", ct.c_void_p), ("hist_dec", ct.c_void_p), ("hist_perf", ct.c_void_p),
-                ("hist_cap", ct.c_int64)]
+                ("hist_cap", ct.c_int64), ("driver_state", ct.c_void_p)]
+
+
+DRIVER_STATE_WORDS = 2048
 
 
 class DbsResult(ct.Structure):
@@ -186,6 +189,39 @@ def write_history(path, seed, start_map, r, end_map):
         for i, (t, d, pr) in enumerate(zip(r["hist_tile"], r["hist_dec"], r["hist_perf"])):
             fh.write("%d %d %d " % (i, t, d) + "".join("%g " % v for v in pr) + "\n")
         fh.write("".join("x" if v else "." for v in end_map) + "\n")
+
+
+def save_run(path, seed, tile_map, runs):
+    """Resume point of a DBS run, as JSON: the seed, the tile map reached, the flips accepted so far
+    (primary tile, evaluation index counted from the start of the whole run, objective after the
+    accept), the evaluations done and the driver position.  ``runs`` are the dbs_run results of the
+    run so far, in order.  A run continues with load_run + a handle built from the tile map
+    (set_tile_map, init_state: C^-1 formed again) + dbs_run(resume_from=...)."""
+    import json
+    tiles, evals, F, done = [], [], [], 0
+    for r in runs:
+        tiles += [int(t) for t in r["acc_tiles"]]
+        evals += [int(e) + done for e in r["acc_eval"]]
+        F += [float(x) for x in r["acc_F"]]
+        done += int(r["flip_count"])
+    rec = dict(format="pngf-dbs-resume-1", seed=int(seed), tile_map=[int(v) for v in tile_map],
+               acc_tiles=tiles, acc_eval=evals, acc_F=F, flip_count=done,
+               converged=bool(runs[-1]["converged"]) if runs else False,
+               driver_state=[int(v) for v in runs[-1]["driver_state"]])
+    with open(path, "w") as fh:
+        json.dump(rec, fh)
+
+
+def load_run(path):
+    """The record written by save_run, with numpy arrays for the tile map and driver position."""
+    import json
+    with open(path) as fh:
+        rec = json.load(fh)
+    if rec.get("format") != "pngf-dbs-resume-1":
+        raise ValueError("%s is not a DBS resume record" % path)
+    rec["tile_map"] = np.array(rec["tile_map"], dtype=np.int32)
+    rec["driver_state"] = np.array(rec["driver_state"], dtype=np.uint32)
+    return rec
 
 
 class Optimizer:
@@ -327,7 +363,7 @@ class Optimizer:
 
     # ---- DBS loop (main.cpp:338-459)
     def _dbs_params(self, max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap, order=0,
-                    loop=0, hist_cap=0):
+                    loop=0, hist_cap=0, resume_from=None):
         acc = (np.zeros(max_accepts, dtype=np.int32), np.zeros(max_accepts, dtype=np.int64),
                np.zeros(max_accepts))
         log = None
@@ -338,12 +374,16 @@ class Optimizer:
         if hist_cap:
             hist = (np.zeros(hist_cap, dtype=np.int32), np.zeros(hist_cap, dtype=np.int32),
                     np.zeros((hist_cap, 2 + 3 * self.nf)))
+        ds = np.zeros(DRIVER_STATE_WORDS, dtype=np.uint32)
+        if resume_from is not None:
+            ds[:] = np.asarray(resume_from, dtype=np.uint32)
         p = DbsParams(max_evals, seed, max_accepts, acc[0].ctypes.data, acc[1].ctypes.data, acc[2].ctypes.data,
-                      batch_min, batch_max, rng_discard, order, loop, 0, log[0].ctypes.data if log else None,
+                      batch_min, batch_max, rng_discard, order, loop, int(resume_from is not None),
+                      log[0].ctypes.data if log else None,
                       log[1].ctypes.data if log else None, log_cap,
                       hist[0].ctypes.data if hist else None, hist[1].ctypes.data if hist else None,
-                      hist[2].ctypes.data if hist else None, hist_cap)
-        return p, acc, (log, hist)
+                      hist[2].ctypes.data if hist else None, hist_cap, ds.ctypes.data)
+        return p, acc + (ds,), (log, hist)
 
     @staticmethod
     def _dbs_result(r, acc, logs):
@@ -355,6 +395,8 @@ class Optimizer:
                    acc_tiles=acc[0][:na].copy(), acc_eval=acc[1][:na].copy(), acc_F=acc[2][:na].copy(),
                    ranked_scored=r.ranked_scored, full_sweeps=r.full_sweeps, ranked_steps=r.ranked_steps,
                    ranked_hits=r.ranked_hits)
+        if len(acc) > 3:
+            out["driver_state"] = acc[3].copy()
         if log is not None:
             out["log_x"] = log[0][:r.logged].copy()
             out["log_y"] = log[1][:r.logged].copy()
@@ -366,11 +408,13 @@ class Optimizer:
         return out
 
     def dbs_run(self, max_evals=0, seed=-514213645, max_accepts=100000, batch_min=0, batch_max=0,
-                rng_discard=0, log_cap=0, order=ORDER_FIRST, loop=LOOP_AUTO, hist_cap=0):
+                rng_discard=0, log_cap=0, order=ORDER_FIRST, loop=LOOP_AUTO, hist_cap=0, resume_from=None):
         """DBS to convergence (or max_evals).  hist_cap > 0 also returns the reference Logger's
-        records (hist_tile, hist_dec, hist_perf; see write_history)."""
+        records (hist_tile, hist_dec, hist_perf; see write_history).  The result's driver_state is
+        the driver position at return; passing it back as resume_from (on this handle or on one
+        rebuilt from the tile map) continues the visit order from there."""
         p, acc, log = self._dbs_params(max_evals, seed, max_accepts, batch_min, batch_max, rng_discard, log_cap,
-                                       order, loop, hist_cap)
+                                       order, loop, hist_cap, resume_from)
         r = DbsResult()
         self._ck(lib().pngf_dbs_run(self.h, ct.byref(p), ct.byref(r)), "dbs_run")
         return self._dbs_result(r, acc, log)

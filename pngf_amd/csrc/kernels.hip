@@ -1994,6 +1994,24 @@ void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long
     hipLaunchKernelGGL(asym_kernel, grid, dim3(32, 8), 0, st, N, ldN, G, out);
 }
 
+// *flag |= 1 if any entry of the column-major N x N matrix is NaN or infinite (rows >= N of
+// each column are padding and are not read).  grid.y strides over the columns.
+__global__ void __launch_bounds__(256) nonfinite_kernel(int N, int ldN, const zd* __restrict__ G, int32_t* flag) {
+    int bad = 0;
+    for (int col = blockIdx.y; col < N; col += gridDim.y)
+        for (int r = blockIdx.x * 256 + threadIdx.x; r < N; r += gridDim.x * 256) {
+            const zd v = G[(size_t)col * ldN + r];
+            bad |= !(isfinite(v.x) && isfinite(v.y));
+        }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+void launch_nonfinite(hipStream_t st, int N, int ldN, const zd* G, int32_t* flag) {
+    const int gx = (N + 255) / 256 < 8 ? (N + 255) / 256 : 8;
+    dim3 grid(gx, N < 4096 ? N : 4096);
+    hipLaunchKernelGGL(nonfinite_kernel, grid, dim3(256), 0, st, N, ldN, G, flag);
+}
+
 // W = (W + W^T) / 2 in place (N x N, leading dimension ldN).  The workgroup of tile pair
 // (bx, by), bx <= by, owns both tiles: it stages them in LDS and writes both back, and the two
 // entries of a pair get the same value (addition commutes), so W ends exactly symmetric.

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <random>
+#include <sstream>
 #include <string>
 #include <vector>
 #include <sys/stat.h>
@@ -130,6 +131,7 @@ struct pngf_handle {
     bool sym = false;                // LOWRANK accept takes W[t,:] = W[:,t]^T (G symmetric)
     double asym = 0.0;               // max over frequencies of max|G_ij - G_ji| / max|G_ij|
     unsigned long long* d_asym = nullptr;
+    int32_t* d_flag = nullptr;       // non-finite entries found in G (init)
     std::vector<double> tune_ms;     // measured time per variant (colblk [+ rowblk]), ms
     // pinned host staging
     int32_t* h_pin_i = nullptr;   // batch ids / small ints
@@ -432,6 +434,20 @@ int init_freq(pngf_handle* h, int f) {
     hipStream_t st = h->st;
     const rbz one = {1.0, 0.0}, zero = {0.0, 0.0};
     zd* G = h->Gcm[f];
+    // A NaN or an infinity in G gives NaN scores, which never compare below F_best: the run would
+    // stop as "converged" on garbage.  G is refused here instead (the reference does not check).
+    {
+        int bad = 0;
+        HC(h, hipMemsetAsync(h->d_flag, 0, sizeof(int32_t), st));
+        launch_nonfinite(st, N, ld, G, h->d_flag);
+        HC(h, hipMemcpyAsync(h->h_pin_i, h->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HC(h, hipStreamSynchronize(st));
+        bad = h->h_pin_i[0];
+        for (const zd& v : h->gobj_extra[f])
+            if (!std::isfinite(v.x) || !std::isfinite(v.y)) bad = 1;
+        if (bad)
+            return fail(h, PNGF_ERR_NONFINITE, "G of frequency slot " + std::to_string(f) + " holds NaN or Inf");
+    }
     zd* C = nullptr;
     CK(dalloc_t(h, &C, (size_t)ld * N));
     launch_build_C(st, N, ld, h->d_P, G, C);
@@ -847,6 +863,76 @@ int download_eobj(pngf_handle* h, std::vector<zd>& e) {
     return PNGF_OK;
 }
 
+// ---------------------------------------------------------- driver position ---
+// Words of pngf_dbs_params.driver_state: [0] magic, [1] driver, [2] raster position in the
+// visit cycle, [3] raster non-improving count, [4] reference driver's last tile tried, [5] hat
+// size, [6] draws left in the hat, [7] W = words of the generator state, then W words of the
+// std::mt19937 state (its standard text form: 624 state words and the position) and the hat's
+// index array.
+constexpr uint32_t kDrvMagic = 0x50474E44u;
+constexpr int kDrvHead = 8;
+
+int drv_save(pngf_handle* h, const pngf_dbs_params* p, int pos, int64_t nonimp, const Hat* hat, int last_tried) {
+    if (!p->driver_state) return PNGF_OK;
+    uint32_t* w = p->driver_state;
+    std::memset(w, 0, PNGF_DRIVER_STATE_WORDS * sizeof(uint32_t));
+    w[0] = kDrvMagic;
+    w[1] = (uint32_t)h->cfg.driver;
+    w[2] = (uint32_t)pos;
+    w[3] = (uint32_t)nonimp;
+    w[4] = (uint32_t)last_tried;
+    if (hat) {
+        std::ostringstream os;
+        os << hat->gen;
+        std::istringstream is(os.str());
+        std::vector<uint32_t> gw;
+        unsigned long long x;
+        while (is >> x) gw.push_back((uint32_t)x);
+        if (kDrvHead + gw.size() + (size_t)hat->total > (size_t)PNGF_DRIVER_STATE_WORDS)
+            return fail(h, PNGF_ERR_ARG, "driver position does not fit PNGF_DRIVER_STATE_WORDS");
+        w[5] = (uint32_t)hat->total;
+        w[6] = (uint32_t)hat->left;
+        w[7] = (uint32_t)gw.size();
+        for (size_t i = 0; i < gw.size(); i++) w[kDrvHead + i] = gw[i];
+        for (int i = 0; i < hat->total; i++) w[kDrvHead + gw.size() + i] = (uint32_t)hat->idx[i];
+    }
+    return PNGF_OK;
+}
+
+// Reads a saved position (resume = 1); the checks keep every index in range.
+int drv_load(pngf_handle* h, const pngf_dbs_params* p, int ncl, int* pos, int64_t* nonimp, Hat* hat,
+             int* last_tried) {
+    const uint32_t* w = p->driver_state;
+    if (!w || w[0] != kDrvMagic || (int)w[1] != h->cfg.driver)
+        return fail(h, PNGF_ERR_ARG, "resume: no driver position of this driver in driver_state");
+    if (h->cfg.driver == PNGF_DRIVER_RASTER) {
+        if ((int)w[2] < 0 || (int)w[2] >= std::max(ncl, 1) || (int64_t)w[3] > ncl)
+            return fail(h, PNGF_ERR_ARG, "resume: raster position out of range");
+        *pos = (int)w[2];
+        *nonimp = (int64_t)w[3];
+        return PNGF_OK;
+    }
+    const int total = (int)w[5], left = (int)w[6], nw = (int)w[7];
+    if (total != hat->total || left < 0 || left > total || nw < 1 ||
+        kDrvHead + (size_t)nw + (size_t)total > (size_t)PNGF_DRIVER_STATE_WORDS)
+        return fail(h, PNGF_ERR_ARG, "resume: hat state does not match this problem");
+    std::ostringstream os;
+    for (int i = 0; i < nw; i++) os << w[kDrvHead + i] << ' ';
+    std::istringstream is(os.str());
+    is >> hat->gen;
+    if (is.fail()) return fail(h, PNGF_ERR_ARG, "resume: unreadable generator state");
+    for (int i = 0; i < total; i++) {
+        const int v = (int)w[kDrvHead + nw + i];
+        if (v < 0 || v >= total) return fail(h, PNGF_ERR_ARG, "resume: hat index out of range");
+        hat->idx[i] = v;
+    }
+    hat->left = left;
+    const int lt = (int)w[4];
+    if (lt < -1 || lt >= total) return fail(h, PNGF_ERR_ARG, "resume: last tile tried out of range");
+    *last_tried = lt;
+    return PNGF_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -929,6 +1015,7 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     CK(dalloc_t(h, &h->d_sel, 4));
     CK(dalloc_t(h, &h->d_info, 64));
     CK(dalloc_t(h, &h->d_pos, 3 * (size_t)N));
+    CK(dalloc_t(h, &h->d_flag, 4));
 
     CK(dalloc_t(h, &h->d_Fbest, 4));
     CK(dalloc_t(h, &h->d_Fsel, 4));
@@ -1005,34 +1092,61 @@ int pngf_load_g(pngf_handle* h, int32_t f, const char* path) {
     if ((size_t)sb.st_size != M * M * sizeof(zd))
         return fail(h, PNGF_ERR_SIZE, std::string("G file ") + path + " has " + std::to_string(sb.st_size) +
                                           " bytes, expected " + std::to_string(M * M * sizeof(zd)));
-    FILE* fp = fopen(path, "rb");
-    if (!fp) return fail(h, PNGF_ERR_IO, std::string("cannot open ") + path);
     CK(alloc_gcm(h, f));
-    h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
+    // Two pinned staging buffers of whole file columns (~64 MB each): the read of one chunk
+    // overlaps the upload of the previous one (hipMemcpy2DAsync on the handle's stream, which
+    // also drops the Gobj rows N.. of each column and pads the columns to ldN).
     const size_t cols = std::max<size_t>(1, (64u << 20) / (M * sizeof(zd)));
-    zd* stage = nullptr;
-    if (hipHostMalloc((void**)&stage, cols * M * sizeof(zd), hipHostMallocDefault) != hipSuccess) {
-        fclose(fp);
-        return fail(h, PNGF_ERR_NOMEM, "pinned staging buffer");
-    }
+    zd* stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    FILE* fp = nullptr;
     int rc = PNGF_OK;
-    for (size_t c0 = 0; c0 < M && rc == PNGF_OK; c0 += cols) {
+    for (int i = 0; i < 2 && rc == PNGF_OK; i++) {
+        if (hipHostMalloc((void**)&stage[i], cols * M * sizeof(zd), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            stage[i] = nullptr;
+            rc = fail(h, PNGF_ERR_NOMEM, "pinned staging buffer");
+        } else if (hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            done[i] = nullptr;
+            rc = fail(h, PNGF_ERR_HIP, "staging event");
+        }
+    }
+    if (rc == PNGF_OK) {
+        fp = fopen(path, "rb");
+        if (!fp) rc = fail(h, PNGF_ERR_IO, std::string("cannot open ") + path);
+    }
+    if (rc == PNGF_OK) h->gobj_extra[f].assign((size_t)(ne - 1) * N, zmk(0.0, 0.0));
+    int b = 0;
+    for (size_t c0 = 0, chunk = 0; c0 < M && rc == PNGF_OK; c0 += cols, chunk++, b ^= 1) {
         const size_t nc = std::min(cols, M - c0);
-        if (fread(stage, sizeof(zd), nc * M, fp) != nc * M) { rc = fail(h, PNGF_ERR_IO, "short read of G file"); break; }
+        // buffer b was last handed to the upload two chunks ago
+        if (chunk >= 2) {
+            hipError_t e = hipEventSynchronize(done[b]);
+            if (e != hipSuccess) { rc = fail(h, PNGF_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e)); break; }
+        }
+        if (fread(stage[b], sizeof(zd), nc * M, fp) != nc * M) { rc = fail(h, PNGF_ERR_IO, "short read of G file"); break; }
         for (size_t c = 0; c < nc; c++) {
             const size_t col = c0 + c;
             if (col >= (size_t)N) continue;
-            for (int r = 0; r < ne - 1; r++) h->gobj_extra[f][(size_t)r * N + col] = stage[c * M + N + r];
+            for (int r = 0; r < ne - 1; r++) h->gobj_extra[f][(size_t)r * N + col] = stage[b][c * M + N + r];
         }
         const size_t ncol = std::min(nc, (size_t)N > c0 ? (size_t)N - c0 : 0);
         if (ncol) {
-            hipError_t e = hipMemcpy2D(h->Gcm[f] + c0 * h->d.ldN, h->d.ldN * sizeof(zd), stage, M * sizeof(zd),
-                                       N * sizeof(zd), ncol, hipMemcpyHostToDevice);
-            if (e != hipSuccess) rc = fail(h, PNGF_ERR_HIP, std::string("hipMemcpy2D: ") + hipGetErrorString(e));
+            hipError_t e = hipMemcpy2DAsync(h->Gcm[f] + c0 * h->d.ldN, h->d.ldN * sizeof(zd), stage[b],
+                                            M * sizeof(zd), N * sizeof(zd), ncol, hipMemcpyHostToDevice, h->st);
+            if (e == hipSuccess) e = hipEventRecord(done[b], h->st);
+            if (e != hipSuccess) rc = fail(h, PNGF_ERR_HIP, std::string("hipMemcpy2DAsync: ") + hipGetErrorString(e));
         }
     }
-    fclose(fp);
-    (void)hipHostFree(stage);
+    // the uploads must finish before the staging buffers go
+    hipError_t es = h->st ? hipStreamSynchronize(h->st) : hipSuccess;
+    if (es != hipSuccess && rc == PNGF_OK) rc = fail(h, PNGF_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(es));
+    if (fp) fclose(fp);
+    for (int i = 0; i < 2; i++) {
+        if (stage[i]) (void)hipHostFree(stage[i]);
+        if (done[i]) (void)hipEventDestroy(done[i]);
+    }
     if (rc == PNGF_OK) h->g_loaded[f] = 1;
     return rc;
 }
@@ -1256,6 +1370,9 @@ int pngf_init_state(pngf_handle* h) {
     }
     std::vector<zd> e;
     CK(download_eobj(h, e));
+    for (const zd& v : e)
+        if (!std::isfinite(v.x) || !std::isfinite(v.y))
+            return fail(h, PNGF_ERR_NONFINITE, "initial Eobj is not finite (C^-1 overflowed)");
     h->inited = true;
     return PNGF_OK;
 }
@@ -1537,7 +1654,8 @@ bool device_loop_ok(const pngf_handle* h, const pngf_dbs_params* p) {
            h->cfg.method == PNGF_METHOD_LOWRANK && !p->log_x && !hist_on(p) && h->d.gmax <= kLdsGmax;
 }
 
-int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best, int bmin, int bmax) {
+int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best, int bmin, int bmax, int pos = 0,
+                        int nonimp = 0) {
     if (!h->d_loop) {
         CK(dalloc_t(h, &h->d_loop, 1));
         HC(h, hipHostMalloc((void**)&h->h_loop, sizeof(LoopState), hipHostMallocDefault));
@@ -1582,6 +1700,8 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
     for (int g = 0; g < h->ngroups_drv; g++) ncl += h->grp.ntiles[g] > 0 ? 1 : 0;
     ls.ncl = ncl;
     ls.sel = -1;
+    ls.pos = pos;
+    ls.nonimp = nonimp;
     ls.dyn[kDynKK] = h->K;
     HC(h, hipMemcpyAsync(h->d_loop, &ls, sizeof(LoopState), hipMemcpyHostToDevice, h->st));
     launch_loop_batch(h->st, h->d_loop, h->d_cl, h->d_batch);
@@ -1763,8 +1883,24 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     int64_t flip_count = 0, nacc = 0, spec = 0, batches = 0;
     const double acc0 = h->acc_seconds;
     h->flushes = 0;
+    if (p->resume && p->order != PNGF_ORDER_FIRST)
+        return fail(h, PNGF_ERR_ARG, "resume follows first-improvement order");
+    // candidate groups of the raster cycle, in visit order
+    std::vector<int> cl;
+    for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
+    const int ncl = (int)cl.size();
+    // where the visit order starts: its beginning, or a saved driver position
+    int pos0 = 0, last_tried0 = -1;
+    int64_t nonimp0 = 0;
+    Hat hat;
+    if (h->cfg.driver == PNGF_DRIVER_REF) {
+        hat.init(p->seed, h->ngroups_drv);
+        if (p->rng_discard > 0 && !p->resume) hat.gen.discard(p->rng_discard);
+    }
+    if (p->resume) CK(drv_load(h, p, ncl, &pos0, &nonimp0, &hat, &last_tried0));
     const bool dev_loop = device_loop_ok(h, p);
-    if (dev_loop) CK(device_loop_prepare(h, p, F_best, bmin, (int)std::min<int64_t>(bmax_cfg, 1 << 30)));
+    if (dev_loop)
+        CK(device_loop_prepare(h, p, F_best, bmin, (int)std::min<int64_t>(bmax_cfg, 1 << 30), pos0, (int)nonimp0));
     HC(h, hipStreamSynchronize(h->st));
     auto t0 = std::chrono::high_resolution_clock::now();
     auto record = [&](int g, double F) -> int {
@@ -1779,9 +1915,6 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
     int bs = bmin;
     if (h->cfg.driver == PNGF_DRIVER_RASTER && p->order == PNGF_ORDER_BEST) {
         // steepest descent: every step scores all candidates exactly and commits the best
-        std::vector<int> cl;
-        for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
-        const int ncl = (int)cl.size();
         std::vector<double> F(ncl);
         while (true) {
             if (p->max_evals > 0 && flip_count + ncl > p->max_evals) break;
@@ -1809,12 +1942,10 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
         int conv = 0;
         rc = device_loop_run(h, p, &flip_count, &nacc, &spec, &batches, &F_best, &conv);
         r->converged = conv;
+        if (rc == PNGF_OK) rc = drv_save(h, p, h->h_loop->pos, h->h_loop->nonimp, nullptr, -1);
     } else if (h->cfg.driver == PNGF_DRIVER_RASTER) {
-        std::vector<int> cl;
-        for (int g = 0; g < h->ngroups_drv; g++) if (h->grp.ntiles[g] > 0) cl.push_back(g);
-        const int ncl = (int)cl.size();
-        int pos = 0;
-        int64_t nonimp = 0;
+        int pos = pos0;
+        int64_t nonimp = nonimp0;
         while (nonimp < ncl) {
             if (p->max_evals > 0 && flip_count >= p->max_evals) break;
             int64_t B = ncl - nonimp;
@@ -1849,12 +1980,9 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
             }
         }
         r->converged = nonimp >= ncl;
+        if (rc == PNGF_OK) rc = drv_save(h, p, pos, nonimp, nullptr, -1);
     } else {
-        Hat hat;
-        const int nhat = h->ngroups_drv;
-        hat.init(p->seed, nhat);
-        if (p->rng_discard > 0) hat.gen.discard(p->rng_discard);
-        int last_tried = -1;
+        int last_tried = last_tried0;
         bool done = false;
         while (!done) {
             if (p->max_evals > 0 && flip_count >= p->max_evals) break;
@@ -1907,6 +2035,7 @@ int pngf_dbs_run(pngf_handle* h, const pngf_dbs_params* p, pngf_dbs_result* r) {
                 bs = std::min<int64_t>((int64_t)bs * 2, (int64_t)1 << 30);
             }
         }
+        if (rc == PNGF_OK) rc = drv_save(h, p, 0, 0, &hat, last_tried);
     }
     HC(h, hipStreamSynchronize(h->st));
     prof_collect(h);
