@@ -277,6 +277,14 @@ int pngf_mlp_num_params(const pngf_mlp* m);
 int pngf_mlp_get_params(pngf_mlp* m, float* out);
 int pngf_mlp_set_params(pngf_mlp* m, const float* in);
 int pngf_mlp_predict(pngf_mlp* m, const float* X, int32_t n, float* out);
+/* The forward pass of pngf_mlp_predict for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+   reference.  X_dev [n][din] and out_dev [n][dout] are device pointers on the network's device;
+   the work is enqueued on `stream` (a hipStream_t; null: the network's own stream) and the call
+   returns without waiting.  It allocates nothing: the activations live in a fixed per-device
+   scratch array of 2^20 floats, so 2 n dh must not exceed 1048576 (PNGF_ERR_ARG otherwise).
+   Calls that overlap on one GPU share that array and race. */
+int pngf_mlp_predict_dev(pngf_mlp* m, const float* X_dev, int32_t n, float* out_dev, void* stream);
 /* gradient of mean((f(X) - Y)^2) for one batch, no update */
 int pngf_mlp_grad(pngf_mlp* m, const float* X, const float* Y, int32_t n, float* grads, float* loss);
 int pngf_mlp_train(pngf_mlp* m, const float* X, const float* Y, int32_t n, int32_t epochs, int32_t batch,

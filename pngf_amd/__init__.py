@@ -38,7 +38,7 @@ EXPORTS = [
     "pngf_mlp_create", "pngf_mlp_destroy", "pngf_mlp_last_error", "pngf_mlp_num_params",
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many", "pngf_dbs_run_pool",
-    "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity", "pngf_sweep",
+    "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity", "pngf_sweep", "pngf_mlp_predict_dev",
 ]
 
 
@@ -135,6 +135,7 @@ def lib():
         "pngf_mlp_get_params": (I, [P, P]),
         "pngf_mlp_set_params": (I, [P, P]),
         "pngf_mlp_predict": (I, [P, P, I, P]),
+        "pngf_mlp_predict_dev": (I, [P, P, I, P, P]),
         "pngf_mlp_grad": (I, [P, P, P, I, P, P]),
         "pngf_mlp_train": (I, [P, P, P, I, I, I, ct.c_float, I, P]),
         "pngf_ranker_dims": (I, [P, ct.POINTER(I), ct.POINTER(I)]),

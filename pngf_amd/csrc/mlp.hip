@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -135,8 +136,17 @@ __global__ void adam_kernel(int n, float* w, const float* __restrict__ g, float*
 
 }  // namespace
 
+// Scratch of pngf_mlp_predict_dev (the entry point that allocates nothing): the two n x dh
+// activation arrays of one forward pass, at most kNoAllocFloats floats (4 MiB) in all.  Each GPU
+// holds its own copy for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the name itself is a host-side stand-in).
+constexpr size_t kNoAllocFloats = (size_t)1 << 20;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
 struct pngf_mlp {
     int din = 0, dh = 0, dout = 0, cap = 0;
+    int device = 0;
+    float* scratch_dev = nullptr;  // this device's `scratch`, from hipGetSymbolAddress
     int nparam = 0;
     float* w = nullptr;   // all parameters: W1 b1 W2 b2 W3 b3 (row-major W[in][out])
     float* g = nullptr;   // gradients, same layout
@@ -162,11 +172,16 @@ int mfail(pngf_mlp* p, int code, const std::string& m) {
         if (e_ != hipSuccess) return mfail(p, PNGF_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+void gemm_on(hipStream_t st, int M, int N, int K, const float* A, int sai, int sak, const float* B, int sbk,
+             int sbj, float* C, int ldc, const float* bias, int acc) {
+    const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+    hipLaunchKernelGGL(gemm_bf16_kernel, dim3((tiles + 3) / 4), dim3(256), 0, st, M, N, K, A, sai, sak, B, sbk, sbj,
+                       C, ldc, bias, acc);
+}
+
 void gemm(pngf_mlp* p, int M, int N, int K, const float* A, int sai, int sak, const float* B, int sbk, int sbj,
           float* C, int ldc, const float* bias, int acc) {
-    const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
-    hipLaunchKernelGGL(gemm_bf16_kernel, dim3((tiles + 3) / 4), dim3(256), 0, p->st, M, N, K, A, sai, sak, B, sbk, sbj,
-                       C, ldc, bias, acc);
+    gemm_on(p->st, M, N, K, A, sai, sak, B, sbk, sbj, C, ldc, bias, acc);
 }
 
 int grow(pngf_mlp* p, int n) {
@@ -189,14 +204,34 @@ int grow(pngf_mlp* p, int n) {
     return PNGF_OK;
 }
 
-void forward(pngf_mlp* p, int n) {
-    float* w = p->w;
+// Forward pass of n rows X -> O on stream st through the given activation arrays (every one is
+// written before it is read: the products overwrite their outputs).
+void forward_into(pngf_mlp* p, int n, const float* X, float* Z1, float* H1, float* Z2, float* H2, float* O,
+                  hipStream_t st) {
+    const float* w = p->w;
     const int e = 256;
-    gemm(p, n, p->dh, p->din, p->X, p->din, 1, w + p->off_W1, p->dh, 1, p->Z1, p->dh, w + p->off_b1, 0);
-    hipLaunchKernelGGL(gelu_kernel, dim3((n * p->dh + e - 1) / e), dim3(e), 0, p->st, n * p->dh, p->Z1, p->H1);
-    gemm(p, n, p->dh, p->dh, p->H1, p->dh, 1, w + p->off_W2, p->dh, 1, p->Z2, p->dh, w + p->off_b2, 0);
-    hipLaunchKernelGGL(gelu_kernel, dim3((n * p->dh + e - 1) / e), dim3(e), 0, p->st, n * p->dh, p->Z2, p->H2);
-    gemm(p, n, p->dout, p->dh, p->H2, p->dh, 1, w + p->off_W3, p->dout, 1, p->O, p->dout, w + p->off_b3, 0);
+    gemm_on(st, n, p->dh, p->din, X, p->din, 1, w + p->off_W1, p->dh, 1, Z1, p->dh, w + p->off_b1, 0);
+    hipLaunchKernelGGL(gelu_kernel, dim3((n * p->dh + e - 1) / e), dim3(e), 0, st, n * p->dh, Z1, H1);
+    gemm_on(st, n, p->dh, p->dh, H1, p->dh, 1, w + p->off_W2, p->dh, 1, Z2, p->dh, w + p->off_b2, 0);
+    hipLaunchKernelGGL(gelu_kernel, dim3((n * p->dh + e - 1) / e), dim3(e), 0, st, n * p->dh, Z2, H2);
+    gemm_on(st, n, p->dout, p->dh, H2, p->dh, 1, w + p->off_W3, p->dout, 1, O, p->dout, w + p->off_b3, 0);
+}
+
+void forward(pngf_mlp* p, int n) { forward_into(p, n, p->X, p->Z1, p->H1, p->Z2, p->H2, p->O, p->st); }
+
+// This device's address of `scratch`, looked up once per device (with that device current).
+int scratch_of_device(pngf_mlp* p, int dev) {
+    static std::mutex mu;
+    static float* addr[64] = {};
+    if (dev < 0 || dev >= 64) return mfail(p, PNGF_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* a = nullptr;
+        MHC(p, hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        addr[dev] = (float*)a;
+    }
+    p->scratch_dev = addr[dev];
+    return PNGF_OK;
 }
 
 // Gradients of the mean squared error over the n rows into p->g.
@@ -240,6 +275,9 @@ int pngf_mlp_create(int32_t din, int32_t dh, int32_t dout, int32_t seed, pngf_ml
     p->off_W3 = o; o += (size_t)dh * dout;
     p->off_b3 = o; o += dout;
     p->nparam = (int)o;
+    MHC(p, hipGetDevice(&p->device));
+    int rc = scratch_of_device(p, p->device);
+    if (rc) return rc;
     MHC(p, hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     MHC(p, hipMalloc(&p->w, sizeof(float) * o));
     MHC(p, hipMalloc(&p->g, sizeof(float) * o));
@@ -296,6 +334,28 @@ int pngf_mlp_predict(pngf_mlp* p, const float* X, int32_t n, float* out) {
     MHC(p, hipGetLastError());
     MHC(p, hipMemcpyAsync(out, p->O, sizeof(float) * n * p->dout, hipMemcpyDeviceToHost, p->st));
     MHC(p, hipStreamSynchronize(p->st));
+    return PNGF_OK;
+}
+
+// Forward pass of n rows of device-resident inputs into device-resident outputs, enqueued on
+// `stream` (null: the network's own stream) without waiting and without allocating: the
+// activations live in this device's `scratch` (2 n dh floats, at most kNoAllocFloats).  Calls
+// that overlap on one GPU share that scratch and race; nothing in it needs clearing between calls.
+int pngf_mlp_predict_dev(pngf_mlp* p, const float* X_dev, int32_t n, float* out_dev, void* stream) {
+    if (!p || n < 0 || (n > 0 && (!X_dev || !out_dev))) return mfail(p, PNGF_ERR_ARG, "bad predict_dev arguments");
+    if (n == 0) return PNGF_OK;
+    const size_t need = 2 * (size_t)n * p->dh;
+    if (need > kNoAllocFloats)
+        return mfail(p, PNGF_ERR_ARG, "predict_dev: 2 n dh = " + std::to_string(need) + " floats exceed the " +
+                                          std::to_string(kNoAllocFloats) + " of its scratch");
+    int dev = -1;
+    MHC(p, hipGetDevice(&dev));
+    if (dev != p->device) return mfail(p, PNGF_ERR_ARG, "predict_dev: the network lives on another device");
+    hipStream_t st = stream ? (hipStream_t)stream : p->st;
+    float* A = p->scratch_dev;
+    float* B = p->scratch_dev + (size_t)n * p->dh;
+    forward_into(p, n, X_dev, A, B, A, B, out_dev, st);
+    MHC(p, hipGetLastError());
     return PNGF_OK;
 }
 
