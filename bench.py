@@ -41,13 +41,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0):
+def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0, perturb=0.0):
     T, freqs = wl["T"], wl["freqs"]
     opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred,
                       symmetry=symmetry, tile_variant=tile_variant)
     for f, hz in enumerate(freqs):
         opt.set_frequency(f, hz)
         opt.synth_g(f)
+        if perturb > 0.0:
+            opt.perturb_g(f, perturb, seed=f + 1)
     opt.set_tile_map(initial_pattern(T, 0))
     mid = (T - 1) // 2
     opt.set_source_port(mid, mid, 1, 2)
@@ -90,6 +92,57 @@ def roofline(opt, gmax=24):
         rec["traffic_source"] = "model 1 + 7k/N fitted to rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE"
     allk = {P.K_NAMES[i]: opt.kernel_stats(i) for i in range(6)}
     return rec, allk
+
+
+def strip_stats(opt):
+    """Event-timed column and row strips of a profiled run: launches, average us and the
+    fraction of FP64 peak at their algorithmic flops."""
+    out = {}
+    for k in (P.K_COLBLK, P.K_ROWBLK):
+        s = opt.kernel_stats(k)
+        if s["launches"]:
+            avg = s["total_ms"] / s["launches"]
+            tf = s["flops"] / (s["total_ms"] * 1e-3) / 1e12
+            out[P.K_NAMES[k]] = dict(launches=s["launches"], avg_us=avg * 1e3, tflops=tf,
+                                     frac=tf / PEAK_FP64_TFLOPS)
+    return out
+
+
+def variant(wl, name, symmetry, perturb, headline):
+    """The LOWRANK run from the same start on another accept path: one untimed warmup, one run
+    for the rate, one profiled run for the strips' event times."""
+    opt, init_s = make(wl, P.METHOD_LOWRANK, symmetry=symmetry, perturb=perturb)
+    try:
+        used, asym = opt.symmetry()
+        opt.checkpoint()
+        opt.dbs_run(max_evals=wl["max_evals"])
+        opt.restore()
+        r = opt.dbs_run(max_evals=wl["max_evals"])
+        opt.restore()
+        opt.profile(True)
+        opt.dbs_run(max_evals=wl["max_evals"])
+        strips = strip_stats(opt)
+        opt.profile(False)
+    finally:
+        opt.close()
+    out = dict(value=r["flip_count"] / r["seconds"], seconds=r["seconds"], flip_count=r["flip_count"],
+               accepts=r["accepts"], F_final=r["F_final"], symmetric_accept=used, rel_asym=asym,
+               init_s=init_s, strips=strips,
+               same_accept_sequence=bool(np.array_equal(r["acc_tiles"], headline["acc_tiles"])))
+    log("[bench] %s: %d evals %d accepts %.3f s (%.0f flips/s), strips %s" % (
+        name, r["flip_count"], r["accepts"], r["seconds"], out["value"],
+        {k: "%.1f us %.3f" % (v["avg_us"], v["frac"]) for k, v in strips.items()}))
+    return out
+
+
+def plain_ceiling_s(wl, run):
+    """Seconds the PLAIN path needs at 8 TB/s for a run's committed work: 16 N (2k + 1 + ne) B per
+    evaluation and frequency (k rows of G, k columns of C^-1, S, Gobj) and 48 N^2 B per accept and
+    frequency (V C^-1 reads C^-1, the rank-k update reads and writes it)."""
+    N = P.layout_size(wl["layout"], wl["T"])
+    nf, k = len(wl["freqs"]), 24
+    b = nf * (run["flip_count"] * 16.0 * N * (2 * k + 2) + run["accepts"] * 48.0 * N * N)
+    return b / (PEAK_HBM_GBS * 1e9)
 
 
 def cpu_baseline(wl, g0, cinv0, budget_s, gpu_run):
@@ -196,6 +249,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the two-strip and non-reciprocal-G runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
@@ -288,6 +343,13 @@ def main():
     opt.close()
     del opt
 
+    # the accept path a real (non-reciprocal) G takes: both strips, forced on the symmetric G and
+    # taken by itself on a G perturbed at the single-precision level
+    variants = None
+    if not args.no_variants and method == P.METHOD_LOWRANK:
+        variants = {"symmetry_off": variant(wl, "symmetry off", P.SYM_OFF, 0.0, last),
+                    "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last)}
+
     plain = None
     g0 = cinv0 = None
     if not args.no_plain or not args.no_cpu:
@@ -296,13 +358,25 @@ def main():
             g0 = popt.g(0)
             cinv0 = popt.cinv(0)
         if not args.no_plain:
+            popt.checkpoint()
             pr = popt.dbs_run(max_evals=wl["max_evals"])
+            # a second run with event timing for the kernel breakdown (its accepts synchronise)
+            popt.restore()
+            popt.profile(True)
+            popt.dbs_run(max_evals=wl["max_evals"])
+            pk = {P.K_NAMES[i]: popt.kernel_stats(i) for i in range(6)}
+            popt.profile(False)
+            ceil_s = plain_ceiling_s(wl, pr)
             plain = dict(value=pr["flip_count"] / pr["seconds"], seconds=pr["seconds"],
                          flip_count=pr["flip_count"], accepts=pr["accepts"], F_final=pr["F_final"],
-                         init_s=pinit,
-                         same_accept_sequence=bool(np.array_equal(pr["acc_tiles"], last["acc_tiles"])))
-            log("[bench] plain: %d evals %d accepts %.3f s F %.6g" % (pr["flip_count"], pr["accepts"],
-                                                                     pr["seconds"], pr["F_final"]))
+                         speculative_evals=pr["speculative_evals"], init_s=pinit,
+                         same_accept_sequence=bool(np.array_equal(pr["acc_tiles"], last["acc_tiles"])),
+                         hbm_ceiling_s=ceil_s, hbm_ceiling_value=pr["flip_count"] / ceil_s,
+                         kernels={k: dict(v, avg_ms=v["total_ms"] / v["launches"] if v["launches"] else 0.0,
+                                          GBps=v["bytes"] / (v["total_ms"] * 1e-3) / 1e9 if v["total_ms"] else 0.0)
+                                  for k, v in pk.items() if v["launches"]})
+            log("[bench] plain: %d evals %d accepts %.3f s F %.6g (HBM ceiling %.3f s)" % (
+                pr["flip_count"], pr["accepts"], pr["seconds"], pr["F_final"], ceil_s))
         popt.close()
         del popt
     cpu = None
@@ -341,6 +415,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "speedup_vs_plain_ceiling": (value / plain["hbm_ceiling_value"]) if plain else None,
+        "accept_path_variants": variants,
         "F_final_vs_plain": (last["F_final"] / plain["F_final"]) if plain else None,
         "best_improvement_variant": best,
         "sweep": sweep,

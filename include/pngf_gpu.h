@@ -128,6 +128,13 @@ int pngf_set_g(pngf_handle* h, int32_t f, const pngf_c128* file_buf);
    medium sampled at the component positions, softened self term), generated on the
    device for slot f at its frequency.  See pngf_synth_g_host for the definition. */
 int pngf_synth_g(pngf_handle* h, int32_t f);
+/* Adds a deterministic non-reciprocal perturbation to the G of slot f (before pngf_init_state):
+   G(i, j) += rel * max|G| * (u1 + i u2), with u1, u2 uniform in [-1, 1) drawn from a hash of
+   (seed, i, j) (|z| = |Re z| + |Im z|; u = (mix64(seed * 2^40 + 2 (i N + j) + c) >> 11) * 2^-52 - 1,
+   c = 0, 1, mix64 the splitmix64 finaliser).  A stand-in for acmefdfd's G, whose single-precision
+   MUMPS Schur complement (acmefdfd/apf_solver.hpp:188-198) is reciprocal only to rounding.  No
+   counterpart in the reference. */
+int pngf_perturb_g(pngf_handle* h, int32_t f, double rel, int32_t seed);
 /* Host version of the synthetic generator, written in the acmefdfd file layout
    (M x M column-major).  The device version evaluates the same formula. */
 int pngf_synth_g_host(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x,

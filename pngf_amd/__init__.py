@@ -39,6 +39,7 @@ EXPORTS = [
     "pngf_mlp_get_params", "pngf_mlp_set_params", "pngf_mlp_predict", "pngf_mlp_grad", "pngf_mlp_train",
     "pngf_ranker_dims", "pngf_features", "pngf_dbs_run_ranked", "pngf_dbs_run_many", "pngf_dbs_run_pool",
     "pngf_autotune_result", "pngf_symmetry", "pngf_factor_capacity", "pngf_sweep", "pngf_mlp_predict_dev",
+    "pngf_perturb_g",
 ]
 
 
@@ -113,6 +114,7 @@ def lib():
         "pngf_load_g": (I, [P, I, ct.c_char_p]),
         "pngf_set_g": (I, [P, I, P]),
         "pngf_synth_g": (I, [P, I]),
+        "pngf_perturb_g": (I, [P, I, D, I]),
         "pngf_synth_g_host": (I, [I, I, I, I, I, I, D, D, P]),
         "pngf_set_tile_map": (I, [P, P]),
         "pngf_set_source_port": (I, [P, I, I, I, I]),
@@ -178,6 +180,28 @@ def synth_g_host(layout, tiles, num_eobj, hz, cells=3, soft=0.0):
     if rc != 0:
         raise PNGFError(rc, "pngf_synth_g_host")
     return buf
+
+
+def _mix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def perturb_block(G, N, rel, seed=1):
+    """G (file layout, at least N x N) with its N x N block perturbed as pngf_perturb_g does."""
+    G = np.array(G, dtype=np.complex128, order="F")
+    blk = G[:N, :N]
+    amax = np.max(np.abs(blk.real) + np.abs(blk.imag))
+    i, j = np.meshgrid(np.arange(N, dtype=np.uint64), np.arange(N, dtype=np.uint64), indexing="ij")
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) << np.uint64(40)) + np.uint64(2) * (i * np.uint64(N) + j)
+        u1 = (_mix64(key) >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+        u2 = (_mix64(key + np.uint64(1)) >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+    amp = rel * amax
+    G[:N, :N] = (blk.real + amp * u1) + 1j * (blk.imag + amp * u2)
+    return G
 
 
 def write_history(path, seed, start_map, r, end_map):
@@ -274,6 +298,10 @@ class Optimizer:
 
     def synth_g(self, f):
         self._ck(lib().pngf_synth_g(self.h, f), "synth_g")
+
+    def perturb_g(self, f, rel, seed=1):
+        """Non-reciprocal perturbation of slot f's G: rel * max|G| * (u1 + i u2); see perturb_block."""
+        self._ck(lib().pngf_perturb_g(self.h, f, float(rel), int(seed)), "perturb_g")
 
     def set_tile_map(self, tm):
         a = np.ascontiguousarray(tm, dtype=np.int32)

@@ -164,6 +164,8 @@ void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* G
 void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v);
 void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out);
 void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out);
+// G(i, j) += amp (u1 + i u2), uniform u from a hash of (seed, i, j): a non-reciprocal perturbation
+void launch_perturb(hipStream_t st, int N, int ldN, zd* G, double amp, unsigned long long seed);
 // *flag |= 1 when the column-major N x N matrix holds a NaN or an infinity (flag zeroed by the caller)
 void launch_nonfinite(hipStream_t st, int N, int ldN, const zd* G, int32_t* flag);
 void launch_symmetrize(hipStream_t st, int N, int ldN, zd* W);

@@ -1994,6 +1994,37 @@ void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long
     hipLaunchKernelGGL(asym_kernel, grid, dim3(32, 8), 0, st, N, ldN, G, out);
 }
 
+// 64-bit mixer (splitmix64 finaliser) of the perturbation below.
+__host__ __device__ inline unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// G(i, j) += amp (u1 + i u2), u1, u2 uniform in [-1, 1) from the top 53 bits of
+// mix64(seed * 2^40 + (i N + j) * 2 + {0, 1}): a deterministic perturbation with G(i, j) and
+// G(j, i) drawn independently, i.e. a non-reciprocal one.  Column-major, ld ldN.
+__global__ void __launch_bounds__(256) perturb_kernel(int N, int ldN, zd* G, double amp, unsigned long long seed) {
+    // product and sum rounded separately, as on the host (the default contraction would fuse them)
+#pragma clang fp contract(off)
+    for (int col = blockIdx.y; col < N; col += gridDim.y)
+        for (int r = blockIdx.x * 256 + threadIdx.x; r < N; r += gridDim.x * 256) {
+            const unsigned long long key = (seed << 40) + 2ull * ((unsigned long long)r * N + col);
+            const double u1 = (double)(mix64(key) >> 11) * 0x1.0p-52 - 1.0;
+            const double u2 = (double)(mix64(key + 1) >> 11) * 0x1.0p-52 - 1.0;
+            zd& g = G[(size_t)col * ldN + r];
+            g.x = g.x + amp * u1;
+            g.y = g.y + amp * u2;
+        }
+}
+
+void launch_perturb(hipStream_t st, int N, int ldN, zd* G, double amp, unsigned long long seed) {
+    const int gx = (N + 255) / 256 < 8 ? (N + 255) / 256 : 8;
+    dim3 grid(gx, N < 4096 ? N : 4096);
+    hipLaunchKernelGGL(perturb_kernel, grid, dim3(256), 0, st, N, ldN, G, amp, seed);
+}
+
 // *flag |= 1 if any entry of the column-major N x N matrix is NaN or infinite (rows >= N of
 // each column are padding and are not read).  grid.y strides over the columns.
 __global__ void __launch_bounds__(256) nonfinite_kernel(int N, int ldN, const zd* __restrict__ G, int32_t* flag) {

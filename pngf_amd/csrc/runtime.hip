@@ -152,6 +152,7 @@ struct pngf_handle {
     // device-driven loop with profiling on: events around each iteration's column strip and the
     // iteration's (pending, k, K) record, read after every chunk
     hipEvent_t lev[2 * kLoopChunkMax] = {};
+    hipEvent_t levr[2 * kLoopChunkMax] = {};  // the same around the row strip (non-symmetric W)
     int32_t* d_iterlog = nullptr;
     int32_t* h_iterlog = nullptr;
     int nleft = 0;
@@ -1048,6 +1049,7 @@ int pngf_destroy(pngf_handle* h) {
     if (h->h_loop) (void)hipHostFree(h->h_loop);
     if (h->h_iterlog) (void)hipHostFree(h->h_iterlog);
     for (auto e : h->lev) if (e) (void)hipEventDestroy(e);
+    for (auto e : h->levr) if (e) (void)hipEventDestroy(e);
     if (h->loop_exec) (void)hipGraphExecDestroy(h->loop_exec);
     if (h->loop_graph) (void)hipGraphDestroy(h->loop_graph);
     if (h->rb) rocblas_destroy_handle(h->rb);
@@ -1169,6 +1171,28 @@ int pngf_synth_g(pngf_handle* h, int32_t f) {
             h->gobj_extra[f][(size_t)r * N + n] = zmk((r == 0 ? (o == 0) : (o == 1)) ? b : 0.0, 0.0);
         }
     h->g_loaded[f] = 1;
+    return PNGF_OK;
+}
+
+int pngf_perturb_g(pngf_handle* h, int32_t f, double rel, int32_t seed) {
+    if (!h || f < 0 || f >= h->d.nf || !(rel >= 0.0) || seed < 0) return fail(h, PNGF_ERR_ARG, "bad perturbation");
+    if (h->inited) return fail(h, PNGF_ERR_STATE, "G cannot change after pngf_init_state");
+    if (!h->g_loaded[f]) return fail(h, PNGF_ERR_STATE, "G of this slot is not set");
+    const int N = h->d.N;
+    // amp = rel * max |G_ij| (|z| = |Re z| + |Im z|), the largest entry from the asymmetry kernel
+    unsigned long long* m = nullptr;
+    CK(dalloc_t(h, &m, 2));
+    HC(h, hipMemsetAsync(m, 0, 2 * sizeof(unsigned long long), h->st));
+    launch_asym(h->st, N, h->d.ldN, h->Gcm[f], m);
+    unsigned long long hm[2];
+    HC(h, hipMemcpyAsync(hm, m, sizeof(hm), hipMemcpyDeviceToHost, h->st));
+    HC(h, hipStreamSynchronize(h->st));
+    dfree(h, m);
+    double amax;
+    std::memcpy(&amax, &hm[1], sizeof(double));
+    launch_perturb(h->st, N, h->d.ldN, h->Gcm[f], rel * amax, (unsigned long long)seed);
+    HC(h, hipGetLastError());
+    HC(h, hipStreamSynchronize(h->st));
     return PNGF_OK;
 }
 
@@ -1686,6 +1710,7 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
         CK(dalloc_t(h, &h->d_iterlog, 3 * kLoopChunk));
         HC(h, hipHostMalloc((void**)&h->h_iterlog, 3 * kLoopChunk * sizeof(int32_t), hipHostMallocDefault));
         for (auto& e : h->lev) HC(h, hipEventCreate(&e));
+        for (auto& e : h->levr) HC(h, hipEventCreate(&e));
     }
     LoopState& ls = *h->h_loop;
     std::memset(&ls, 0, sizeof(ls));
@@ -1748,9 +1773,12 @@ void device_loop_iteration(pngf_handle* h, int i) {
     if (tm) (void)hipEventRecord(h->lev[2 * i], st);
     launch_lr_colblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, (int)h->small_stride, h->skv, dyn);
     if (tm) (void)hipEventRecord(h->lev[2 * i + 1], st);
-    if (!h->sym)
+    if (!h->sym) {
+        if (tm) (void)hipEventRecord(h->levr[2 * i], st);
         launch_lr_rowblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv,
                          dyn);
+        if (tm) (void)hipEventRecord(h->levr[2 * i + 1], st);
+    }
     (void)hipStreamWaitEvent(st, h->ev_join, 0);
     FlipUpdate fu;
     fu.P = h->d_P;
@@ -1817,18 +1845,21 @@ int device_loop_run(pngf_handle* h, const pngf_dbs_params* p, int64_t* flip_coun
         HC(h, hipMemcpyAsync(&ls, h->d_loop, sizeof(LoopState), hipMemcpyDeviceToHost, st));
         HC(h, hipStreamSynchronize(st));
         if (tm) {
-            // event time of every column strip that ran (iterations with an accept)
+            // event time of every column strip (and row strip) that ran (iterations with an accept)
             const Dims& d = h->d;
-            auto& ks = h->kst[PNGF_K_COLBLK];
-            for (int i = 0; i < S; i++) {
-                const int32_t* r = h->h_iterlog + 3 * i;
-                float ms = 0.f;
-                if (!r[0] || hipEventElapsedTime(&ms, h->lev[2 * i], h->lev[2 * i + 1]) != hipSuccess) continue;
-                const double N = d.N, kd = r[1], Kd = r[2];
-                ks.launches++;
-                ks.total_ms += ms;
-                ks.bytes += d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd);
-                ks.flops += d.nf * 8.0 * N * Kd * kd;
+            for (int which = 0; which < (h->sym ? 1 : 2); which++) {
+                auto& ks = h->kst[which == 0 ? PNGF_K_COLBLK : PNGF_K_ROWBLK];
+                hipEvent_t* ev = which == 0 ? h->lev : h->levr;
+                for (int i = 0; i < S; i++) {
+                    const int32_t* r = h->h_iterlog + 3 * i;
+                    float ms = 0.f;
+                    if (!r[0] || hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) continue;
+                    const double N = d.N, kd = r[1], Kd = r[2];
+                    ks.launches++;
+                    ks.total_ms += ms;
+                    ks.bytes += d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd);
+                    ks.flops += d.nf * 8.0 * N * Kd * kd;
+                }
             }
         }
         Kknown = ls.dyn[kDynKK] + (ls.dyn[kDynActive] ? ls.dyn[kDynK] : 0);

@@ -386,6 +386,46 @@ def test_asymmetric_g_forms_both_strips():
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
+def test_device_perturbation_matches_host_and_oracle():
+    """pngf_perturb_g at 1e-7 of the largest entry (the rounding level of acmefdfd's single-
+    precision Schur complement): the device's G equals perturb_block's on the host bitwise, the
+    perturbed G is detected as non-reciprocal, the accept forms both strips, and the DBS run is
+    the oracle's on the host-perturbed G."""
+    T = 7
+    layout = P.LAYOUT_DISJOINT
+    freqs = freq_list(2, 29e9, 31e9)
+    N = P.layout_size(layout, T)
+    Gs = [P.synth_g_host(layout, T, 1, hz, soft=SOFT_WELL) for hz in freqs]
+    Gp = [P.perturb_block(G, N, 1e-7, seed=f + 1) for f, G in enumerate(Gs)]
+    tm = initial_pattern(T, 0)
+    plain = P.Optimizer(layout, T, num_freq=len(freqs), method=P.METHOD_PLAIN)
+    gpu = P.Optimizer(layout, T, num_freq=len(freqs), method=P.METHOD_LOWRANK)
+    for o in (plain, gpu):
+        for f, (hz, G) in enumerate(zip(freqs, Gs)):
+            o.set_frequency(f, hz)
+            o.set_g(f, G)
+            o.perturb_g(f, 1e-7, seed=f + 1)
+        o.set_tile_map(tm)
+        o.set_source_port(3, 3, 1, 2)
+        o.init_state()
+    for f in range(len(freqs)):
+        assert np.array_equal(plain.g(f), Gp[f][:N, :N])
+    plain.close()
+    used, asym = gpu.symmetry()
+    assert not used and 1e-8 < asym < 5e-7
+    ors = []
+    for G in Gp:
+        o = O.Optimizer(layout, T, G)
+        assert o.init_state(tm, (3, 3, 1, 2)) == 0
+        ors.append(o)
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    gpu.close()
+
+
 @pytest.mark.parametrize("asym,factors", [(False, P.FACTORS_AUTO), (False, P.FACTORS_REBUILT),
                                           (True, P.FACTORS_REBUILT)])
 def test_get_w_after_folds_and_pending_factors(asym, factors):
