@@ -14,7 +14,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_pngf.so")
-_lib = None
+# the complex64 build (the reference's USE_SINGLE_PRECISION, pngf-opt/main.cpp:58-72)
+_LIB_PATH_C64 = os.path.join(_HERE, "liboracle_pngf_c64.so")
+_libs = {}
 
 _P = ct.c_void_p
 _I = ct.c_int
@@ -22,16 +24,18 @@ _D = ct.c_double
 
 
 def build():
-    """Compile the oracle with its Makefile (g++ only; no GPU needed)."""
+    """Compile the oracle (complex128 and complex64 builds) with its Makefile (g++ only)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(c64=False):
+    """The complex128 oracle, or with c64=True its complex64 build (same source and C ABI, whose
+    arrays stay complex128: the build converts at the boundary)."""
+    if c64 not in _libs:
+        path = _LIB_PATH_C64 if c64 else _LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = ct.CDLL(_LIB_PATH)
+        L = ct.CDLL(path)
         sig = {
             "or_layout_size": (_I, [_I, _I, _I, _I]),
             "or_layout_mask": (None, [_I, _I, _I, _I, _P, _P]),
@@ -68,13 +72,14 @@ def lib():
             "or_dbs_run": (_I, [_P, _I, _I, _P, _D, _D, _D, _I, _I, _I,
                                 _P, _P, _P, _P, _I, _P, _P, _D, _P, _P, _P, _I, _P]),
             "or_num_threads": (_I, []),
+            "or_is_c64": (_I, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[c64] = L
+    return _libs[c64]
 
 
 def _ptr(a):
@@ -130,47 +135,50 @@ def invert(a):
 
 
 class Optimizer:
-    """One frequency's PNGF_Optimizer (pngf-opt/pngfopt.hpp:18-679), on the CPU."""
+    """One frequency's PNGF_Optimizer (pngf-opt/pngfopt.hpp:18-679), on the CPU; c64=True runs
+    it in complex64 like the reference's USE_SINGLE_PRECISION build."""
 
-    def __init__(self, kind, T, file_buf, ne=1, max_flips=1, cx=3, cy=3):
+    def __init__(self, kind, T, file_buf, ne=1, max_flips=1, cx=3, cy=3, c64=False):
         fb = np.asfortranarray(file_buf, dtype=np.complex128)
-        self.h = lib().or_opt_create(kind, T, cx, cy, ne, max_flips, _ptr(fb))
-        self.N = lib().or_opt_N(self.h)
+        self.c64 = c64
+        self.L = lib(c64)
+        self.h = self.L.or_opt_create(kind, T, cx, cy, ne, max_flips, _ptr(fb))
+        self.N = self.L.or_opt_N(self.h)
         self.ne = ne
         self.T = T
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_opt_destroy(self.h)
+            self.L.or_opt_destroy(self.h)
             self.h = None
 
     def set_tile_map(self, tm):
         tm = np.ascontiguousarray(tm, dtype=np.int32)
-        lib().or_opt_set_tile_map(self.h, _ptr(tm))
+        self.L.or_opt_set_tile_map(self.h, _ptr(tm))
 
     def set_source_port(self, tx, ty, yoff=1, ysz=2):
-        return lib().or_opt_set_source_port(self.h, tx, ty, yoff, ysz)
+        return self.L.or_opt_set_source_port(self.h, tx, ty, yoff, ysz)
 
     def compute_C_matrix(self):
-        lib().or_opt_compute_C(self.h)
+        self.L.or_opt_compute_C(self.h)
 
     def invert(self):
-        return lib().or_opt_invert(self.h)
+        return self.L.or_opt_invert(self.h)
 
     def set_cinv(self, cinv_colmajor):
         c = np.asfortranarray(cinv_colmajor, dtype=np.complex128)
-        lib().or_opt_set_cinv(self.h, _ptr(c))
+        self.L.or_opt_set_cinv(self.h, _ptr(c))
 
     def cinv(self):
         out = np.zeros((self.N, self.N), dtype=np.complex128, order="F")
-        lib().or_opt_get_cinv(self.h, _ptr(out))
+        self.L.or_opt_get_cinv(self.h, _ptr(out))
         return out
 
     def compute_PEinc(self):
-        lib().or_opt_compute_PEinc(self.h)
+        self.L.or_opt_compute_PEinc(self.h)
 
     def compute_R_matrix(self, flag):
-        lib().or_opt_compute_R(self.h, flag)
+        self.L.or_opt_compute_R(self.h, flag)
 
     def init_state(self, tile_map, port):
         self.set_tile_map(tile_map)
@@ -184,14 +192,14 @@ class Optimizer:
     def compute_updated_Eobj(self, tiles):
         t = np.ascontiguousarray(tiles, dtype=np.int32)
         out = np.zeros(self.ne, dtype=np.complex128)
-        lib().or_opt_eval(self.h, _ptr(t), len(t), _ptr(out))
+        self.L.or_opt_eval(self.h, _ptr(t), len(t), _ptr(out))
         return out
 
     def last_k(self):
-        return lib().or_opt_last_k(self.h)
+        return self.L.or_opt_last_k(self.h)
 
     def update_C_inv(self):
-        lib().or_opt_update(self.h)
+        self.L.or_opt_update(self.h)
 
     def accept(self):
         """update_C_inv + compute_PEinc + compute_R_matrix(0) (main.cpp:449-451)."""
@@ -201,19 +209,19 @@ class Optimizer:
 
     def scalars(self):
         out = np.zeros(self.ne + 3, dtype=np.complex128)
-        lib().or_opt_scalars(self.h, _ptr(out))
+        self.L.or_opt_scalars(self.h, _ptr(out))
         return dict(Eobj=out[: self.ne].copy(), Vinc=out[self.ne], LN1inc=out[self.ne + 1],
                     LN2inc=out[self.ne + 2])
 
     def vec(self, which):
         n = self.N * (self.ne if which == 3 else 1)
         out = np.zeros(n, dtype=np.complex128)
-        lib().or_opt_get_vec(self.h, {"S": 0, "Einc": 1, "PEinc": 2, "R": 3}[which], _ptr(out))
+        self.L.or_opt_get_vec(self.h, {"S": 0, "Einc": 1, "PEinc": 2, "R": 3}[which], _ptr(out))
         return out.reshape(self.N, self.ne).T.copy() if which == 3 else out
 
     def tile_map(self):
         out = np.zeros(self.T * self.T, dtype=np.int32)
-        lib().or_opt_get_tile_map(self.h, _ptr(out))
+        self.L.or_opt_get_tile_map(self.h, _ptr(out))
         return out
 
 
@@ -258,8 +266,10 @@ def mt19937_nth(seed, n):
 
 def dbs_run(opts, mode, freqs, dx, dy, dz, seed=0, max_evals=0, max_acc=100000, trace_cap=0,
             max_seconds=0.0, hist_cap=0):
-    """Full DBS loop over all frequencies (main.cpp:338-459 or the raster S11 driver)."""
+    """Full DBS loop over all frequencies (main.cpp:338-459 or the raster S11 driver), in the
+    precision of the optimizers' build (all complex128 or all complex64)."""
     nf = len(opts)
+    assert len({o.c64 for o in opts}) == 1
     arr = (ct.c_void_p * nf)(*[o.h for o in opts])
     T = opts[0].T
     acc_t = np.zeros(max_acc, dtype=np.int32)
@@ -274,7 +284,7 @@ def dbs_run(opts, mode, freqs, dx, dy, dz, seed=0, max_evals=0, max_acc=100000, 
     hp = np.zeros((max(hist_cap, 1), 2 + 3 * nf))
     hc = np.zeros(1, dtype=np.int32)
     h_on = hist_cap > 0
-    rc = lib().or_dbs_run(arr, nf, mode, _ptr(fr), dx, dy, dz, seed, max_evals, max_acc,
+    rc = opts[0].L.or_dbs_run(arr, nf, mode, _ptr(fr), dx, dy, dz, seed, max_evals, max_acc,
                           _ptr(acc_t), _ptr(acc_e), _ptr(acc_F), _ptr(tr) if trace_cap else None,
                           trace_cap, _ptr(fm), _ptr(st), max_seconds,
                           _ptr(ht) if h_on else None, _ptr(hd) if h_on else None, _ptr(hp) if h_on else None,

@@ -21,6 +21,12 @@
 //                              invariants
 //   * objective             -> hand-computed Zin/S11/D known answers
 //   * zgetrf/zgetrs/zgetri  -> scipy.linalg.lapack on the same matrices
+//
+// Built twice (oracle/Makefile): in complex128, and with PNGF_ORACLE_C64 in complex64 -- the
+// reference's USE_SINGLE_PRECISION build (main.cpp:58-72: `precision` = float throughout, the
+// c128 file cast to c64 at load, pngfopt.hpp:128-156; c-prefixed LAPACK/BLAS,
+// pngfopt.hpp:285-347, 488-510, linalg.hpp:25-78).  The C ABI below stays in double either way:
+// arrays crossing it are converted at the boundary.
 #include <complex>
 #include <vector>
 #include <random>
@@ -32,7 +38,13 @@
 #include <algorithm>
 #include <omp.h>
 
-typedef std::complex<double> cd;
+#ifdef PNGF_ORACLE_C64
+typedef float real;
+#else
+typedef double real;
+#endif
+typedef std::complex<real> cd;
+typedef std::complex<double> c128;
 
 namespace {
 
@@ -102,20 +114,20 @@ struct Layout {
 int lu_factor(int n, cd* a, int lda, int* ipiv) {
     int info = 0;
     for (int j = 0; j < n; j++) {
-        int p = j; double best = -1.0;
+        int p = j; real best = -1;
         for (int i = j; i < n; i++) {
-            double v = std::fabs(a[i + (size_t)j * lda].real()) + std::fabs(a[i + (size_t)j * lda].imag());
+            real v = std::fabs(a[i + (size_t)j * lda].real()) + std::fabs(a[i + (size_t)j * lda].imag());
             if (v > best) { best = v; p = i; }
         }
         ipiv[j] = p;
-        if (a[p + (size_t)j * lda] == cd(0.0, 0.0)) { if (!info) info = j + 1; continue; }
+        if (a[p + (size_t)j * lda] == cd(0, 0)) { if (!info) info = j + 1; continue; }
         if (p != j)
             for (int c = 0; c < n; c++) std::swap(a[j + (size_t)c * lda], a[p + (size_t)c * lda]);
-        cd r = 1.0 / a[j + (size_t)j * lda];
+        cd r = real(1) / a[j + (size_t)j * lda];
         for (int i = j + 1; i < n; i++) a[i + (size_t)j * lda] *= r;
         for (int c = j + 1; c < n; c++) {
             cd f = a[j + (size_t)c * lda];
-            if (f == cd(0.0, 0.0)) continue;
+            if (f == cd(0, 0)) continue;
             cd* col = a + (size_t)c * lda;
             const cd* lj = a + (size_t)j * lda;
             for (int i = j + 1; i < n; i++) col[i] -= lj[i] * f;
@@ -142,12 +154,12 @@ void lu_solve(int n, const cd* a, int lda, const int* ipiv, cd* b) {
 void lu_inverse(int n, cd* a, int lda, const int* ipiv) {
     // inv(U) in place (upper triangle)
     for (int j = 0; j < n; j++) {
-        a[j + (size_t)j * lda] = 1.0 / a[j + (size_t)j * lda];
+        a[j + (size_t)j * lda] = real(1) / a[j + (size_t)j * lda];
         cd ajj = -a[j + (size_t)j * lda];
         // column j above the diagonal: x = inv(U(0:j,0:j)) * U(0:j, j)
         std::vector<cd> x(j);
         for (int i = 0; i < j; i++) {
-            cd s = 0.0;
+            cd s = 0;
             for (int l = i; l < j; l++) s += a[i + (size_t)l * lda] * a[l + (size_t)j * lda];
             x[i] = s;
         }
@@ -156,9 +168,9 @@ void lu_inverse(int n, cd* a, int lda, const int* ipiv) {
     // solve X * L = inv(U) for X, column by column from the right
     std::vector<cd> work(n);
     for (int j = n - 2; j >= 0; j--) {
-        for (int i = j + 1; i < n; i++) { work[i] = a[i + (size_t)j * lda]; a[i + (size_t)j * lda] = 0.0; }
+        for (int i = j + 1; i < n; i++) { work[i] = a[i + (size_t)j * lda]; a[i + (size_t)j * lda] = 0; }
         for (int i = 0; i < n; i++) {
-            cd s = 0.0;
+            cd s = 0;
             for (int l = j + 1; l < n; l++) s += a[i + (size_t)l * lda] * work[l];
             a[i + (size_t)j * lda] -= s;
         }
@@ -179,18 +191,18 @@ int invert_dense(int n, cd* a) {
     // right-looking LU with the trailing update parallel over columns
     int info = 0;
     for (int j = 0; j < n; j++) {
-        int p = j; double best = -1.0;
+        int p = j; real best = -1;
         for (int i = j; i < n; i++) {
-            double v = std::fabs(a[i + (size_t)j * n].real()) + std::fabs(a[i + (size_t)j * n].imag());
+            real v = std::fabs(a[i + (size_t)j * n].real()) + std::fabs(a[i + (size_t)j * n].imag());
             if (v > best) { best = v; p = i; }
         }
         ipiv[j] = p;
-        if (a[p + (size_t)j * n] == cd(0.0, 0.0)) { info = j + 1; break; }
+        if (a[p + (size_t)j * n] == cd(0, 0)) { info = j + 1; break; }
         if (p != j) {
             #pragma omp parallel for
             for (int c = 0; c < n; c++) std::swap(a[j + (size_t)c * n], a[p + (size_t)c * n]);
         }
-        cd r = 1.0 / a[j + (size_t)j * n];
+        cd r = real(1) / a[j + (size_t)j * n];
         for (int i = j + 1; i < n; i++) a[i + (size_t)j * n] *= r;
         #pragma omp parallel for schedule(static)
         for (int c = j + 1; c < n; c++) {
@@ -206,8 +218,8 @@ int invert_dense(int n, cd* a) {
     #pragma omp parallel for schedule(dynamic, 8)
     for (int c = 0; c < n; c++) {
         cd* b = inv.data() + (size_t)c * n;
-        for (int i = 0; i < n; i++) b[i] = 0.0;
-        b[c] = 1.0;
+        for (int i = 0; i < n; i++) b[i] = 0;
+        b[c] = 1;
         lu_solve(n, a, n, ipiv.data(), b);
     }
     std::memcpy(a, inv.data(), sizeof(cd) * (size_t)n * n);
@@ -222,7 +234,7 @@ struct Opt {
     std::vector<cd> G, C, Gobj, R, Einc, PEinc, S, Snew;   // column-major
     std::vector<cd> V, CinvU, VCinv, VinvCU, RU, VSnew;
     std::vector<int> tile_map, yee_grid, diff_P, P_idx, ipiv;
-    cd Vinc = 0.0, LN1inc = 0.0, LN2inc = 0.0;
+    cd Vinc = 0, LN1inc = 0, LN2inc = 0;
     std::vector<cd> Eobj, newEobj;
     int nk = 0;                   // num_yee_flipped
     int last_flipped[16];
@@ -231,29 +243,29 @@ struct Opt {
 
     // reference: pngfopt.hpp:66-191 (init).  `file` is the acmefdfd raw buffer:
     // M x M column-major complex128 with M = N + ne - 1; G = file[0:N,0:N],
-    // Gobj(1+r, c) = file[N+r, c].
-    void init(int kind, int T, int cx, int cy, int tne, int max_flips, const cd* file) {
+    // Gobj(1+r, c) = file[N+r, c] (cast to complex64 in the c64 build, pngfopt.hpp:128-156).
+    void init(int kind, int T, int cx, int cy, int tne, int max_flips, const c128* file) {
         L.setup(kind, T, cx, cy);
         N = L.N; ne = tne;
         max_k = L.per_tile * max_flips;
         size_t NN = (size_t)N * N;
-        G.assign(NN, 0.0); C.assign(NN, 0.0);
-        Gobj.assign((size_t)ne * N, 0.0); R.assign((size_t)ne * N, 0.0);
-        Einc.assign(N, 0.0); PEinc.assign(N, 0.0); S.assign(N, 0.0); Snew.assign(N, 0.0);
-        V.assign((size_t)max_k * N, 0.0); CinvU.assign((size_t)max_k * N, 0.0);
-        VCinv.assign((size_t)max_k * N, 0.0); VinvCU.assign((size_t)max_k * max_k, 0.0);
-        RU.assign((size_t)ne * N, 0.0); VSnew.assign(N, 0.0);
+        G.assign(NN, 0); C.assign(NN, 0);
+        Gobj.assign((size_t)ne * N, 0); R.assign((size_t)ne * N, 0);
+        Einc.assign(N, 0); PEinc.assign(N, 0); S.assign(N, 0); Snew.assign(N, 0);
+        V.assign((size_t)max_k * N, 0); CinvU.assign((size_t)max_k * N, 0);
+        VCinv.assign((size_t)max_k * N, 0); VinvCU.assign((size_t)max_k * max_k, 0);
+        RU.assign((size_t)ne * N, 0); VSnew.assign(N, 0);
         tile_map.assign(T * T, 0); yee_grid.assign(N, 0); diff_P.assign(N, 0);
         P_idx.assign(N, 0); ipiv.assign(N, 0);
-        Eobj.assign(ne, 0.0); newEobj.assign(ne, 0.0);
+        Eobj.assign(ne, 0); newEobj.assign(ne, 0);
         size_t M = (size_t)N + ne - 1;
         #pragma omp parallel for
         for (long long ii = 0; ii < (long long)NN; ii++) {
             size_t col = ii / N, row = ii - col * N;
-            G[ii] = file[row + M * col];
+            G[ii] = (cd)file[row + M * col];
         }
         for (int r = 0; r < ne - 1; r++)
-            for (int c = 0; c < N; c++) Gobj[(1 + r) + (size_t)ne * c] = file[(N + r) + M * c];
+            for (int c = 0; c < N; c++) Gobj[(1 + r) + (size_t)ne * c] = (cd)file[(N + r) + M * c];
     }
     // reference: pngfopt.hpp:669-674
     void set_tile_map(const int* tm) {
@@ -264,15 +276,15 @@ struct Opt {
     int set_source_port(int tx, int ty, int yoff, int ysz) {
         if (yoff + ysz > L.cy) return -1;
         L.set_port(tx, ty, yoff, ysz);
-        for (int c = 0; c < N; c++) Gobj[0 + (size_t)ne * c] = 0.0;
-        std::fill(Einc.begin(), Einc.end(), cd(0.0));
-        LN1inc = LN2inc = 0.0;
+        for (int c = 0; c < N; c++) Gobj[0 + (size_t)ne * c] = 0;
+        std::fill(Einc.begin(), Einc.end(), cd(0));
+        LN1inc = LN2inc = 0;
         for (int p : L.port) {
             for (int c = 0; c < N; c++) Gobj[0 + (size_t)ne * c] += G[p + (size_t)N * c];
             for (int r = 0; r < N; r++) Einc[r] += G[r + (size_t)N * p];
             if (ne > 1) { LN1inc += Gobj[1 + (size_t)ne * p]; LN2inc += Gobj[2 + (size_t)ne * p]; }
         }
-        Vinc = 0.0;
+        Vinc = 0;
         for (int p : L.port) Vinc += Einc[p];
         return 0;
     }
@@ -282,36 +294,36 @@ struct Opt {
         for (int col = 0; col < N; col++) {
             size_t base = (size_t)col * N;
             for (int row = 0; row < N; row++)
-                C[base + row] = yee_grid[row] == 1 ? G[base + row] : (row == col ? cd(1.0) : cd(0.0));
+                C[base + row] = yee_grid[row] == 1 ? G[base + row] : (row == col ? cd(1) : cd(0));
         }
     }
     int invert() { return invert_dense(N, C.data()); }
     // reference: pngfopt.hpp:542-552
     void compute_PEinc() {
-        for (int i = 0; i < N; i++) PEinc[i] = yee_grid[i] ? -Einc[i] : cd(0.0);
+        for (int i = 0; i < N; i++) PEinc[i] = yee_grid[i] ? -Einc[i] : cd(0);
     }
     // reference: pngfopt.hpp:237-250;  R = Gobj*Cinv, S = Cinv*PEinc
     void compute_R(int compute_eobj) {
         #pragma omp parallel for
         for (int c = 0; c < N; c++) {
             for (int r = 0; r < ne; r++) {
-                cd s = 0.0;
+                cd s = 0;
                 const cd* cc = C.data() + (size_t)c * N;
                 for (int i = 0; i < N; i++) s += Gobj[r + (size_t)ne * i] * cc[i];
                 R[r + (size_t)ne * c] = s;
             }
         }
-        std::vector<cd> s(N, 0.0);
+        std::vector<cd> s(N, cd(0));
         for (int c = 0; c < N; c++) {
             cd pe = PEinc[c];
-            if (pe == cd(0.0)) continue;
+            if (pe == cd(0)) continue;
             const cd* cc = C.data() + (size_t)c * N;
             for (int i = 0; i < N; i++) s[i] += cc[i] * pe;
         }
         S = s;
         if (compute_eobj) {
             for (int r = 0; r < ne; r++) {
-                cd e = 0.0;
+                cd e = 0;
                 for (int c = 0; c < N; c++) e += R[r + (size_t)ne * c] * PEinc[c];
                 Eobj[r] = e;
             }
@@ -334,7 +346,7 @@ struct Opt {
         for (int i = 0; i < N; i++) {
             if (diff_P[i] != 0) {
                 P_idx[nk] = i;
-                double sg = diff_P[i] > 0 ? 1.0 : -1.0;
+                real sg = diff_P[i] > 0 ? 1 : -1;
                 for (int r = 0; r < ne; r++) RU[(size_t)nk * ne + r] = sg * R[r + (size_t)ne * i];
                 nk++;
             }
@@ -345,42 +357,42 @@ struct Opt {
             size_t base = (size_t)jj * N;
             int t = P_idx[jj];
             for (int ii = 0; ii < N; ii++) V[base + ii] = G[t + (size_t)ii * N];
-            V[base + t] -= 1.0;
+            V[base + t] -= real(1);
         }
         #pragma omp parallel for
         for (int ii = 0; ii < k * k; ii++) {
             int col = ii / k, row = ii - col * k;
             const cd* v = V.data() + (size_t)row * N;
             const cd* cc = C.data() + (size_t)P_idx[col] * N;
-            cd s = 0.0;
+            cd s = 0;
             for (int i = 0; i < N; i++) s += v[i] * cc[i];
             if (diff_P[P_idx[col]] < 0) s = -s;
-            if (row == col) s += 1.0;
+            if (row == col) s += real(1);
             VinvCU[ii] = s;
         }
         #pragma omp parallel for
         for (int ii = 0; ii < N; ii++) {
-            cd s = 0.0;
+            cd s = 0;
             for (int jj = 0; jj < k; jj++)
-                s += C[ii + (size_t)P_idx[jj] * N] * Einc[P_idx[jj]] * (double)diff_P[P_idx[jj]];
+                s += C[ii + (size_t)P_idx[jj] * N] * Einc[P_idx[jj]] * (real)diff_P[P_idx[jj]];
             Snew[ii] = S[ii] - s;
         }
         #pragma omp parallel for
         for (int ii = 0; ii < k; ii++) {
             const cd* v = V.data() + (size_t)ii * N;
-            cd s = 0.0;
+            cd s = 0;
             for (int jj = 0; jj < N; jj++) s += v[jj] * Snew[jj];
             VSnew[ii] = s;
         }
         last_lu_info = lu_factor(k, VinvCU.data(), k, ipiv.data());
         lu_solve(k, VinvCU.data(), k, ipiv.data(), VSnew.data());
         for (int r = 0; r < ne; r++) {
-            cd s = 0.0;
+            cd s = 0;
             for (int i = 0; i < N; i++) s += Gobj[r + (size_t)ne * i] * Snew[i];
             newEobj[r] = s;
         }
         for (int r = 0; r < ne; r++) {
-            cd s = 0.0;
+            cd s = 0;
             for (int i = 0; i < k; i++) s += RU[(size_t)i * ne + r] * VSnew[i];
             newEobj[r] -= s;
         }
@@ -394,7 +406,7 @@ struct Opt {
         #pragma omp parallel for
         for (int ii = 0; ii < k; ii++) {
             size_t base = (size_t)ii * N, cb = (size_t)P_idx[ii] * N;
-            double sg = diff_P[P_idx[ii]] > 0 ? 1.0 : -1.0;
+            real sg = diff_P[P_idx[ii]] > 0 ? 1 : -1;
             for (int jj = 0; jj < N; jj++) CinvU[base + jj] = sg * C[cb + jj];
         }
         lu_inverse(k, VinvCU.data(), k, ipiv.data());
@@ -403,7 +415,7 @@ struct Opt {
             int col = (int)(ii / k), row = (int)(ii - (long long)col * k);
             const cd* v = V.data() + (size_t)row * N;
             const cd* cc = C.data() + (size_t)col * N;
-            cd s = 0.0;
+            cd s = 0;
             for (int i = 0; i < N; i++) s += v[i] * cc[i];
             VCinv[ii] = s;
         }
@@ -412,7 +424,7 @@ struct Opt {
         for (int col = 0; col < N; col++) {
             cd tmp[128];
             for (int r = 0; r < k; r++) {
-                cd s = 0.0;
+                cd s = 0;
                 for (int l = 0; l < k; l++) s += VinvCU[r + (size_t)l * k] * VCinv[l + (size_t)col * k];
                 tmp[r] = s;
             }
@@ -437,16 +449,49 @@ struct Opt {
 const double kC0 = 2.997924579998211E8;
 const double kEta0 = 3.767303134118051e+02;
 
-struct Geo { double dx, dy, dz; };
+// The objective in the arithmetic type T of the build (the reference computes it in `precision`,
+// main.cpp:265-300, 401-423); T = double is the complex128 build's.
+template <class T>
+struct Objective {
+    typedef std::complex<T> C;
+    T dx, dy, dz;
+    // reference: main.cpp:277-289 / 407-413
+    C zin(C e0, C vinc) const { return -(e0 + vinc) * dx * T(0.5) / (T(2.0) * dy * dz); }
+    static C s11(C zin) { return (zin - T(50.0)) / (zin + T(50.0)); }
+    T dir(const C* e, C vinc, C ln1, C ln2, T f) const {
+        T k0 = T(2.0 * M_PI) * f / T(kC0);
+        C a = e[1] + ln1, b = e[2] + ln2;
+        T d = (k0 * k0 / T(32 * M_PI * M_PI * kEta0)) * std::real(std::conj(a) * a + std::conj(b) * b);
+        return T(4.0 * M_PI) * d / std::real(-(e[0] + vinc) * dx * T(0.5) * (T(2.0) * dy * dz) * T(0.5));
+    }
+    // F of main.cpp:407-420 (mode 0) or mean_f |S11_f|^2 (mode 1); S11 and D per frequency if asked
+    T operator()(int mode, int nf, const C* E, int ne, const C* Vi, const C* LN, const double* freqs, double* s11_out,
+                 double* d_out) const {
+        T acc = 0;
+        for (int f = 0; f < nf; f++) {
+            C s = s11(zin(E[f * ne], Vi[f]));
+            T m = std::abs(s);
+            if (s11_out) { s11_out[2 * f] = s.real(); s11_out[2 * f + 1] = s.imag(); }
+            if (mode == 0) {
+                T D = dir(E + f * ne, Vi[f], LN[2 * f], LN[2 * f + 1], T(freqs[f]));
+                if (d_out) d_out[f] = D;
+                acc += T(200.0) * m * m + (T(15.0) - D) * (T(15.0) - D);
+            } else {
+                acc += m * m;
+            }
+        }
+        return mode == 0 ? acc : acc / T(nf);
+    }
+};
 
-// reference: main.cpp:277-289 / 407-413
-inline cd zin_of(cd e0, cd vinc, const Geo& g) { return -(e0 + vinc) * g.dx * 0.5 / (2.0 * g.dy * g.dz); }
-inline cd s11_of(cd zin) { return (zin - 50.0) / (zin + 50.0); }
-inline double dir_of(const cd* e, cd vinc, cd ln1, cd ln2, double f, const Geo& g) {
-    double k0 = 2.0 * M_PI * f / kC0;
-    cd a = e[1] + ln1, b = e[2] + ln2;
-    double d = (k0 * k0 / (32 * M_PI * M_PI * kEta0)) * std::real(std::conj(a) * a + std::conj(b) * b);
-    return 4.0 * M_PI * d / std::real(-(e[0] + vinc) * g.dx * 0.5 * (2.0 * g.dy * g.dz) * 0.5);
+// double arrays across the C ABI <-> the build's complex type
+std::vector<cd> from_abi(const double* p, size_t n) {
+    std::vector<cd> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = cd((real)p[2 * i], (real)p[2 * i + 1]);
+    return v;
+}
+void to_abi(const cd* v, size_t n, double* p) {
+    for (size_t i = 0; i < n; i++) { p[2 * i] = v[i].real(); p[2 * i + 1] = v[i].imag(); }
 }
 
 // ---------------------------------------------------------------- MagicHat --
@@ -494,15 +539,35 @@ int or_layout_port(int kind, int T, int cx, int cy, int tx, int ty, int yoff, in
 }
 
 // ---- dense LAPACK-equivalent helpers (for pinning against scipy)
-int or_zgetrf(int n, double* a, int* ipiv) { return lu_factor(n, (cd*)a, n, ipiv); }
-void or_zgetrs(int n, const double* a, const int* ipiv, double* b) { lu_solve(n, (const cd*)a, n, ipiv, (cd*)b); }
-void or_zgetri(int n, double* a, const int* ipiv) { lu_inverse(n, (cd*)a, n, ipiv); }
-int or_invert(int n, double* a) { return invert_dense(n, (cd*)a); }
+int or_zgetrf(int n, double* a, int* ipiv) {
+    std::vector<cd> m = from_abi(a, (size_t)n * n);
+    int info = lu_factor(n, m.data(), n, ipiv);
+    to_abi(m.data(), m.size(), a);
+    return info;
+}
+void or_zgetrs(int n, const double* a, const int* ipiv, double* b) {
+    std::vector<cd> m = from_abi(a, (size_t)n * n), x = from_abi(b, n);
+    lu_solve(n, m.data(), n, ipiv, x.data());
+    to_abi(x.data(), n, b);
+}
+void or_zgetri(int n, double* a, const int* ipiv) {
+    std::vector<cd> m = from_abi(a, (size_t)n * n);
+    lu_inverse(n, m.data(), n, ipiv);
+    to_abi(m.data(), m.size(), a);
+}
+int or_invert(int n, double* a) {
+    std::vector<cd> m = from_abi(a, (size_t)n * n);
+    int info = invert_dense(n, m.data());
+    to_abi(m.data(), m.size(), a);
+    return info;
+}
+// 1 in the complex64 build
+int or_is_c64() { return sizeof(real) == sizeof(float) ? 1 : 0; }
 
 // ---- optimizer object (mirrors PNGF_Optimizer's methods)
 void* or_opt_create(int kind, int T, int cx, int cy, int ne, int max_flips, const double* file) {
     Opt* o = new Opt();
-    o->init(kind, T, cx, cy, ne, max_flips, (const cd*)file);
+    o->init(kind, T, cx, cy, ne, max_flips, (const c128*)file);
     return o;
 }
 void or_opt_destroy(void* h) { delete (Opt*)h; }
@@ -512,52 +577,40 @@ int or_opt_set_source_port(void* h, int tx, int ty, int yoff, int ysz) { return 
 void or_opt_compute_C(void* h) { ((Opt*)h)->compute_C(); }
 int or_opt_invert(void* h) { return ((Opt*)h)->invert(); }
 void or_opt_set_cinv(void* h, const double* cinv) {  // column-major N x N
-    Opt* o = (Opt*)h; std::memcpy(o->C.data(), cinv, sizeof(cd) * (size_t)o->N * o->N);
+    Opt* o = (Opt*)h; o->C = from_abi(cinv, (size_t)o->N * o->N);
 }
 void or_opt_get_cinv(void* h, double* out) {
-    Opt* o = (Opt*)h; std::memcpy(out, o->C.data(), sizeof(cd) * (size_t)o->N * o->N);
+    Opt* o = (Opt*)h; to_abi(o->C.data(), (size_t)o->N * o->N, out);
 }
 void or_opt_compute_PEinc(void* h) { ((Opt*)h)->compute_PEinc(); }
 void or_opt_compute_R(void* h, int flag) { ((Opt*)h)->compute_R(flag); }
 void or_opt_eval(void* h, const int* tiles, int n, double* new_eobj) {
     Opt* o = (Opt*)h; o->compute_updated_Eobj(tiles, n);
-    std::memcpy(new_eobj, o->newEobj.data(), sizeof(cd) * o->ne);
+    to_abi(o->newEobj.data(), o->ne, new_eobj);
 }
 int or_opt_last_k(void* h) { return ((Opt*)h)->nk; }
 int or_opt_last_info(void* h) { return ((Opt*)h)->last_lu_info; }
 void or_opt_update(void* h) { ((Opt*)h)->update_C_inv(); }
 // out: Eobj[ne], Vinc, LN1inc, LN2inc (complex)
 void or_opt_scalars(void* h, double* out) {
-    Opt* o = (Opt*)h; cd* c = (cd*)out;
-    for (int r = 0; r < o->ne; r++) c[r] = o->Eobj[r];
-    c[o->ne] = o->Vinc; c[o->ne + 1] = o->LN1inc; c[o->ne + 2] = o->LN2inc;
+    Opt* o = (Opt*)h;
+    std::vector<cd> c(o->Eobj);
+    c.push_back(o->Vinc); c.push_back(o->LN1inc); c.push_back(o->LN2inc);
+    to_abi(c.data(), c.size(), out);
 }
 void or_opt_get_vec(void* h, int which, double* out) {  // 0 S, 1 Einc, 2 PEinc, 3 R (ne x N col-major)
     Opt* o = (Opt*)h;
     const std::vector<cd>* v = which == 0 ? &o->S : which == 1 ? &o->Einc : which == 2 ? &o->PEinc : &o->R;
-    std::memcpy(out, v->data(), sizeof(cd) * v->size());
+    to_abi(v->data(), v->size(), out);
 }
 void or_opt_get_tile_map(void* h, int* out) { Opt* o = (Opt*)h; for (int i = 0; i < o->L.T * o->L.T; i++) out[i] = o->tile_map[i]; }
 
 // ---- objective known answers (reference: main.cpp:277-289, 407-413)
 void or_objective(int mode, int nf, const double* eobj, int ne, const double* vinc, const double* ln,
                   const double* freqs, double dx, double dy, double dz, double* F, double* s11_out, double* d_out) {
-    Geo g{dx, dy, dz};
-    const cd* E = (const cd*)eobj; const cd* Vi = (const cd*)vinc; const cd* LN = (const cd*)ln;
-    double acc = 0.0;
-    for (int f = 0; f < nf; f++) {
-        cd s = s11_of(zin_of(E[f * ne], Vi[f], g));
-        double m = std::abs(s);
-        if (s11_out) { s11_out[2 * f] = s.real(); s11_out[2 * f + 1] = s.imag(); }
-        if (mode == 0) {
-            double D = dir_of(E + f * ne, Vi[f], LN[2 * f], LN[2 * f + 1], freqs[f], g);
-            if (d_out) d_out[f] = D;
-            acc += 200.0 * m * m + (15.0 - D) * (15.0 - D);
-        } else {
-            acc += m * m;
-        }
-    }
-    *F = mode == 0 ? acc : acc / nf;
+    // always in double: the objective's known answers
+    Objective<double> obj{dx, dy, dz};
+    *F = obj(mode, nf, (const c128*)eobj, ne, (const c128*)vinc, (const c128*)ln, freqs, s11_out, d_out);
 }
 
 // ---- MagicHat
@@ -598,7 +651,7 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
                int* final_map, double* stats, double max_seconds,
                int* hist_tile, int* hist_dec, double* hist_perf, int hist_cap, int* hist_count) {
     Opt** O = (Opt**)opts;
-    Geo g{dx, dy, dz};
+    const Objective<real> obj{(real)dx, (real)dy, (real)dz};
     const int T = O[0]->L.T, ne = O[0]->ne;
     std::vector<cd> E((size_t)nf * ne), Vi(nf), LN(2 * nf);
     auto objective = [&](bool use_new) {
@@ -606,9 +659,7 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
             for (int r = 0; r < ne; r++) E[(size_t)f * ne + r] = use_new ? O[f]->newEobj[r] : O[f]->Eobj[r];
             Vi[f] = O[f]->Vinc; LN[2 * f] = O[f]->LN1inc; LN[2 * f + 1] = O[f]->LN2inc;
         }
-        double F;
-        or_objective(mode, nf, (double*)E.data(), ne, (double*)Vi.data(), (double*)LN.data(), freqs, g.dx, g.dy, g.dz, &F, nullptr, nullptr);
-        return F;
+        return (double)obj(mode, nf, E.data(), ne, Vi.data(), LN.data(), freqs, nullptr, nullptr);
     };
     const int nperf = 2 + 3 * nf;
     std::vector<double> s11(2 * nf), dir(nf, 0.0);
@@ -621,9 +672,7 @@ int or_dbs_run(void** opts, int nf, int mode, const double* freqs, double dx, do
             for (int r = 0; r < ne; r++) E[(size_t)f * ne + r] = use_new ? O[f]->newEobj[r] : O[f]->Eobj[r];
             Vi[f] = O[f]->Vinc; LN[2 * f] = O[f]->LN1inc; LN[2 * f + 1] = O[f]->LN2inc;
         }
-        double F;
-        or_objective(mode, nf, (double*)E.data(), ne, (double*)Vi.data(), (double*)LN.data(), freqs, g.dx, g.dy,
-                     g.dz, &F, s11.data(), dir.data());
+        const double F = obj(mode, nf, E.data(), ne, Vi.data(), LN.data(), freqs, s11.data(), dir.data());
         double* pr = hist_perf + (size_t)nhist * nperf;
         pr[0] = use_new ? F : F_best_now;
         pr[1] = F_best_now;

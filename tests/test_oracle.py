@@ -163,6 +163,58 @@ def test_accept_equals_reinversion_and_flip_twice_restores(kind):
     assert rel(o.cinv(), c_start) < 1e-10
 
 
+# ------------------------------------------------- complex64 build (FP32) --
+def test_c64_build_is_single_precision():
+    assert O.lib(c64=True).or_is_c64() == 1 and O.lib().or_is_c64() == 0
+
+
+@pytest.mark.parametrize("kind", [O.LAYOUT_SHARED, O.LAYOUT_DISJOINT])
+@pytest.mark.parametrize("ne", [1, 3])
+def test_c64_eval_tracks_dense_resolve(kind, ne):
+    """The complex64 build (the reference's USE_SINGLE_PRECISION) scores flips to FP32 accuracy of
+    the dense re-solve (a wrong term would be an O(1) error), and it does compute in FP32: its
+    scores are not the complex128 build's."""
+    T = 5
+    G, extra, fb, tm = random_problem(kind, T, ne, 11 + ne + 7 * kind)
+    port = O.layout_port(kind, T, 2, 2)
+    o32 = O.Optimizer(kind, T, fb, ne=ne, max_flips=4, c64=True)
+    o64 = O.Optimizer(kind, T, fb, ne=ne, max_flips=4)
+    assert o32.init_state(tm, (2, 2, 1, 2)) == 0 and o64.init_state(tm, (2, 2, 1, 2)) == 0
+    assert rel(o32.scalars()["Eobj"], dense_eobj(kind, T, G, extra, tm, port)) < 1e-5
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(12):
+        nt = int(rng.integers(1, 5))
+        tiles = sorted(rng.choice(T * T, nt, replace=False).tolist())
+        e32 = o32.compute_updated_Eobj(tiles)
+        e64 = o64.compute_updated_Eobj(tiles)
+        t2 = tm.copy()
+        t2[tiles] ^= 1
+        ref = dense_eobj(kind, T, G, extra, t2, port)
+        assert rel(e32, ref) < 1e-5
+        worst = max(worst, rel(e32, e64))
+    assert worst > 1e-9
+
+
+def test_c64_accept_equals_reinversion():
+    """Woodbury accepts in complex64 keep C^-1 equal to the inverse of the updated C to FP32
+    accuracy (numpy complex128 inverse as the truth)."""
+    kind, T = O.LAYOUT_DISJOINT, 4
+    G, extra, fb, tm = random_problem(kind, T, 1, 29)
+    N = G.shape[0]
+    o = O.Optimizer(kind, T, fb, ne=1, c64=True)
+    assert o.init_state(tm, (2, 2, 1, 2)) == 0
+    cur = tm.copy()
+    for t in [5, 0, 15, 5, 9]:
+        o.compute_updated_Eobj([t])
+        o.accept()
+        cur[t] ^= 1
+        P = O.layout_mask(kind, T, cur)
+        C = np.eye(N, dtype=complex)
+        C[P == 1, :] = G[P == 1, :]
+        assert 1e-9 < rel(o.cinv(), np.linalg.inv(C)) < 1e-5
+
+
 def test_golden_fixture():
     z = np.load(os.path.join(HERE, "golden", "woodbury_small.npz"))
     names = sorted({k.split("__")[0] for k in z.files})
