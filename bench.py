@@ -34,6 +34,7 @@ from pngf_amd.problem import MULTI16, WORKLOADS, initial_pattern  # noqa: E402
 METRIC = "DBS evaluated flips/s, 21×21 tiles × 9 freqs, vs plain FP64 GPU at matched S11"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
 
 
@@ -41,10 +42,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0, perturb=0.0):
+def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0, perturb=0.0, precision=P.PREC_C128):
     T, freqs = wl["T"], wl["freqs"]
     opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=method, max_deferred=max_deferred,
-                      symmetry=symmetry, tile_variant=tile_variant)
+                      symmetry=symmetry, tile_variant=tile_variant, precision=precision)
     for f, hz in enumerate(freqs):
         opt.set_frequency(f, hz)
         opt.synth_g(f)
@@ -94,24 +95,24 @@ def roofline(opt, gmax=24):
     return rec, allk
 
 
-def strip_stats(opt):
+def strip_stats(opt, peak=PEAK_FP64_TFLOPS):
     """Event-timed column and row strips of a profiled run: launches, average us and the
-    fraction of FP64 peak at their algorithmic flops."""
+    fraction of the MFMA peak (FP64, or FP32 for the complex64 mode) at their algorithmic flops."""
     out = {}
     for k in (P.K_COLBLK, P.K_ROWBLK):
         s = opt.kernel_stats(k)
         if s["launches"]:
             avg = s["total_ms"] / s["launches"]
             tf = s["flops"] / (s["total_ms"] * 1e-3) / 1e12
-            out[P.K_NAMES[k]] = dict(launches=s["launches"], avg_us=avg * 1e3, tflops=tf,
-                                     frac=tf / PEAK_FP64_TFLOPS)
+            out[P.K_NAMES[k]] = dict(launches=s["launches"], avg_us=avg * 1e3, tflops=tf, frac=tf / peak,
+                                     peak_tflops=peak)
     return out
 
 
-def variant(wl, name, symmetry, perturb, headline):
-    """The LOWRANK run from the same start on another accept path: one untimed warmup, one run
-    for the rate, one profiled run for the strips' event times."""
-    opt, init_s = make(wl, P.METHOD_LOWRANK, symmetry=symmetry, perturb=perturb)
+def variant(wl, name, symmetry, perturb, headline, precision=P.PREC_C128):
+    """The LOWRANK run from the same start on another accept path (or in complex64): one untimed
+    warmup, one run for the rate, one profiled run for the strips' event times."""
+    opt, init_s = make(wl, P.METHOD_LOWRANK, symmetry=symmetry, perturb=perturb, precision=precision)
     try:
         used, asym = opt.symmetry()
         opt.checkpoint()
@@ -121,14 +122,31 @@ def variant(wl, name, symmetry, perturb, headline):
         opt.restore()
         opt.profile(True)
         opt.dbs_run(max_evals=wl["max_evals"])
-        strips = strip_stats(opt)
+        strips = strip_stats(opt, PEAK_FP32_TFLOPS if precision == P.PREC_C64 else PEAK_FP64_TFLOPS)
         opt.profile(False)
+        final_map = opt.tile_map()
     finally:
         opt.close()
     out = dict(value=r["flip_count"] / r["seconds"], seconds=r["seconds"], flip_count=r["flip_count"],
                accepts=r["accepts"], F_final=r["F_final"], symmetric_accept=used, rel_asym=asym,
-               init_s=init_s, strips=strips,
+               init_s=init_s, strips=strips, F_final_vs_headline=r["F_final"] / headline["F_final"],
                same_accept_sequence=bool(np.array_equal(r["acc_tiles"], headline["acc_tiles"])))
+    if precision == P.PREC_C64:
+        # the complex64 state's objective against a complex128 init_state of the design it reached
+        fresh = P.Optimizer(wl["layout"], wl["T"], num_freq=len(wl["freqs"]), method=P.METHOD_LOWRANK)
+        try:
+            for f, hz in enumerate(wl["freqs"]):
+                fresh.set_frequency(f, hz)
+                fresh.synth_g(f)
+            fresh.set_tile_map(final_map)
+            mid = (wl["T"] - 1) // 2
+            fresh.set_source_port(mid, mid, 1, 2)
+            fresh.init_state()
+            F64 = fresh.state()["F"]
+        finally:
+            fresh.close()
+        out["F_final_fp64_of_design"] = F64
+        out["F_error_vs_fp64"] = abs(r["F_final"] - F64) / F64
     log("[bench] %s: %d evals %d accepts %.3f s (%.0f flips/s), strips %s" % (
         name, r["flip_count"], r["accepts"], r["seconds"], out["value"],
         {k: "%.1f us %.3f" % (v["avg_us"], v["frac"]) for k, v in strips.items()}))
@@ -348,7 +366,8 @@ def main():
     variants = None
     if not args.no_variants and method == P.METHOD_LOWRANK:
         variants = {"symmetry_off": variant(wl, "symmetry off", P.SYM_OFF, 0.0, last),
-                    "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last)}
+                    "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last),
+                    "precision_c64": variant(wl, "complex64", P.SYM_AUTO, 0.0, last, P.PREC_C64)}
 
     plain = None
     g0 = cinv0 = None

@@ -76,6 +76,13 @@ typedef struct { double re, im; } pngf_c128;
                             history, flips of at most 32 components); otherwise the host */
 #define PNGF_LOOP_HOST 1 /* the host: one synchronisation per scored batch */
 
+/* arithmetic of the optimizer state (LOWRANK) */
+#define PNGF_PREC_C128 0 /* complex128 throughout (the reference's default build) */
+#define PNGF_PREC_C64  1 /* complex64, as the reference's USE_SINGLE_PRECISION build (main.cpp:58-72): G cast
+                            to c64 at init (pngfopt.hpp:128-156), C inverted by cgetrf + cgetri, W, the factors,
+                            the blocks, q and R in c64, capacitance solves in FP32, the strips on FP32 MFMA with
+                            FP32 accumulation; Eobj (ne scalars per frequency) and the objective stay FP64 */
+
 /* DBS visit order */
 #define PNGF_DRIVER_REF    0 /* MagicHat order, mirror-symmetric 2/4-tile flips (main.cpp:338-459) */
 #define PNGF_DRIVER_RASTER 1 /* raster order, single-tile flips, cyclic until one full cycle
@@ -105,6 +112,7 @@ typedef struct {
                              give better-conditioned C */
     int32_t symmetry;     /* LOWRANK: PNGF_SYM_AUTO or PNGF_SYM_OFF */
     int32_t factors;      /* LOWRANK: PNGF_FACTORS_* */
+    int32_t precision;    /* PNGF_PREC_* (PNGF_PREC_C64 needs PNGF_METHOD_LOWRANK) */
 } pngf_config;
 
 /* Creates a handle; selects the device and allocates nothing large yet.

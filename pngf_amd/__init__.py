@@ -22,6 +22,7 @@ ORDER_FIRST, ORDER_BEST = 0, 1
 SYM_AUTO, SYM_OFF = 0, 1
 FACTORS_AUTO, FACTORS_STORED, FACTORS_REBUILT = 0, 1, 2
 LOOP_AUTO, LOOP_HOST = 0, 1
+PREC_C128, PREC_C64 = 0, 1
 K_EVAL, K_COLBLK, K_ROWBLK, K_ACC_SMALL, K_VCINV, K_RANKK = range(6)
 K_NAMES = ["eval", "colblk", "rowblk", "acc_small", "vcinv", "rankk"]
 
@@ -57,7 +58,7 @@ class Config(ct.Structure):
                 ("device", ct.c_int32), ("tile_variant", ct.c_int32), ("dx", ct.c_double),
                 ("dy", ct.c_double), ("dz", ct.c_double), ("z0", ct.c_double),
                 ("mem_limit", ct.c_int64), ("synth_soft", ct.c_double), ("symmetry", ct.c_int32),
-                ("factors", ct.c_int32)]
+                ("factors", ct.c_int32), ("precision", ct.c_int32)]
 
 
 class DbsParams(ct.Structure):
@@ -255,10 +256,10 @@ class Optimizer:
     def __init__(self, layout, tiles, num_freq=1, num_eobj=1, max_flips=1, method=METHOD_LOWRANK,
                  objective=OBJ_S11, driver=DRIVER_RASTER, max_deferred=0, device=0, mem_limit=0,
                  dx=0.0, dy=0.0, dz=0.0, z0=50.0, cells=3, synth_soft=0.0, tile_variant=0, symmetry=0,
-                 factors=0):
+                 factors=0, precision=PREC_C128):
         self.cfg = Config(layout, tiles, tiles, cells, cells, num_eobj, max_flips, num_freq, method,
                           objective, driver, max_deferred, device, tile_variant, dx, dy, dz, z0, mem_limit,
-                          synth_soft, symmetry, factors)
+                          synth_soft, symmetry, factors, precision)
         self.h = ct.c_void_p()
         rc = lib().pngf_create(ct.byref(self.cfg), ct.byref(self.h))
         if rc != 0:

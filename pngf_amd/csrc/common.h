@@ -31,6 +31,46 @@ __host__ __device__ inline zd zrecip(zd b) {
 __host__ __device__ inline double zabs1(zd a) { return fabs(a.x) + fabs(a.y); }
 __host__ __device__ inline double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
 
+// complex64: the state and arithmetic of the single-precision mode (the reference's
+// USE_SINGLE_PRECISION build, pngf-opt/main.cpp:58-72), same operations as the zd ones
+typedef float2 zf;
+__host__ __device__ inline zf zmkf(float x, float y) { zf r; r.x = x; r.y = y; return r; }
+__host__ __device__ inline zf zadd(zf a, zf b) { return zmkf(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline zf zsub(zf a, zf b) { return zmkf(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline zf zmul(zf a, zf b) { return zmkf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline zf zscale(zf a, float s) { return zmkf(a.x * s, a.y * s); }
+__host__ __device__ inline zf zfma(zf a, zf b, zf acc) {
+    return zmkf(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline zf zfms(zf a, zf b, zf acc) {
+    return zmkf(acc.x - a.x * b.x + a.y * b.y, acc.y - a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline zf zdiv(zf a, zf b) {
+    float d = b.x * b.x + b.y * b.y;
+    return zmkf((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__host__ __device__ inline zf zrecip(zf b) {
+    float d = b.x * b.x + b.y * b.y;
+    return zmkf(b.x / d, -b.y / d);
+}
+__host__ __device__ inline float zabs1(zf a) { return fabsf(a.x) + fabsf(a.y); }
+__host__ __device__ inline float zabs2(zf a) { return a.x * a.x + a.y * a.y; }
+
+// The real type of a complex type, and a constructor usable in code generic over both.
+template <class Z> struct zreal;
+template <> struct zreal<zd> { typedef double type; };
+template <> struct zreal<zf> { typedef float type; };
+template <class Z>
+__host__ __device__ inline Z zc(typename zreal<Z>::type x, typename zreal<Z>::type y) {
+    Z r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <class Z> __host__ __device__ inline Z zneg(Z a) { return zc<Z>(-a.x, -a.y); }
+__host__ __device__ inline zd zwide(zd a) { return a; }
+__host__ __device__ inline zd zwide(zf a) { return zmk(a.x, a.y); }
+
 // Per-frequency constants read by the objective.
 struct FreqConst {
     zd vinc, ln1, ln2;

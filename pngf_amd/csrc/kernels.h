@@ -4,33 +4,50 @@
 
 namespace pngf {
 
-// Per-frequency pointer tables live in device memory (arrays of nf pointers).
-struct DevTabs {
-    zd** W0;     // LOWRANK: N x ldN row-major
-    zd** Aall;   // LOWRANK: N x Kmax row-major
-    zd** Mall;   // LOWRANK: Kmax x ldN row-major (null when the factors are kept as Lam)
-    zd** Grow;   // PLAIN: G row-major
-    zd** Cinv;   // PLAIN: C^-1 column-major
+// Per-frequency pointer tables live in device memory (arrays of nf pointers).  Z is the complex
+// type of the state: zd (complex128, the default) or zf (complex64 precision mode, LOWRANK).
+template <class Z>
+struct DevTabsT {
+    Z** W0;      // LOWRANK: N x ldN row-major
+    Z** Aall;    // LOWRANK: N x Kmax row-major
+    Z** Mall;    // LOWRANK: Kmax x ldN row-major (null when the factors are kept as Lam)
+    Z** Grow;    // PLAIN: G row-major
+    Z** Cinv;    // PLAIN: C^-1 column-major
     // LOWRANK, symmetric W, groups of <= kLdsGmax components: M is not stored.  Row K' of M
     // comes from the accept that appended A column K' (columns fstart[K'] .. + flen[K'] - 1):
     //   M[K'][n] = sum_l Lam[K'][l] A[n][fstart[K'] + l],  Lam[K'] = s_c cap^-1[c][:]
     // (c = K' - fstart[K']), which is M_new = diag(s) cap^-1 W[t,:] with W[t,:] = A_new^T.
-    zd** Lam;          // [Kmax][gmax] per frequency
+    Z** Lam;           // [Kmax][gmax] per frequency
     int32_t* fstart;   // [Kmax]
     int32_t* flen;     // [Kmax]
 };
+typedef DevTabsT<zd> DevTabs;
 
-// Contiguous per-frequency small arrays.
-struct DevVecs {
-    zd* q;       // [nf][ldN]    LOWRANK: W PEinc
-    zd* S;       // [nf][ldN]    PLAIN:   C^-1 PEinc
-    zd* R;       // [nf][ne][ldN]
-    zd* Gobj;    // [nf][ne][ldN]
-    zd* Einc;    // [nf][ldN]
+// Contiguous per-frequency small arrays.  Eobj (ne scalars per frequency, accumulated over every
+// accept) and the objective stay complex128 in both precision modes.
+template <class Z>
+struct DevVecsT {
+    Z* q;        // [nf][ldN]    LOWRANK: W PEinc
+    Z* S;        // [nf][ldN]    PLAIN:   C^-1 PEinc
+    Z* R;        // [nf][ne][ldN]
+    Z* Gobj;     // [nf][ne][ldN]
+    Z* Einc;     // [nf][ldN]
     zd* Eobj;    // [nf][ne]
-    zd* Dblk;    // [nf][ngroups][gmax][gmax]  LOWRANK blocks of W over group components
+    Z* Dblk;     // [nf][ngroups][gmax][gmax]  LOWRANK blocks of W over group components
     FreqConst* fc;  // [nf]
 };
+typedef DevVecsT<zd> DevVecs;
+
+// The same device memory viewed with another complex type (a handle in complex64 mode allocates
+// its state at 8 bytes per complex and keeps the pointers in its zd-typed tables).
+template <class Z>
+inline DevTabsT<Z> tabs_as(const DevTabs& t) {
+    return DevTabsT<Z>{(Z**)t.W0, (Z**)t.Aall, (Z**)t.Mall, (Z**)t.Grow, (Z**)t.Cinv, (Z**)t.Lam, t.fstart, t.flen};
+}
+template <class Z>
+inline DevVecsT<Z> vecs_as(const DevVecs& v) {
+    return DevVecsT<Z>{(Z*)v.q, (Z*)v.S, (Z*)v.R, (Z*)v.Gobj, (Z*)v.Einc, v.Eobj, (Z*)v.Dblk, v.fc};
+}
 
 struct Dims {
     int N, ldN, nf, ne, ngroups, gmax, Kmax;
@@ -83,9 +100,10 @@ void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32
 // ---- eval
 // batches of at least this many (candidate, frequency) pairs are scored one pair per wave
 constexpr int kWaveEvalMin = 2048;
-void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+template <class Z>
+void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
-                    int32_t* out_info, zd* gscratch, const int32_t* nb_dev = nullptr);
+                    int32_t* out_info, Z* gscratch, const int32_t* nb_dev = nullptr);
 void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
                        const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
                        int nb, int nsplit, zd* partial, zd* out_eobj, int32_t* out_info,
@@ -97,27 +115,38 @@ void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, 
 // ---- LOWRANK accept (all frequencies in grid.y)
 // acc buffer per frequency (zd): capinv [0,k^2), scratch [k^2,2k^2), y [2k^2,+k),
 // w = dPE - s o y [2k^2+k,+k), newEobj [2k^2+2k,+4), Rt = R[:,t] [2k^2+2k+4,+ne*k)
-void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
-                           const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
-                           int32_t* info, zd* gscratch, const int32_t* dyn = nullptr,
+template <class Z>
+void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, int g, const int32_t* t,
+                           const int32_t* s, const int32_t* lp, int k, Z* acc, int acc_stride,
+                           int32_t* info, Z* gscratch, const int32_t* dyn = nullptr,
                            const int32_t* g_dev = nullptr);
-void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      zd* Mc, zd* Ar, int small_stride, const int32_t* dyn = nullptr);
+template <class Z>
+void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const int32_t* t, int k, int K,
+                      Z* Mc, Z* Ar, int small_stride, const int32_t* dyn = nullptr);
+// complex128: v_mfma_f64_16x16x4f64; complex64: v_mfma_f32_16x16x4f32
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Mc, int small_stride, int variant, const int32_t* dyn = nullptr);
+void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k, int K,
+                      const zf* Mc, int small_stride, int variant, const int32_t* dyn = nullptr);
 // Rows kk0 .. kk0 + nrows - 1 of M (as ld-strided rows of out, all N columns) from Lam and A
 // (the factors kept without M); one frequency f.
-void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, int kk0, int nrows, zd* out, int ld);
+template <class Z>
+void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, int f, int kk0, int nrows, Z* out, int ld);
 void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Ar, int small_stride, zd* Bt, size_t bt_stride, int variant,
                       const int32_t* dyn = nullptr);
-// number of tile-shape variants of the skinny products for flips of k components
-int skinny_variants(int k);
-void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
-                         size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
-                         zd* out_single, size_t out_stride, int out_row0, int ld_out,
-                         const zd* const* b_tab = nullptr, int b_col0 = 0);
-void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
+void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k, int K,
+                      const zf* Ar, int small_stride, zf* Bt, size_t bt_stride, int variant,
+                      const int32_t* dyn = nullptr);
+// number of tile-shape variants of the skinny products for flips of k components (c64: FP32 kernel)
+int skinny_variants(int k, bool c64 = false);
+template <class Z>
+void launch_apply_capinv(hipStream_t st, const Dims& d, const Z* acc, int acc_stride, const Z* Bt,
+                         size_t bt_stride, int ldb, int k, const int32_t* s, Z* const* out_tab,
+                         Z* out_single, size_t out_stride, int out_row0, int ld_out,
+                         const Z* const* b_tab = nullptr, int b_col0 = 0);
+template <class Z>
+void launch_lr_small(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, const Z* acc,
                      int acc_stride, int k, int K);
 // fused apply_capinv + lr_small + lr_blocks for gmax <= 32
 // left != null: partition layout (groups [0, ngroups_part) have disjoint component lists; `left`
@@ -132,11 +161,13 @@ struct FlipUpdate {
     const int32_t* t = nullptr;
     const int32_t* gtiles = nullptr;
 };
-void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
-                      const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
+template <class Z>
+void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt,
+                      const Z* acc, int acc_stride, const Z* Bt, size_t bt_stride, const int32_t* s, int k,
                       int K, const int32_t* dyn = nullptr, const int32_t* left = nullptr, int nleft = 0,
                       int ngroups_part = 0, FlipUpdate fu = FlipUpdate());
-void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
+template <class Z>
+void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt,
                       int k, int K);
 void launch_mask_update(hipStream_t st, int8_t* P, int32_t* tile, const int32_t* t, const int32_t* s, int k,
                         const int32_t* gtiles, int ntiles);
@@ -160,21 +191,33 @@ void launch_labels(hipStream_t st, const Dims& d, const DevVecs& v, const ObjPar
 // ---- init
 void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos /*[N][3]*/, double kr, zd scale,
                     double soft, int row_major, zd* out);
-void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* Gcm, zd* C);
-void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v);
+template <class Z>
+void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const Z* Gcm, Z* C);
+template <class Z>
+void launch_add_diag(hipStream_t st, int N, int ldN, Z* A, double v);
+// out (complex64, ld ldN) = the complex128 matrix in rounded to nearest (the reference's cast of
+// the file to c64, pngfopt.hpp:128-156)
+void launch_cast_c64(hipStream_t st, int N, int ldN, const zd* in, zf* out);
 void launch_transpose(hipStream_t st, int N, int ldN, const zd* in, zd* out);
 void launch_asym(hipStream_t st, int N, int ldN, const zd* G, unsigned long long* out);
 // G(i, j) += amp (u1 + i u2), uniform u from a hash of (seed, i, j): a non-reciprocal perturbation
 void launch_perturb(hipStream_t st, int N, int ldN, zd* G, double amp, unsigned long long seed);
 // *flag |= 1 when the column-major N x N matrix holds a NaN or an infinity (flag zeroed by the caller)
 void launch_nonfinite(hipStream_t st, int N, int ldN, const zd* G, int32_t* flag);
-void launch_symmetrize(hipStream_t st, int N, int ldN, zd* W);
-void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
-                      zd* einc);
-void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe);
-void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const zd* A, const zd* x, zd* out);
-void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabs& tb, GroupTab gt, int g, int K,
-                             zd* Dblk);
-void launch_extract_blocks(hipStream_t st, const Dims& d, const zd* W0, GroupTab gt, zd* Dblk_f);
+template <class Z>
+void launch_symmetrize(hipStream_t st, int N, int ldN, Z* W);
+template <class Z>
+void launch_port_sums(hipStream_t st, int N, int ldN, const Z* Gcm, const int32_t* port, int np, Z* gobj0,
+                      Z* einc);
+template <class Z>
+void launch_pe(hipStream_t st, int N, const int8_t* P, const Z* einc, Z* pe);
+// out[r] = sum_n A[r][n] x[n] in the arithmetic of Z, written as complex128
+template <class Z>
+void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const Z* A, const Z* x, zd* out);
+template <class Z>
+void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, GroupTab gt, int g, int K,
+                             Z* Dblk);
+template <class Z>
+void launch_extract_blocks(hipStream_t st, const Dims& d, const Z* W0, GroupTab gt, Z* Dblk_f);
 
 }  // namespace pngf

@@ -8,7 +8,13 @@
 // Both give the same newEobj in exact arithmetic; see DESIGN notes in the runtime.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace pngf {
+
+// Bit pattern of a non-negative real as an unsigned key that orders like the value.
+__device__ inline unsigned long long rbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ inline unsigned long long rbits(float x) { return (unsigned long long)__float_as_uint(x); }
 
 // ------------------------------------------------------------------ helpers --
 
@@ -44,19 +50,21 @@ __device__ int change_set(const GroupTab& gt, int g, const int32_t* tile, const 
 // izamax / zgetf2); rows are swapped across all ncol columns and the multipliers
 // l = a_ij * (1/a_jj) are stored below the diagonal.  piv[j] = pivot row.
 // All threads of the block call it.  Returns LAPACK-style info (0 or j+1).
-__device__ int lds_lu(zd* a, int lda, int k, int ncol, int* piv) {
+template <class Z>
+__device__ int lds_lu(Z* a, int lda, int k, int ncol, int* piv) {
+    typedef typename zreal<Z>::type R;
     const int tid = threadIdx.x, nth = blockDim.x;
     __shared__ int s_info;
     if (tid == 0) s_info = 0;
     for (int j = 0; j < k; j++) {
         if (tid < 64) {
-            double best = -1.0; int bi = 0x7fffffff;
+            R best = -1; int bi = 0x7fffffff;
             for (int i = j + tid; i < k; i += 64) {
-                double v = zabs1(a[i * lda + j]);
+                R v = zabs1(a[i * lda + j]);
                 if (v > best) { best = v; bi = i; }
             }
             for (int off = 32; off > 0; off >>= 1) {
-                double ov = __shfl_xor(best, off, 64);
+                R ov = __shfl_xor(best, off, 64);
                 int oi = __shfl_xor(bi, off, 64);
                 if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
             }
@@ -65,15 +73,15 @@ __device__ int lds_lu(zd* a, int lda, int k, int ncol, int* piv) {
         __syncthreads();
         const int p = piv[j];
         if (p != j)
-            for (int c = tid; c < ncol; c += nth) { zd x = a[j * lda + c]; a[j * lda + c] = a[p * lda + c]; a[p * lda + c] = x; }
+            for (int c = tid; c < ncol; c += nth) { Z x = a[j * lda + c]; a[j * lda + c] = a[p * lda + c]; a[p * lda + c] = x; }
         __syncthreads();
-        const zd pv = a[j * lda + j];
+        const Z pv = a[j * lda + j];
         if (pv.x == 0.0 && pv.y == 0.0) {
             if (tid == 0 && s_info == 0) s_info = j + 1;
             __syncthreads();
             continue;
         }
-        const zd r = zrecip(pv);
+        const Z r = zrecip(pv);
         for (int i = j + 1 + tid; i < k; i += nth) a[i * lda + j] = zmul(a[i * lda + j], r);
         __syncthreads();
         const int w = ncol - j - 1, h = k - j - 1;
@@ -88,12 +96,13 @@ __device__ int lds_lu(zd* a, int lda, int k, int ncol, int* piv) {
 }
 
 // Back substitution U x = c for the column `col` of the factored LDS matrix.
-__device__ void lds_backsolve(zd* a, int lda, int k, int col) {
+template <class Z>
+__device__ void lds_backsolve(Z* a, int lda, int k, int col) {
     const int tid = threadIdx.x, nth = blockDim.x;
     for (int j = k - 1; j >= 0; j--) {
         if (tid == 0) a[j * lda + col] = zdiv(a[j * lda + col], a[j * lda + j]);
         __syncthreads();
-        const zd xj = a[j * lda + col];
+        const Z xj = a[j * lda + col];
         for (int i = tid; i < j; i += nth) a[i * lda + col] = zfms(a[i * lda + j], xj, a[i * lda + col]);
         __syncthreads();
     }
@@ -103,19 +112,20 @@ __device__ void lds_backsolve(zd* a, int lda, int k, int col) {
 // (row interchanges, unit-lower forward and upper back substitution).  One thread per
 // column, each writing only its own column of X (row-major, leading dimension ldx),
 // so the columns need no synchronisation between them.
-__device__ void lu_inverse_cols(const zd* a, int lda, int k, const int* piv, zd* X, int ldx) {
+template <class Z>
+__device__ void lu_inverse_cols(const Z* a, int lda, int k, const int* piv, Z* X, int ldx) {
     for (int c = threadIdx.x; c < k; c += blockDim.x) {
-        for (int i = 0; i < k; i++) X[i * ldx + c] = zmk(i == c ? 1.0 : 0.0, 0.0);
+        for (int i = 0; i < k; i++) X[i * ldx + c] = zc<Z>(i == c ? 1 : 0, 0);
         for (int j = 0; j < k; j++) {
             const int p = piv[j];
-            if (p != j) { zd x = X[j * ldx + c]; X[j * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = x; }
+            if (p != j) { Z x = X[j * ldx + c]; X[j * ldx + c] = X[p * ldx + c]; X[p * ldx + c] = x; }
         }
         for (int j = 0; j < k; j++) {
-            const zd xj = X[j * ldx + c];
+            const Z xj = X[j * ldx + c];
             for (int i = j + 1; i < k; i++) X[i * ldx + c] = zfms(a[i * lda + j], xj, X[i * ldx + c]);
         }
         for (int j = k - 1; j >= 0; j--) {
-            const zd xj = zdiv(X[j * ldx + c], a[j * lda + j]);
+            const Z xj = zdiv(X[j * ldx + c], a[j * lda + j]);
             X[j * ldx + c] = xj;
             for (int i = 0; i < j; i++) X[i * ldx + c] = zfms(a[i * lda + j], xj, X[i * ldx + c]);
         }
@@ -127,11 +137,13 @@ __device__ void lu_inverse_cols(const zd* a, int lda, int k, const int* piv, zd*
 // symmetric W the block is read as (D + D^T) / 2, which removes the rounding asymmetry that the
 // eager block updates accumulate (a symmetric-representation Woodbury update fed an asymmetric
 // block amplifies that asymmetry at every near-singular capacitance matrix).
-__device__ inline zd dblk(const zd* D, int gmax, int r, int c, int sym) {
-    const zd w = D[r * gmax + c];
+template <class Z>
+__device__ inline Z dblk(const Z* D, int gmax, int r, int c, int sym) {
+    typedef typename zreal<Z>::type R;
+    const Z w = D[r * gmax + c];
     if (!sym) return w;
-    const zd u = D[c * gmax + r];
-    return zmk(0.5 * (w.x + u.x), 0.5 * (w.y + u.y));
+    const Z u = D[c * gmax + r];
+    return zc<Z>(R(0.5) * (w.x + u.x), R(0.5) * (w.y + u.y));
 }
 
 // change_set computed by the 64 lanes of one wave (all of them call it): lane a tests
@@ -188,7 +200,8 @@ extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(u
 // the solution in columns k..ncol-1.  Returns LAPACK-style info (0, or j+1 for the first zero
 // pivot, whose elimination step is skipped).  All threads of the block call it, after a barrier
 // that follows the writes of a; blockDim.x is a multiple of 64.
-__device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
+template <class Z>
+__device__ int lds_gj(Z* a, int lda, int k, int ncol, int* perm) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int ri = tid & 31, cg = tid >> 5, ncg = blockDim.x >> 5;
     // every wave keeps its own copy of the row permutation (lane i: position i) and repeats the
@@ -201,7 +214,7 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
         // the value; + 1 puts every candidate above the lanes outside [j, k) (key 0), and the
         // first lane holding the maximum key is the first logical row of the maximum
         const unsigned long long key =
-            lane >= j && lane < k ? (unsigned long long)__double_as_longlong(zabs1(a[pm * lda + j])) + 1ull : 0ull;
+            lane >= j && lane < k ? rbits(zabs1(a[pm * lda + j])) + 1ull : 0ull;
         const unsigned long long mk = __ockl_wfred_max_u64(key);
         const int bi = __ffsll((unsigned long long)__ballot(key == mk)) - 1;
         const int pj = __builtin_amdgcn_readlane(pm, j), pb = __builtin_amdgcn_readlane(pm, bi);
@@ -209,12 +222,12 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
         else if (lane == bi) pm = pj;
         if (mk == 1ull && info == 0) info = j + 1;  // the maximum is 0
         const int pr = pb;
-        const zd pv = a[pr * lda + j];
-        if ((pv.x != 0.0 || pv.y != 0.0) && ri < k && ri != pr) {
+        const Z pv = a[pr * lda + j];
+        if ((pv.x != 0 || pv.y != 0) && ri < k && ri != pr) {
             // all reads of this thread's columns first, then the updates (rows pr and ri differ,
             // and only this thread writes row ri's columns j+1+cg, j+1+cg+ncg, ...)
             constexpr int U = 9;
-            zd pvv[U], rvv[U];
+            Z pvv[U], rvv[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int c = j + 1 + cg + u * ncg;
@@ -223,7 +236,7 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
                     rvv[u] = a[ri * lda + c];
                 }
             }
-            const zd l = zmul(a[ri * lda + j], zrecip(pv));
+            const Z l = zmul(a[ri * lda + j], zrecip(pv));
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int c = j + 1 + cg + u * ncg;
@@ -241,25 +254,29 @@ __device__ int lds_gj(zd* a, int lda, int k, int ncol, int* perm) {
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
 // gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
 // per-block slot of a global scratch buffer instead.
-static size_t lds_matrix_bytes(int gmax, const zd* gscratch) {
-    return gscratch ? 0 : (size_t)gmax * (2 * gmax + 1) * sizeof(zd);
+template <class Z>
+static size_t lds_matrix_bytes(int gmax, const Z* gscratch) {
+    return gscratch ? 0 : (size_t)gmax * (2 * gmax + 1) * sizeof(Z);
 }
 
 // --------------------------------------------------------- LOWRANK eval ------
 // One workgroup per (candidate b, frequency f): cap = I + D[t,t] diag(s),
 // rhs = q_t + D[t,t] dPE, y = cap^-1 rhs, newEobj = Eobj + R[:,t] (dPE - s y),
 // where dPE = -s Einc_t and D the maintained block of W over the group's components.
-__device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt, const int32_t* __restrict__ tile,
+template <class Z>
+__device__ void lr_eval_one(const Dims& d, const DevVecsT<Z>& v, const GroupTab& gt, const int32_t* __restrict__ tile,
                             const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
                             const int32_t* batch, zd* __restrict__ out_eobj, int32_t* __restrict__ out_info,
-                            zd* gscratch) {
-    extern __shared__ zd smz[];
+                            Z* gscratch) {
+    typedef typename zreal<Z>::type R;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    Z* smz = (Z*)smraw;
     const int b = blockIdx.y, f = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int gmax = d.gmax;
-    zd* dpe = smz;                                // [gmax]
-    zd* wdg = dpe + gmax;                         // [gmax] diagonal of the block
-    zd* qt = wdg + gmax;                          // [gmax] q[t]
-    zd* rt = qt + gmax;                           // [ne][gmax] R[:, t]
+    Z* dpe = smz;                                 // [gmax]
+    Z* wdg = dpe + gmax;                          // [gmax] diagonal of the block
+    Z* qt = wdg + gmax;                           // [gmax] q[t]
+    Z* rt = qt + gmax;                            // [ne][gmax] R[:, t]
     int* t = (int*)(rt + 3 * gmax);
     int* sg = t + gmax;
     int* lp = sg + gmax;
@@ -279,17 +296,17 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     // [k][k+1] capacitance | rhs: LDS when it fits, else this block's global slot.  The rest of
     // the candidate is written once for a matrix pointer and expanded for each case, so that
     // the LDS case compiles to LDS instructions (a pointer that may be either is a flat one).
-    auto rest = [&](zd* a) __attribute__((always_inline)) {
+    auto rest = [&](Z* a) __attribute__((always_inline)) {
     const int lda = k + 1;
-    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
-    const zd* q = v.q + (size_t)f * d.ldN;
-    const zd* einc = v.Einc + (size_t)f * d.ldN;
+    const Z* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    const Z* q = v.q + (size_t)f * d.ldN;
+    const Z* einc = v.Einc + (size_t)f * d.ldN;
     // every global read of the candidate in one phase: dPE, q_t, R[:, t] and the block (the
     // block gather: independent loads, several in flight per thread; the diagonal of W is kept
     // apart (wdg) so that the rhs below reads W exactly from LDS)
     for (int c = tid; c < k; c += nth) {
-        zd e = einc[t[c]];
-        dpe[c] = sg[c] > 0 ? zmk(-e.x, -e.y) : e;
+        Z e = einc[t[c]];
+        dpe[c] = sg[c] > 0 ? zc<Z>(-e.x, -e.y) : e;
         qt[c] = q[t[c]];
     }
     for (int e = tid; e < d.ne * k; e += nth) {
@@ -299,23 +316,23 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
 #pragma unroll 4
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
-        zd w = dblk(D, gmax, lp[r], lp[c], d.sym);
-        zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
+        Z w = dblk(D, gmax, lp[r], lp[c], d.sym);
+        Z val = sg[c] > 0 ? w : zc<Z>(-w.x, -w.y);
         if (r == c) {
             wdg[r] = w;
-            val.x += 1.0;
+            val.x += R(1);
         }
         a[r * lda + c] = val;
     }
     __syncthreads();
     // rhs_r = q_t_r + sum_c W[t_r][t_c] dPE_c, with W[t_r][t_c] = s_c a[r][c] off the diagonal
     for (int r = tid; r < k; r += nth) {
-        zd acc = qt[r];
+        Z acc = qt[r];
         for (int c = 0; c < k; c++) {
-            zd w = wdg[r];
+            Z w = wdg[r];
             if (c != r) {
-                const zd v = a[r * lda + c];
-                w = sg[c] > 0 ? v : zmk(-v.x, -v.y);
+                const Z v = a[r * lda + c];
+                w = sg[c] > 0 ? v : zc<Z>(-v.x, -v.y);
             }
             acc = zfma(w, dpe[c], acc);
         }
@@ -327,7 +344,7 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
         // Gauss-Jordan in LDS; y_c then goes to column k of storage row c (after a barrier, so
         // that no thread still reads the eliminated matrix)
         info = lds_gj(a, lda, k, k + 1, piv);
-        zd y = zmk(0.0, 0.0);
+        Z y = zc<Z>(0, 0);
         if (tid < k) y = zdiv(a[piv[tid] * lda + k], a[piv[tid] * lda + tid]);
         __syncthreads();
         if (tid < k) a[tid * lda + k] = y;
@@ -338,18 +355,29 @@ __device__ void lr_eval_one(const Dims& d, const DevVecs& v, const GroupTab& gt,
     }
     if (tid < d.ne) {
         const int r = tid;
-        zd acc = v.Eobj[f * d.ne + r];
-        for (int c = 0; c < k; c++) {
-            zd y = a[c * lda + k];
-            zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
-            acc = zfma(rt[r * gmax + c], w, acc);
+        if constexpr (std::is_same<Z, zd>::value) {
+            zd acc = v.Eobj[f * d.ne + r];
+            for (int c = 0; c < k; c++) {
+                zd y = a[c * lda + k];
+                zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+                acc = zfma(rt[r * gmax + c], w, acc);
+            }
+            out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
+        } else {
+            // complex64: the correction in single precision, added to the complex128 Eobj
+            Z acc = zc<Z>(0, 0);
+            for (int c = 0; c < k; c++) {
+                Z y = a[c * lda + k];
+                Z w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+                acc = zfma(rt[r * gmax + c], w, acc);
+            }
+            out_eobj[((size_t)b * d.nf + f) * d.ne + r] = zadd(v.Eobj[f * d.ne + r], zwide(acc));
         }
-        out_eobj[((size_t)b * d.nf + f) * d.ne + r] = acc;
     }
     if (tid == 0 && out_info) out_info[b * d.nf + f] = info;
     };
     if (!gscratch)
-        rest((zd*)(piv + gmax));
+        rest((Z*)(piv + gmax));
     else
         rest(gscratch + ((size_t)b * gridDim.x + f) * gmax * (2 * gmax + 1));
 }
@@ -361,10 +389,11 @@ __device__ void loop_next_batch(LoopState* ls, int* B_out, int* pos_out, int* nc
 
 // One workgroup per (candidate b, frequency f); workgroups past the batch size (read from
 // nb_dev) return at once.
-__global__ void lr_eval_kernel(Dims d, DevVecs v, GroupTab gt, const int32_t* __restrict__ tile,
+template <class Z>
+__global__ void lr_eval_kernel(Dims d, DevVecsT<Z> v, GroupTab gt, const int32_t* __restrict__ tile,
                                const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
                                const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
-                               int32_t* __restrict__ out_info, zd* gscratch, const int32_t* nb_dev) {
+                               int32_t* __restrict__ out_info, Z* gscratch, const int32_t* nb_dev) {
     if (nb_dev && (int)blockIdx.y >= *nb_dev) return;
     lr_eval_one(d, v, gt, tile, P, owners, batch, out_eobj, out_info, gscratch);
 }
@@ -437,6 +466,10 @@ __device__ inline zd readlane_z(zd x, int src) {
                                   ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(ry >> 32), src) << 32);
     return zmk(__longlong_as_double((long long)ox), __longlong_as_double((long long)oy));
 }
+__device__ inline zf readlane_z(zf x, int src) {
+    return zmkf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.x), src)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.y), src)));
+}
 
 // LOWRANK scoring of one (candidate b, frequency f) per wave, for flips of at most G <= 32
 // components: lane r holds row r of [cap | rhs] in registers (cap = I + D[t,t] diag(s), D the
@@ -445,25 +478,26 @@ __device__ inline zd readlane_z(zd x, int src) {
 // rows stay in their lanes; pos is the lane's logical row, the order zgetf2's interchanges
 // produce.  The operations and their order are those of lr_eval_one with lds_gj, so the scores
 // are bitwise the same.
-template <int G>
-__global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, GroupTab gt,
+template <class Z, int G>
+__global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecsT<Z> v, GroupTab gt,
                                                           const int32_t* __restrict__ tile,
                                                           const int8_t* __restrict__ P,
                                                           const int32_t* __restrict__ owners,
                                                           const int32_t* __restrict__ batch, zd* __restrict__ out_eobj,
                                                           int32_t* __restrict__ out_info, const int32_t* nb_dev) {
+    typedef typename zreal<Z>::type R;
     const int b = blockIdx.y, f = blockIdx.x, lane = threadIdx.x;
     if (nb_dev && b >= *nb_dev) return;
-    __shared__ zd dpe[G], qt[G], rt[3 * G], ys[G];
+    __shared__ Z dpe[G], qt[G], rt[3 * G], ys[G];
     __shared__ int t[G], sg[G], lp[G];
     const int gmax = d.gmax;
     const int g = batch[b];
     const int k = change_set_wave(gt, g, tile, P, owners, t, sg, lp);
     __syncthreads();
-    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    const Z* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
     if (lane < k) {
-        const zd e = v.Einc[(size_t)f * d.ldN + t[lane]];
-        dpe[lane] = sg[lane] > 0 ? zmk(-e.x, -e.y) : e;
+        const Z e = v.Einc[(size_t)f * d.ldN + t[lane]];
+        dpe[lane] = sg[lane] > 0 ? zc<Z>(-e.x, -e.y) : e;
         qt[lane] = v.q[(size_t)f * d.ldN + t[lane]];
     }
     for (int e = lane; e < d.ne * k; e += 64) {
@@ -472,31 +506,31 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, Gro
     }
     const bool row = lane < k;
     const int lr = row ? lp[lane] : 0;
-    zd a[G];
-    zd wd = zmk(0.0, 0.0);  // W[t_r][t_r]
+    Z a[G];
+    Z wd = zc<Z>(0, 0);  // W[t_r][t_r]
 #pragma unroll
     for (int c = 0; c < G; c++) {
-        a[c] = zmk(0.0, 0.0);
+        a[c] = zc<Z>(0, 0);
         if (row && c < k) {
-            const zd w = dblk(D, gmax, lr, lp[c], d.sym);
-            zd val = sg[c] > 0 ? w : zmk(-w.x, -w.y);
+            const Z w = dblk(D, gmax, lr, lp[c], d.sym);
+            Z val = sg[c] > 0 ? w : zc<Z>(-w.x, -w.y);
             if (c == lane) {
                 wd = w;
-                val.x += 1.0;
+                val.x += R(1);
             }
             a[c] = val;
         }
     }
     __syncthreads();
     // rhs_r = q_t_r + sum_c W[t_r][t_c] dPE_c, with W[t_r][t_c] = s_c cap[r][c] off the diagonal
-    zd bv = zmk(0.0, 0.0);
+    Z bv = zc<Z>(0, 0);
     if (row) {
         bv = qt[lane];
 #pragma unroll
         for (int c = 0; c < G; c++) {
             if (c < k) {
-                zd w = wd;
-                if (c != lane) w = sg[c] > 0 ? a[c] : zmk(-a[c].x, -a[c].y);
+                Z w = wd;
+                if (c != lane) w = sg[c] > 0 ? a[c] : zc<Z>(-a[c].x, -a[c].y);
                 bv = zfma(w, dpe[c], bv);
             }
         }
@@ -509,32 +543,32 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, Gro
             // pattern of a non-negative double orders like its value; + 1 keeps every row above
             // the lanes that take no part)
             const unsigned long long key =
-                row && pos >= j ? (unsigned long long)__double_as_longlong(zabs1(a[j])) + 1ull : 0ull;
+                row && pos >= j ? rbits(zabs1(a[j])) + 1ull : 0ull;
             const unsigned long long mk = __ockl_wfred_max_u64(key);
             const unsigned int mp = __ockl_wfred_min_u32(key == mk ? (unsigned)pos : 0xffffffffu);
             const int pr = __ffsll((unsigned long long)__ballot(row && pos == (int)mp)) - 1;
             if (pos == j) pos = (int)mp;
             else if (pos == (int)mp) pos = j;
             if (mk == 1ull && info == 0) info = j + 1;  // the maximum is 0
-            const zd pv = readlane_z(a[j], pr);
-            if (pv.x != 0.0 || pv.y != 0.0) {
+            const Z pv = readlane_z(a[j], pr);
+            if (pv.x != 0 || pv.y != 0) {
                 const bool upd = row && lane != pr;
-                const zd l = zmul(a[j], zrecip(pv));
+                const Z l = zmul(a[j], zrecip(pv));
 #pragma unroll
                 for (int c = j + 1; c < G; c++) {
                     if (c < k) {
-                        const zd pc = readlane_z(a[c], pr);
+                        const Z pc = readlane_z(a[c], pr);
                         if (upd) a[c] = zfms(l, pc, a[c]);
                     }
                 }
-                const zd pb = readlane_z(bv, pr);
+                const Z pb = readlane_z(bv, pr);
                 if (upd) bv = zfms(l, pb, bv);
             }
         }
     }
     // this lane's storage row is logical row pos: d_pos in column pos, d_pos y_pos in the rhs
     if (row) {
-        zd dd = a[0];
+        Z dd = a[0];
 #pragma unroll
         for (int c = 1; c < G; c++)
             if (c == pos) dd = a[c];
@@ -542,20 +576,31 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecs v, Gro
     }
     __syncthreads();
     if (lane < d.ne) {
-        zd acc = v.Eobj[f * d.ne + lane];
-        for (int c = 0; c < k; c++) {
-            const zd y = ys[c];
-            const zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
-            acc = zfma(rt[lane * G + c], w, acc);
+        if constexpr (std::is_same<Z, zd>::value) {
+            zd acc = v.Eobj[f * d.ne + lane];
+            for (int c = 0; c < k; c++) {
+                const zd y = ys[c];
+                const zd w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+                acc = zfma(rt[lane * G + c], w, acc);
+            }
+            out_eobj[((size_t)b * d.nf + f) * d.ne + lane] = acc;
+        } else {
+            Z acc = zc<Z>(0, 0);
+            for (int c = 0; c < k; c++) {
+                const Z y = ys[c];
+                const Z w = sg[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
+                acc = zfma(rt[lane * G + c], w, acc);
+            }
+            out_eobj[((size_t)b * d.nf + f) * d.ne + lane] = zadd(v.Eobj[f * d.ne + lane], zwide(acc));
         }
-        out_eobj[((size_t)b * d.nf + f) * d.ne + lane] = acc;
     }
     if (lane == 0 && out_info) out_info[b * d.nf + f] = info;
 }
 
-void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt, const int32_t* tile,
+template <class Z>
+void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
-                    int32_t* out_info, zd* gscratch, const int32_t* nb_dev) {
+                    int32_t* out_info, Z* gscratch, const int32_t* nb_dev) {
     // frequency fastest in dispatch order: with a device-resident batch size the workgroups of
     // the batch's candidates are dispatched first, the ones past it (which return at once) after
     dim3 grid(d.nf, nb);
@@ -566,18 +611,18 @@ void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecs& v, GroupTab gt
     // workgroup kernel.  Both give bitwise the same scores.
     const bool wave = !gscratch && d.gmax <= 32 && !nb_dev && (int64_t)nb * d.nf >= kWaveEvalMin;
     if (wave && d.gmax <= 24) {
-        hipLaunchKernelGGL(lr_eval_wave_kernel<24>, grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch, out_eobj,
-                           out_info, nb_dev);
+        hipLaunchKernelGGL((lr_eval_wave_kernel<Z, 24>), grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch,
+                           out_eobj, out_info, nb_dev);
         return;
     }
     if (wave) {
-        hipLaunchKernelGGL(lr_eval_wave_kernel<32>, grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch, out_eobj,
-                           out_info, nb_dev);
+        hipLaunchKernelGGL((lr_eval_wave_kernel<Z, 32>), grid, dim3(64), 0, st, d, v, gt, tile, P, owners, batch,
+                           out_eobj, out_info, nb_dev);
         return;
     }
     int nth = d.gmax <= 32 ? 128 : 256;
-    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
-    hipLaunchKernelGGL(lr_eval_kernel, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
+    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(Z) + lds_matrix_bytes(d.gmax, gscratch);
+    hipLaunchKernelGGL(lr_eval_kernel<Z>, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
                        out_info, gscratch, nb_dev);
 }
 
@@ -844,10 +889,12 @@ void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, 
 //   [2k*k+k, 2k*k+2k)   w = dPE - s o y
 //   [2k*k+2k, +ne)      newEobj
 //   [2k*k+2k+4, +ne*k)  Rt = R[:, t] (snapshot)
-__global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restrict__ tt,
-                               const int32_t* __restrict__ ss, const int32_t* __restrict__ lpp, int k, zd* acc,
-                               int acc_stride, int32_t* info_out, zd* gscratch, const int32_t* dyn,
+template <class Z>
+__global__ void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __restrict__ tt,
+                               const int32_t* __restrict__ ss, const int32_t* __restrict__ lpp, int k, Z* acc,
+                               int acc_stride, int32_t* info_out, Z* gscratch, const int32_t* dyn,
                                const int32_t* g_dev) {
+    typedef typename zreal<Z>::type R;
     extern __shared__ char smem[];
     if (dyn) {
         if (!dyn[kDynActive]) return;
@@ -857,41 +904,41 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     const int f = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int gmax = d.gmax, ne = d.ne;
     int* piv = (int*)smem;
-    zd* dpe = (zd*)(piv + 100);
-    zd* tmp = dpe + 96;
+    Z* dpe = (Z*)(piv + 100);
+    Z* tmp = dpe + 96;
     // the matrix: LDS for k <= 32, else this frequency's global slot (the body is expanded for
     // each, so that the LDS case compiles to LDS instructions)
-    auto rest = [&](zd* a) __attribute__((always_inline)) {
+    auto rest = [&](Z* a) __attribute__((always_inline)) {
     // k <= 32 (LDS): Gauss-Jordan on [cap | rhs | I]; larger k: LU, back substitution, inverse
     const bool gj = gscratch == nullptr;
     const int lda = gj ? 2 * k + 1 : k + 1;
-    const zd* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
-    const zd* q = v.q + (size_t)f * d.ldN;
-    const zd* einc = v.Einc + (size_t)f * d.ldN;
-    zd* st = acc + (size_t)f * acc_stride;
-    for (int c = tid; c < k; c += nth) { zd e = einc[tt[c]]; dpe[c] = ss[c] > 0 ? zmk(-e.x, -e.y) : e; }
+    const Z* D = v.Dblk + ((size_t)f * d.ngroups + g) * gmax * gmax;
+    const Z* q = v.q + (size_t)f * d.ldN;
+    const Z* einc = v.Einc + (size_t)f * d.ldN;
+    Z* st = acc + (size_t)f * acc_stride;
+    for (int c = tid; c < k; c += nth) { Z e = einc[tt[c]]; dpe[c] = ss[c] > 0 ? zc<Z>(-e.x, -e.y) : e; }
     __syncthreads();
-    zd* wdg = tmp;  // [k] W[t_r][t_r]
+    Z* wdg = tmp;  // [k] W[t_r][t_r]
 #pragma unroll 4
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
-        zd w = dblk(D, gmax, lpp[r], lpp[c], d.sym);
-        zd val = ss[c] > 0 ? w : zmk(-w.x, -w.y);
+        Z w = dblk(D, gmax, lpp[r], lpp[c], d.sym);
+        Z val = ss[c] > 0 ? w : zc<Z>(-w.x, -w.y);
         if (r == c) {
             wdg[r] = w;
-            val.x += 1.0;
+            val.x += R(1);
         }
         a[r * lda + c] = val;
     }
     __syncthreads();
     // rhs from the LDS copy of the block (W[t_r][t_c] = s_c a[r][c] off the diagonal)
     for (int r = tid; r < k; r += nth) {
-        zd s = q[tt[r]];
+        Z s = q[tt[r]];
         for (int c = 0; c < k; c++) {
-            zd w = wdg[r];
+            Z w = wdg[r];
             if (c != r) {
-                const zd v = a[r * lda + c];
-                w = ss[c] > 0 ? v : zmk(-v.x, -v.y);
+                const Z v = a[r * lda + c];
+                w = ss[c] > 0 ? v : zc<Z>(-v.x, -v.y);
             }
             s = zfma(w, dpe[c], s);
         }
@@ -900,7 +947,7 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     if (gj)
         for (int e = tid; e < k * k; e += nth) {
             const int r = e / k, c = e - r * k;
-            a[r * lda + k + 1 + c] = zmk(r == c ? 1.0 : 0.0, 0.0);
+            a[r * lda + k + 1 + c] = zc<Z>(r == c ? 1 : 0, 0);
         }
     __syncthreads();
     int info;
@@ -911,7 +958,7 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         lds_backsolve(a, lda, k, k);
     }
     for (int c = tid; c < k; c += nth) {
-        zd y = gj ? zdiv(a[piv[c] * lda + k], a[piv[c] * lda + c]) : a[c * lda + k];
+        Z y = gj ? zdiv(a[piv[c] * lda + k], a[piv[c] * lda + c]) : a[c * lda + k];
         st[2 * k * k + c] = y;
         st[2 * k * k + k + c] = ss[c] > 0 ? zsub(dpe[c], y) : zadd(dpe[c], y);
     }
@@ -922,13 +969,21 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     __syncthreads();
     if (tid < ne) {
         const int r = tid;
-        zd s = v.Eobj[f * ne + r];
-        for (int c = 0; c < k; c++) s = zfma(st[2 * k * k + 2 * k + 4 + r * k + c], st[2 * k * k + k + c], s);
-        st[2 * k * k + 2 * k + r] = s;
-        v.Eobj[f * ne + r] = s;
+        if constexpr (std::is_same<Z, zd>::value) {
+            zd s = v.Eobj[f * ne + r];
+            for (int c = 0; c < k; c++) s = zfma(st[2 * k * k + 2 * k + 4 + r * k + c], st[2 * k * k + k + c], s);
+            st[2 * k * k + 2 * k + r] = s;
+            v.Eobj[f * ne + r] = s;
+        } else {
+            Z s = zc<Z>(0, 0);
+            for (int c = 0; c < k; c++) s = zfma(st[2 * k * k + 2 * k + 4 + r * k + c], st[2 * k * k + k + c], s);
+            const zd e = zadd(v.Eobj[f * ne + r], zwide(s));
+            st[2 * k * k + 2 * k + r] = zc<Z>((R)e.x, (R)e.y);
+            v.Eobj[f * ne + r] = e;
+        }
     }
     // cap^-1: X[r][c] = a[perm r][k+1+c] / d_r after Gauss-Jordan, else from the LU factors
-    zd* X = a + k * lda;
+    Z* X = a + k * lda;
     if (!gj) lu_inverse_cols(a, lda, k, piv, X, k);
     auto xinv = [&](int r, int c) {
         return gj ? zdiv(a[piv[r] * lda + k + 1 + c], a[piv[r] * lda + r]) : X[r * k + c];
@@ -937,11 +992,11 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
     // stored exactly symmetric: X'[r][c] = (X[r][c] + s_r s_c X[c][r]) / 2, so diag(s) X' = sym(Lambda)
     for (int e = tid; e < k * k; e += nth) {
         const int r = e / k, c = e - r * k;
-        zd x = xinv(r, c);
+        Z x = xinv(r, c);
         if (d.sym) {
-            zd u = xinv(c, r);
-            if (ss[r] != ss[c]) u = zmk(-u.x, -u.y);
-            x = zmk(0.5 * (x.x + u.x), 0.5 * (x.y + u.y));
+            Z u = xinv(c, r);
+            if (ss[r] != ss[c]) u = zc<Z>(-u.x, -u.y);
+            x = zc<Z>(R(0.5) * (x.x + u.x), R(0.5) * (x.y + u.y));
         }
         st[e] = x;
     }
@@ -953,38 +1008,41 @@ __global__ void lr_prep_kernel(Dims d, DevVecs v, int g, const int32_t* __restri
         rest(gscratch + (size_t)f * gmax * (2 * gmax + 1));
 }
 
-void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecs& v, int g, const int32_t* t,
-                           const int32_t* s, const int32_t* lp, int k, zd* acc, int acc_stride,
-                           int32_t* info, zd* gscratch, const int32_t* dyn, const int32_t* g_dev) {
-    size_t sm = 100 * sizeof(int) + 192 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
-    hipLaunchKernelGGL(lr_prep_kernel, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
+template <class Z>
+void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, int g, const int32_t* t,
+                           const int32_t* s, const int32_t* lp, int k, Z* acc, int acc_stride,
+                           int32_t* info, Z* gscratch, const int32_t* dyn, const int32_t* g_dev) {
+    size_t sm = 100 * sizeof(int) + 192 * sizeof(Z) + lds_matrix_bytes(d.gmax, gscratch);
+    hipLaunchKernelGGL(lr_prep_kernel<Z>, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
                        acc_stride, info, gscratch, dyn, g_dev);
 }
 
 // M[kk][n]: stored, or from Lam and A (see DevTabs); the sum runs in the order the accept's
 // epilogue used when it stored M rows, with the sign folded into Lam.
-__device__ inline zd m_entry(const DevTabs& tb, const Dims& d, int f, int kk, int n) {
+template <class Z>
+__device__ inline Z m_entry(const DevTabsT<Z>& tb, const Dims& d, int f, int kk, int n) {
     if (!d.lam) return tb.Mall[f][(size_t)kk * d.ldN + n];
-    const zd* L = tb.Lam[f] + (size_t)kk * d.gmax;
-    const zd* A = tb.Aall[f] + (size_t)n * d.Kmax + tb.fstart[kk];
+    const Z* L = tb.Lam[f] + (size_t)kk * d.gmax;
+    const Z* A = tb.Aall[f] + (size_t)n * d.Kmax + tb.fstart[kk];
     const int len = tb.flen[kk];
-    zd s = zmk(0.0, 0.0);
+    Z s = zc<Z>(0, 0);
     for (int l = 0; l < len; l++) s = zfma(L[l], A[l], s);
     return s;
 }
 
 // Mc[K'][c] = M_all[K'][t_c] (K x k) and, unless Ar is null, Ar[c][K'] = A_all[t_c][K'] (k x K).
-__global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k, int K, zd* Mc,
-                                 zd* Ar, int small_stride, const int32_t* dyn) {
+template <class Z>
+__global__ void lr_gather_kernel(Dims d, DevTabsT<Z> tb, const int32_t* __restrict__ t, int k, int K, Z* Mc,
+                                 Z* Ar, int small_stride, const int32_t* dyn) {
     const int f = blockIdx.y;
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
         K = dyn[kDynKK];
     }
-    const zd* A = tb.Aall[f];
-    zd* mc = Mc + (size_t)f * small_stride;
-    zd* ar = Ar ? Ar + (size_t)f * small_stride : nullptr;
+    const Z* A = tb.Aall[f];
+    Z* mc = Mc + (size_t)f * small_stride;
+    Z* ar = Ar ? Ar + (size_t)f * small_stride : nullptr;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * k; e += gridDim.x * blockDim.x) {
         const int kk = e / k, c = e - kk * k;
         mc[e] = m_entry(tb, d, f, kk, t[c]);
@@ -999,22 +1057,24 @@ __global__ void lr_gather_kernel(Dims d, DevTabs tb, const int32_t* __restrict__
 // the rows' range, so they are staged in LDS for every t_c in the same phase as the Lam rows.
 // Same sums as m_entry.
 constexpr int kGatherRows = 32;
-__global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
-                                                           int K, zd* Mc, int small_stride, const int32_t* dyn) {
+template <class Z>
+__global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabsT<Z> tb, const int32_t* __restrict__ t,
+                                                           int k, int K, Z* Mc, int small_stride, const int32_t* dyn) {
     if (dyn) {
         if (!dyn[kDynActive]) return;
         k = dyn[kDynK];
         K = dyn[kDynKK];
     }
-    extern __shared__ zd sml[];
+    extern __shared__ __align__(16) unsigned char smlraw[];
+    Z* sml = (Z*)smlraw;
     const int f = blockIdx.y, tid = threadIdx.x, gmax = d.gmax;
     const int span = kGatherRows + 2 * (gmax - 1);  // A columns a workgroup can need
-    zd* Ls = sml;                                    // [kGatherRows][gmax] Lam rows
-    zd* As = sml + kGatherRows * gmax;               // [k][span] A[t_c][lo ..]
+    Z* Ls = sml;                                     // [kGatherRows][gmax] Lam rows
+    Z* As = sml + kGatherRows * gmax;                // [k][span] A[t_c][lo ..]
     __shared__ int s_fs[kGatherRows], s_len[kGatherRows];
-    const zd* A = tb.Aall[f];
-    const zd* Lam = tb.Lam[f];
-    zd* mc = Mc + (size_t)f * small_stride;
+    const Z* A = tb.Aall[f];
+    const Z* Lam = tb.Lam[f];
+    Z* mc = Mc + (size_t)f * small_stride;
     for (int r0 = blockIdx.x * kGatherRows; r0 < K; r0 += gridDim.x * kGatherRows) {
         const int nr = min(kGatherRows, K - r0);
         const int lo = max(0, r0 - (gmax - 1)), hi = min(K, r0 + nr + gmax - 1);
@@ -1033,37 +1093,40 @@ __global__ void __launch_bounds__(256) lr_gather_lam_kernel(Dims d, DevTabs tb, 
         for (int e = tid; e < nr * k; e += blockDim.x) {
             const int r = e / k, c = e - r * k;
             const int off = s_fs[r] - lo, len = s_len[r];
-            zd acc = zmk(0.0, 0.0);
+            Z acc = zc<Z>(0, 0);
             for (int l = 0; l < len; l++) acc = zfma(Ls[r * gmax + l], As[c * span + off + l], acc);
             mc[(size_t)(r0 + r) * k + c] = acc;
         }
     }
 }
 
-void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
-                      zd* Mc, zd* Ar, int small_stride, const int32_t* dyn) {
+template <class Z>
+void launch_lr_gather(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const int32_t* t, int k, int K,
+                      Z* Mc, Z* Ar, int small_stride, const int32_t* dyn) {
     if (!dyn && K == 0) return;
     if (d.lam) {
         const int rows = dyn ? d.Kmax : K;
         const int blocks = std::min(1024, (rows + kGatherRows - 1) / kGatherRows);
-        const size_t sm = ((size_t)kGatherRows * d.gmax + (size_t)d.gmax * (kGatherRows + 2 * (d.gmax - 1))) * sizeof(zd);
-        hipLaunchKernelGGL(lr_gather_lam_kernel, dim3(blocks, d.nf), dim3(256), sm, st, d, tb, t, k, K, Mc,
+        const size_t sm = ((size_t)kGatherRows * d.gmax + (size_t)d.gmax * (kGatherRows + 2 * (d.gmax - 1))) * sizeof(Z);
+        hipLaunchKernelGGL(lr_gather_lam_kernel<Z>, dim3(blocks, d.nf), dim3(256), sm, st, d, tb, t, k, K, Mc,
                            small_stride, dyn);
         return;
     }
     // grid-stride loop: with a device-resident K the grid is the largest one
     int blocks = dyn ? 1024 : min(1024, (K * k + 255) / 256);
-    hipLaunchKernelGGL(lr_gather_kernel, dim3(blocks, d.nf), dim3(256), 0, st, d, tb, t, k, K, Mc, Ar,
+    hipLaunchKernelGGL(lr_gather_kernel<Z>, dim3(blocks, d.nf), dim3(256), 0, st, d, tb, t, k, K, Mc, Ar,
                        small_stride, dyn);
 }
 
-__global__ void lr_build_m_kernel(Dims d, DevTabs tb, int f, int kk0, zd* out, int ld) {
+template <class Z>
+__global__ void lr_build_m_kernel(Dims d, DevTabsT<Z> tb, int f, int kk0, Z* out, int ld) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
     if (n < d.N) out[(size_t)r * ld + n] = m_entry(tb, d, f, kk0 + r, n);
 }
 
-void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabs& tb, int f, int kk0, int nrows, zd* out, int ld) {
-    hipLaunchKernelGGL(lr_build_m_kernel, dim3((d.N + 255) / 256, nrows), dim3(256), 0, st, d, tb, f, kk0, out, ld);
+template <class Z>
+void launch_lr_build_m(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, int f, int kk0, int nrows, Z* out, int ld) {
+    hipLaunchKernelGGL(lr_build_m_kernel<Z>, dim3((d.N + 255) / 256, nrows), dim3(256), 0, st, d, tb, f, kk0, out, ld);
 }
 
 // Skinny products over the deferred factors (k <= 96 columns), on FP64 MFMA
@@ -1292,7 +1355,7 @@ static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb,
 #undef PNGF_SKV
 }
 
-int skinny_variants(int k) { return k <= 24 ? 6 : 2; }
+int skinny_variants(int k, bool c64) { return c64 ? (k <= 24 ? 4 : 1) : (k <= 24 ? 6 : 2); }
 
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Mc, int small_stride, int variant, const int32_t* dyn) {
@@ -1303,99 +1366,309 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabs& tb, const in
     launch_skinny_mfma<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant, dyn);
 }
 
+// The same skinny products in complex64 on FP32 MFMA (v_mfma_f32_16x16x4f32, FP32 accumulation):
+// the complex product as the real one A_real x B_real of the FP64 kernel above, the staged B in
+// operand form, the register ring of A entries and the double-buffered stage, at 8 bytes per
+// complex.  Fragment layouts (16x16x4 f32, measured on gfx950 with a lane probe): A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i / 4), element i % 4 (the FP64
+// instruction's D layout differs: lane j + 16(i % 4), element i / 4).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const float gflt;
+typedef __attribute__((address_space(1))) float gflt_w;
+
+__device__ inline zf ldgf(const zf* p, size_t i) {
+    gflt* q = (gflt*)(p + i);
+    zf r;
+    r.x = q[0];
+    r.y = q[1];
+    return r;
+}
+
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
+__global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma32_kernel(Dims d, DevTabsT<zf> tb,
+                                                                      const int32_t* __restrict__ t, int k, int K,
+                                                                      const zf* __restrict__ small, int small_stride,
+                                                                      zf* Bt, size_t bt_stride, const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
+    constexpr int BW = CT * 8;   // complex columns held (>= k)
+    constexpr int NJ = CT * 16;  // real columns
+    constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64;
+    constexpr int NBUF = DB ? 2 : 1;
+    constexpr int PER = (KC * BW + NT - 1) / NT;
+    static_assert(NS % DEPTH == 0, "ring slots line up across chunks");
+    __shared__ zf Bp[NBUF][KC][NJ + 2];
+    const int f = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int n0 = (blockIdx.x * WAVES + wave) * 16 * RT;
+    int nc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) nc[r] = min(n0 + 16 * r + lr, d.N - 1);
+    const zf* A = tb.Aall[f];
+    const zf* M = tb.Mall[f];
+    const zf* W0 = tb.W0[f];
+    const zf* sm = small + (size_t)f * small_stride;
+    const int Kc = K - 1;
+    f4v acc[RT][CT];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[r][c] = f4v{0.f, 0.f, 0.f, 0.f};
+    auto ld = [&](int kb, zf (&a)[RT][2]) {
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int Kp = min(kb + 2 * lq + j, Kc);
+                a[r][j] = ROW ? ldgf(M, (size_t)Kp * d.ldN + nc[r]) : ldgf(A, (size_t)nc[r] * d.Kmax + Kp);
+            }
+    };
+    auto body = [&](const zf (&a)[RT][2], int kloc, int bsel) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            zf m[CT];
+#pragma unroll
+            for (int c = 0; c < CT; c++) m[c] = Bp[bsel][kloc + 2 * lq + j][c * 16 + lr];
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < CT; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][j].x, m[c].x, acc[r][c], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < CT; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][j].y, m[c].y, acc[r][c], 0, 0, 0);
+        }
+    };
+    auto bload = [&](int k0, zf (&r)[PER]) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            const int kk = e / BW, c = e - kk * BW;
+            r[u] = zmkf(0.f, 0.f);
+            if (e < KC * BW && k0 + kk < K && c < k)
+                r[u] = ldgf(sm, ROW ? (size_t)c * K + k0 + kk : (size_t)(k0 + kk) * k + c);
+        }
+    };
+    auto bstore = [&](const zf (&r)[PER], int bsel) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            if (e >= KC * BW) continue;
+            const int kk = e / BW, c = e - kk * BW;
+            Bp[bsel][kk][2 * c] = zmkf(r[u].x, -r[u].y);
+            Bp[bsel][kk][2 * c + 1] = zmkf(r[u].y, r[u].x);
+        }
+    };
+    zf buf[DEPTH][RT][2];
+    if (K > 0) {
+#pragma unroll
+        for (int q = 0; q < DEPTH; q++) ld(q * KS, buf[q]);
+    }
+    if (!DB) {
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            __syncthreads();
+            {
+                zf r[PER];
+                bload(k0, r);
+                bstore(r, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+                body(buf[i % DEPTH], i * KS, 0);
+                ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    } else if (K > 0) {
+        {
+            zf r[PER];
+            bload(0, r);
+            bstore(r, 0);
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            const bool more = k0 + KC < K;
+            zf r[PER];
+            if (more) bload(k0 + KC, r);
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+                body(buf[i % DEPTH], i * KS, cur);
+                ld(k0 + (i + DEPTH) * KS, buf[i % DEPTH]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (more) bstore(r, cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+        if (cc >= k) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = n0 + 16 * r + 4 * lq + v;
+            if (row >= d.N) continue;
+            const float a = acc[r][c][v];
+            if (!ROW) {
+                gflt* w = (gflt*)(W0 + (size_t)row * d.ldN + t[cc]);
+                ((gflt_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
+            } else {
+                gflt* w = (gflt*)(W0 + (size_t)t[cc] * d.ldN + row);
+                ((gflt_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
+            }
+        }
+    }
+}
+
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
+static void launch_variant32(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k, int K,
+                             const zf* small, int small_stride, zf* Bt, size_t bt_stride, const int32_t* dyn) {
+    const int rows = 16 * WAVES * RT;
+    dim3 grid((d.N + rows - 1) / rows, d.nf);
+    hipLaunchKernelGGL((lr_skinny_mfma32_kernel<CT, ROW, KC, DEPTH, WAVES, DB, RT>), grid, dim3(WAVES * 64), 0, st, d,
+                       tb, t, k, K, small, small_stride, Bt, bt_stride, dyn);
+}
+
+template <bool ROW>
+static void launch_skinny_mfma32(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k,
+                                 int K, const zf* small, int small_stride, zf* Bt, size_t bt_stride, int variant,
+                                 const int32_t* dyn) {
+#define PNGF_SKV32(CT, KC, DEPTH, W, DB, RT) \
+    launch_variant32<CT, ROW, KC, DEPTH, W, DB, RT>(st, d, tb, t, k, K, small, small_stride, Bt, bt_stride, dyn)
+    if (k <= 24) {
+        switch (variant) {
+            case 1: PNGF_SKV32(3, 16, 2, 8, true, 1); break;
+            case 2: PNGF_SKV32(3, 32, 2, 4, true, 2); break;
+            case 3: PNGF_SKV32(3, 64, 2, 8, true, 1); break;
+            default: PNGF_SKV32(3, 32, 2, 8, true, 1); break;
+        }
+    } else if (k <= 48) {
+        PNGF_SKV32(6, 16, 2, 8, true, 1);
+    } else {
+        PNGF_SKV32(12, 16, 2, 4, false, 1);
+    }
+#undef PNGF_SKV32
+}
+
+void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k, int K,
+                      const zf* Mc, int small_stride, int variant, const int32_t* dyn) {
+    launch_skinny_mfma32<false>(st, d, tb, t, k, K, Mc, small_stride, nullptr, 0, variant, dyn);
+}
+
+void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, const int32_t* t, int k, int K,
+                      const zf* Ar, int small_stride, zf* Bt, size_t bt_stride, int variant, const int32_t* dyn) {
+    launch_skinny_mfma32<true>(st, d, tb, t, k, K, Ar, small_stride, Bt, bt_stride, variant, dyn);
+}
+
 // out[c][n] = sgn_c * sum_l capinv[c][l] B[n][l]   (B row n: k entries at stride ldb;
 // B = b_tab[f] + b_col0 when b_tab is given, else B + f * b_stride)
 // PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
 // LOWRANK: M_new = diag(s) cap^-1 W[t,:]  (W[t,:]^T = W[:,t], the new A_all columns, when W
 //          is symmetric)
-__global__ void apply_capinv_kernel(Dims d, const zd* __restrict__ acc, int acc_stride, const zd* __restrict__ B,
-                                    size_t b_stride, const zd* const* b_tab, int b_col0, int ldb, int k,
-                                    const int32_t* __restrict__ s, zd* const* out_tab, zd* out_single,
+template <class Z>
+__global__ void apply_capinv_kernel(Dims d, const Z* __restrict__ acc, int acc_stride, const Z* __restrict__ B,
+                                    size_t b_stride, const Z* const* b_tab, int b_col0, int ldb, int k,
+                                    const int32_t* __restrict__ s, Z* const* out_tab, Z* out_single,
                                     size_t out_stride, int out_row0, int ld_out) {
-    __shared__ zd Bs[32][97];
+    __shared__ Z Bs[32][97];
     const int f = blockIdx.y, tid = threadIdx.x;
     const int n0 = blockIdx.x * 32;
-    const zd* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
-    const zd* ci = acc + (size_t)f * acc_stride;
+    const Z* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
+    const Z* ci = acc + (size_t)f * acc_stride;
     for (int e = tid; e < 32 * k; e += 256) {
         const int nn = e / k, l = e - nn * k;
         const int n = n0 + nn;
-        Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zmk(0.0, 0.0);
+        Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zc<Z>(0, 0);
     }
     __syncthreads();
-    zd* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
+    Z* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
     for (int e = tid; e < 32 * k; e += 256) {
         const int c = e / 32, nn = e - c * 32;
         const int n = n0 + nn;
         if (n >= d.N) continue;
-        zd sacc = zmk(0.0, 0.0);
+        Z sacc = zc<Z>(0, 0);
         for (int l = 0; l < k; l++) sacc = zfma(ci[c * k + l], Bs[nn][l], sacc);
-        if (s && s[c] < 0) sacc = zmk(-sacc.x, -sacc.y);
+        if (s && s[c] < 0) sacc = zc<Z>(-sacc.x, -sacc.y);
         out[(size_t)(out_row0 + c) * ld_out + n] = sacc;
     }
 }
 
-void launch_apply_capinv(hipStream_t st, const Dims& d, const zd* acc, int acc_stride, const zd* Bt,
-                         size_t bt_stride, int ldb, int k, const int32_t* s, zd* const* out_tab,
-                         zd* out_single, size_t out_stride, int out_row0, int ld_out, const zd* const* b_tab,
+template <class Z>
+void launch_apply_capinv(hipStream_t st, const Dims& d, const Z* acc, int acc_stride, const Z* Bt,
+                         size_t bt_stride, int ldb, int k, const int32_t* s, Z* const* out_tab,
+                         Z* out_single, size_t out_stride, int out_row0, int ld_out, const Z* const* b_tab,
                          int b_col0) {
     dim3 grid((d.N + 31) / 32, d.nf);
-    hipLaunchKernelGGL(apply_capinv_kernel, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, b_tab, b_col0,
-                       ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
+    hipLaunchKernelGGL(apply_capinv_kernel<Z>, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, b_tab,
+                       b_col0, ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
 }
 
 // q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept)
-__global__ void lr_small_kernel(Dims d, DevTabs tb, DevVecs v, const zd* __restrict__ acc, int acc_stride, int k,
-                                int K) {
+template <class Z>
+__global__ void lr_small_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, const Z* __restrict__ acc, int acc_stride,
+                                int k, int K) {
     const int f = blockIdx.y;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.N) return;
-    const zd* st = acc + (size_t)f * acc_stride;
-    const zd* w = st + 2 * k * k + k;
-    const zd* Rt = st + 2 * k * k + 2 * k + 4;
-    const zd* A = tb.Aall[f] + (size_t)n * d.Kmax + K;
-    const zd* M = tb.Mall[f] + (size_t)K * d.ldN + n;
-    zd qa = v.q[(size_t)f * d.ldN + n];
+    const Z* st = acc + (size_t)f * acc_stride;
+    const Z* w = st + 2 * k * k + k;
+    const Z* Rt = st + 2 * k * k + 2 * k + 4;
+    const Z* A = tb.Aall[f] + (size_t)n * d.Kmax + K;
+    const Z* M = tb.Mall[f] + (size_t)K * d.ldN + n;
+    Z qa = v.q[(size_t)f * d.ldN + n];
     for (int c = 0; c < k; c++) qa = zfma(A[c], w[c], qa);
     v.q[(size_t)f * d.ldN + n] = qa;
     for (int r = 0; r < d.ne; r++) {
-        zd* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
-        zd ra = R[n];
+        Z* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
+        Z ra = R[n];
         for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], M[(size_t)c * d.ldN], ra);
         R[n] = ra;
     }
 }
 
-void launch_lr_small(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, const zd* acc,
+template <class Z>
+void launch_lr_small(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, const Z* acc,
                      int acc_stride, int k, int K) {
     dim3 grid((d.N + 255) / 256, d.nf);
-    hipLaunchKernelGGL(lr_small_kernel, grid, dim3(256), 0, st, d, tb, v, acc, acc_stride, k, K);
+    hipLaunchKernelGGL(lr_small_kernel<Z>, grid, dim3(256), 0, st, d, tb, v, acc, acc_stride, k, K);
 }
 
 // D_h -= A_new[comp_h, :] M_new[:, comp_h] for every group h.
-template <int PER>
-__global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt, int k, int K) {
-    __shared__ zd Ac[96][17];
-    __shared__ zd Mc[16][97];
+template <class Z, int PER>
+__global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt, int k,
+                                                        int K) {
+    __shared__ Z Ac[96][17];
+    __shared__ Z Mc[16][97];
     const int h = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int nc = gt.ncomp[h];
     if (nc == 0) return;
     const int32_t* cl = gt.comps + (size_t)h * gt.gmax;
-    const zd* A = tb.Aall[f];
-    const zd* M = tb.Mall[f];
-    zd* D = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
-    zd acc[PER];
+    const Z* A = tb.Aall[f];
+    const Z* M = tb.Mall[f];
+    Z* D = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
+    Z acc[PER];
 #pragma unroll
-    for (int u = 0; u < PER; u++) acc[u] = zmk(0.0, 0.0);
+    for (int u = 0; u < PER; u++) acc[u] = zc<Z>(0, 0);
     for (int c0 = 0; c0 < k; c0 += 16) {
         const int cl_ = min(16, k - c0);
         for (int e = tid; e < nc * 16; e += 256) {
             const int a = e / 16, c = e - a * 16;
-            Ac[a][c] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zmk(0.0, 0.0);
+            Ac[a][c] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zc<Z>(0, 0);
         }
         for (int e = tid; e < 16 * nc; e += 256) {
             const int c = e / nc, bb = e - c * nc;
-            Mc[c][bb] = c < cl_ ? M[(size_t)(K + c0 + c) * d.ldN + cl[bb]] : zmk(0.0, 0.0);
+            Mc[c][bb] = c < cl_ ? M[(size_t)(K + c0 + c) * d.ldN + cl[bb]] : zc<Z>(0, 0);
         }
         __syncthreads();
 #pragma unroll
@@ -1403,7 +1676,7 @@ __global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabs tb, DevV
             const int e = tid + 256 * u;
             if (e < nc * nc) {
                 const int a = e / nc, bb = e - a * nc;
-                zd s = acc[u];
+                Z s = acc[u];
                 for (int c = 0; c < cl_; c++) s = zfma(Ac[a][c], Mc[c][bb], s);
                 acc[u] = s;
             }
@@ -1415,18 +1688,19 @@ __global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabs tb, DevV
         const int e = tid + 256 * u;
         if (e < nc * nc) {
             const int a = e / nc, bb = e - a * nc;
-            zd* p = &D[a * gt.gmax + bb];
+            Z* p = &D[a * gt.gmax + bb];
             *p = zsub(*p, acc[u]);
         }
     }
 }
 
-void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt, int k,
+template <class Z>
+void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt, int k,
                       int K) {
     if (gt.gmax <= 32)
-        hipLaunchKernelGGL((lr_blocks_kernel<4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+        hipLaunchKernelGGL((lr_blocks_kernel<Z, 4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
     else
-        hipLaunchKernelGGL((lr_blocks_kernel<36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+        hipLaunchKernelGGL((lr_blocks_kernel<Z, 36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
 }
 
 // LOWRANK accept epilogue for k <= 32 in one launch.  A block takes a list of up to 32
@@ -1442,10 +1716,25 @@ void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabs& tb, const De
 // disjoint layout's raster groups): block h < ngroups takes group h and does both parts, the
 // remaining blocks take 32 of the components outside every group (`left`).  Same operations in
 // the same order as apply_capinv, lr_small and lr_blocks.
-template <int G>
-__global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
-                                                        const zd* __restrict__ acc, int acc_stride,
-                                                        const zd* __restrict__ Bt, size_t bt_stride,
+// global-space stores of the (re, im) pair i of a complex array (a table entry is otherwise a
+// flat pointer, whose stores the LDS waits of the kernel below would also wait for)
+__device__ inline void gstore2(zd* base, size_t i, double x, double y) {
+    gdbl_w* p = (gdbl_w*)base;
+    p[2 * i] = x;
+    p[2 * i + 1] = y;
+}
+__device__ inline void gstore2(zf* base, size_t i, float x, float y) {
+    gflt_w* p = (gflt_w*)base;
+    p[2 * i] = x;
+    p[2 * i + 1] = y;
+}
+__device__ inline zd ldgz(const zd* p, size_t i) { return ldg(p, i); }
+__device__ inline zf ldgz(const zf* p, size_t i) { return ldgf(p, i); }
+
+template <class Z, int G>
+__global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt,
+                                                        const Z* __restrict__ acc, int acc_stride,
+                                                        const Z* __restrict__ Bt, size_t bt_stride,
                                                         const int32_t* __restrict__ s, int k, int K, int nchunk,
                                                         const int32_t* dyn, const int32_t* __restrict__ left,
                                                         int nleft, int ngroups_part, FlipUpdate fu) {
@@ -1459,9 +1748,9 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
         for (int c = threadIdx.x; c < k; c += 256) fu.P[fu.t[c]] = (int8_t)(fu.P[fu.t[c]] + s[c]);
         for (int i = threadIdx.x; i < dyn[kDynNt]; i += 256) fu.tile[fu.gtiles[i]] ^= 1;
     }
-    __shared__ zd Xs[G][G + 1];
-    __shared__ zd As[G][G + 1];
-    __shared__ zd Ss[G][G + 1];  // row strip, then M_new[c][row]
+    __shared__ Z Xs[G][G + 1];
+    __shared__ Z As[G][G + 1];
+    __shared__ Z Ss[G][G + 1];  // row strip, then M_new[c][row]
     __shared__ int rows[G];
     const int f = blockIdx.y, tid = threadIdx.x, bx = blockIdx.x;
     bool do_rows, do_block;
@@ -1487,16 +1776,14 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
             nrow = min(G, d.N - r0);
         }
     }
-    const zd* stf = acc + (size_t)f * acc_stride;
+    const Z* stf = acc + (size_t)f * acc_stride;
     if (d.lam && bx == 0) {
         // the accept's M rows, kept as Lam[K + c] = s_c cap^-1[c][:] (see DevTabs)
-        gdbl_w* L = (gdbl_w*)(tb.Lam[f] + (size_t)K * d.gmax);
+        Z* L = tb.Lam[f] + (size_t)K * d.gmax;
         for (int e = tid; e < k * k; e += 256) {
             const int c = e / k, l = e - c * k;
-            const zd x = stf[e];
-            const size_t o = ((size_t)c * d.gmax + l) * 2;
-            L[o] = s[c] < 0 ? -x.x : x.x;
-            L[o + 1] = s[c] < 0 ? -x.y : x.y;
+            const Z x = stf[e];
+            gstore2(L, (size_t)c * d.gmax + l, s[c] < 0 ? -x.x : x.x, s[c] < 0 ? -x.y : x.y);
         }
         if (f == 0 && tid < k) {
             tb.fstart[K + tid] = K;
@@ -1510,25 +1797,25 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     }
     if (tid < nrow) rows[tid] = h >= 0 ? gt.comps[(size_t)h * gt.gmax + tid] : (left ? left[r0 + tid] : r0 + tid);
     __syncthreads();
-    const zd* A = tb.Aall[f];
-    const zd* S = Bt + (size_t)f * bt_stride;
+    const Z* A = tb.Aall[f];
+    const Z* S = Bt + (size_t)f * bt_stride;
     for (int e = tid; e < nrow * k; e += 256) {
         const int rr = e / k, c = e - rr * k;
         const int n = rows[rr];
-        As[rr][c] = ldg(A, (size_t)n * d.Kmax + K + c);
+        As[rr][c] = ldgz(A, (size_t)n * d.Kmax + K + c);
         if (!d.sym) Ss[rr][c] = S[(size_t)n * d.gmax + c];
     }
     __syncthreads();
-    zd m[4];
+    Z m[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const int e = tid + 256 * u;
-        m[u] = zmk(0.0, 0.0);
+        m[u] = zc<Z>(0, 0);
         if (e < k * nrow) {
             const int c = e / nrow, rr = e - c * nrow;
-            zd sacc = zmk(0.0, 0.0);
+            Z sacc = zc<Z>(0, 0);
             for (int l = 0; l < k; l++) sacc = zfma(Xs[c][l], d.sym ? As[rr][l] : Ss[rr][l], sacc);
-            m[u] = s[c] < 0 ? zmk(-sacc.x, -sacc.y) : sacc;
+            m[u] = s[c] < 0 ? zc<Z>(-sacc.x, -sacc.y) : sacc;
         }
     }
     __syncthreads();
@@ -1543,59 +1830,56 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabs tb, DevV
     __syncthreads();
     if (do_rows) {
         if (!d.lam) {
-            // global-space stores (a table entry is otherwise a flat pointer, whose stores the
-            // LDS waits below would also wait for)
-            gdbl_w* M = (gdbl_w*)tb.Mall[f];
+            Z* M = tb.Mall[f];
             for (int e = tid; e < k * nrow; e += 256) {
                 const int c = e / nrow, rr = e - c * nrow;
-                const zd v = Ss[c][rr];
-                const size_t o = ((size_t)(K + c) * d.ldN + rows[rr]) * 2;
-                M[o] = v.x;
-                M[o + 1] = v.y;
+                const Z v = Ss[c][rr];
+                gstore2(M, (size_t)(K + c) * d.ldN + rows[rr], v.x, v.y);
             }
         }
-        const zd* w = stf + 2 * k * k + k;
-        const zd* Rt = stf + 2 * k * k + 2 * k + 4;
+        const Z* w = stf + 2 * k * k + k;
+        const Z* Rt = stf + 2 * k * k + 2 * k + 4;
         if (tid < nrow) {
-            zd* qp = v.q + (size_t)f * d.ldN + rows[tid];
-            zd qa = *qp;
+            Z* qp = v.q + (size_t)f * d.ldN + rows[tid];
+            Z qa = *qp;
             for (int c = 0; c < k; c++) qa = zfma(As[tid][c], w[c], qa);
             *qp = qa;
         } else if (tid >= 64 && tid < 64 + 32 * d.ne) {
             const int r = (tid - 64) >> 5, rr = (tid - 64) & 31;
             if (rr < nrow) {
-                zd* rp = v.R + ((size_t)f * d.ne + r) * d.ldN + rows[rr];
-                zd ra = *rp;
+                Z* rp = v.R + ((size_t)f * d.ne + r) * d.ldN + rows[rr];
+                Z ra = *rp;
                 for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], Ss[c][rr], ra);
                 *rp = ra;
             }
         }
     }
     if (do_block) {
-        zd* Dh = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
+        Z* Dh = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
         for (int e = tid; e < nrow * nrow; e += 256) {
             const int a = e / nrow, b = e - a * nrow;
-            zd sacc = zmk(0.0, 0.0);
+            Z sacc = zc<Z>(0, 0);
             for (int c = 0; c < k; c++) sacc = zfma(As[a][c], Ss[c][b], sacc);
             Dh[a * gt.gmax + b] = zsub(Dh[a * gt.gmax + b], sacc);
         }
     }
 }
 
-void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabs& tb, const DevVecs& v, GroupTab gt,
-                      const zd* acc, int acc_stride, const zd* Bt, size_t bt_stride, const int32_t* s, int k,
+template <class Z>
+void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt,
+                      const Z* acc, int acc_stride, const Z* Bt, size_t bt_stride, const int32_t* s, int k,
                       int K, const int32_t* dyn, const int32_t* left, int nleft, int ngroups_part, FlipUpdate fu) {
     // G: components per workgroup (the group size bound); k <= 24 halves the LDS footprint
     if (d.gmax <= 24) {
         const int nchunk = (d.N + 23) / 24;
         const int nblk = left ? ngroups_part + (nleft + 23) / 24 : nchunk + d.ngroups;
-        hipLaunchKernelGGL(lr_finish_kernel<24>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
-                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        hipLaunchKernelGGL((lr_finish_kernel<Z, 24>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                           acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     } else {
         const int nchunk = (d.N + 31) / 32;
         const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
-        hipLaunchKernelGGL(lr_finish_kernel<32>, dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc, acc_stride,
-                           Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        hipLaunchKernelGGL((lr_finish_kernel<Z, 32>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                           acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     }
 }
 
@@ -1915,24 +2199,28 @@ void launch_synth_g(hipStream_t st, int N, int ldN, const double* pos, double kr
 }
 
 // C = I - P + P G, column-major (pngfopt.hpp:196-233)
-__global__ void build_C_kernel(int N, int ldN, const int8_t* __restrict__ P, const zd* __restrict__ G, zd* C) {
+template <class Z>
+__global__ void build_C_kernel(int N, int ldN, const int8_t* __restrict__ P, const Z* __restrict__ G, Z* C) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
     if (row >= N) return;
     const size_t i = (size_t)col * ldN + row;
-    C[i] = P[row] ? G[i] : zmk(row == col ? 1.0 : 0.0, 0.0);
+    C[i] = P[row] ? G[i] : zc<Z>(row == col ? 1 : 0, 0);
 }
 
-void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const zd* Gcm, zd* C) {
-    hipLaunchKernelGGL(build_C_kernel, dim3((N + 255) / 256, N), dim3(256), 0, st, N, ldN, P, Gcm, C);
+template <class Z>
+void launch_build_C(hipStream_t st, int N, int ldN, const int8_t* P, const Z* Gcm, Z* C) {
+    hipLaunchKernelGGL(build_C_kernel<Z>, dim3((N + 255) / 256, N), dim3(256), 0, st, N, ldN, P, Gcm, C);
 }
 
-__global__ void add_diag_kernel(int N, int ldN, zd* A, double v) {
+template <class Z>
+__global__ void add_diag_kernel(int N, int ldN, Z* A, double v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) A[(size_t)i * ldN + i].x += v;
+    if (i < N) A[(size_t)i * ldN + i].x += (typename zreal<Z>::type)v;
 }
 
-void launch_add_diag(hipStream_t st, int N, int ldN, zd* A, double v) {
-    hipLaunchKernelGGL(add_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, A, v);
+template <class Z>
+void launch_add_diag(hipStream_t st, int N, int ldN, Z* A, double v) {
+    hipLaunchKernelGGL(add_diag_kernel<Z>, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, A, v);
 }
 
 __global__ void transpose_kernel(int N, int ldN, const zd* __restrict__ in, zd* __restrict__ out) {
@@ -2046,40 +2334,44 @@ void launch_nonfinite(hipStream_t st, int N, int ldN, const zd* G, int32_t* flag
 // W = (W + W^T) / 2 in place (N x N, leading dimension ldN).  The workgroup of tile pair
 // (bx, by), bx <= by, owns both tiles: it stages them in LDS and writes both back, and the two
 // entries of a pair get the same value (addition commutes), so W ends exactly symmetric.
-__global__ void __launch_bounds__(256) symmetrize_kernel(int N, int ldN, zd* W) {
+template <class Z>
+__global__ void __launch_bounds__(256) symmetrize_kernel(int N, int ldN, Z* W) {
+    typedef typename zreal<Z>::type R;
     if (blockIdx.x > blockIdx.y) return;
-    __shared__ zd t1[32][33], t2[32][33];
+    __shared__ Z t1[32][33], t2[32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     for (int j = threadIdx.y; j < 32; j += 8) {
         const int r1 = bx + j, c1 = by + threadIdx.x;  // t1[j][x] = W[bx + j][by + x]
-        t1[j][threadIdx.x] = (r1 < N && c1 < N) ? W[(size_t)r1 * ldN + c1] : zmk(0.0, 0.0);
+        t1[j][threadIdx.x] = (r1 < N && c1 < N) ? W[(size_t)r1 * ldN + c1] : zc<Z>(0, 0);
         const int r2 = by + j, c2 = bx + threadIdx.x;  // t2[j][x] = W[by + j][bx + x]
-        t2[j][threadIdx.x] = (r2 < N && c2 < N) ? W[(size_t)r2 * ldN + c2] : zmk(0.0, 0.0);
+        t2[j][threadIdx.x] = (r2 < N && c2 < N) ? W[(size_t)r2 * ldN + c2] : zc<Z>(0, 0);
     }
     __syncthreads();
     for (int j = threadIdx.y; j < 32; j += 8) {
         const int r1 = bx + j, c1 = by + threadIdx.x;
-        const zd a = t1[j][threadIdx.x], b = t2[threadIdx.x][j];  // W[r1][c1], W[c1][r1]
-        const zd m = zmk(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+        const Z a = t1[j][threadIdx.x], b = t2[threadIdx.x][j];  // W[r1][c1], W[c1][r1]
+        const Z m = zc<Z>(R(0.5) * (a.x + b.x), R(0.5) * (a.y + b.y));
         if (r1 < N && c1 < N) W[(size_t)r1 * ldN + c1] = m;
         const int r2 = by + j, c2 = bx + threadIdx.x;
-        const zd a2 = t2[j][threadIdx.x], b2 = t1[threadIdx.x][j];  // W[r2][c2], W[c2][r2]
-        const zd m2 = zmk(0.5 * (a2.x + b2.x), 0.5 * (a2.y + b2.y));
+        const Z a2 = t2[j][threadIdx.x], b2 = t1[threadIdx.x][j];  // W[r2][c2], W[c2][r2]
+        const Z m2 = zc<Z>(R(0.5) * (a2.x + b2.x), R(0.5) * (a2.y + b2.y));
         if (r2 < N && c2 < N) W[(size_t)r2 * ldN + c2] = m2;
     }
 }
 
-void launch_symmetrize(hipStream_t st, int N, int ldN, zd* W) {
+template <class Z>
+void launch_symmetrize(hipStream_t st, int N, int ldN, Z* W) {
     dim3 grid((N + 31) / 32, (N + 31) / 32);
-    hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(32, 8), 0, st, N, ldN, W);
+    hipLaunchKernelGGL(symmetrize_kernel<Z>, grid, dim3(32, 8), 0, st, N, ldN, W);
 }
 
 // Gobj row 0 = sum_p G(p, :),  Einc = sum_q G(:, q)   (pngfopt.hpp:584-595)
-__global__ void port_sums_kernel(int N, int ldN, const zd* __restrict__ G, const int32_t* __restrict__ port, int np,
-                                 zd* gobj0, zd* einc) {
+template <class Z>
+__global__ void port_sums_kernel(int N, int ldN, const Z* __restrict__ G, const int32_t* __restrict__ port, int np,
+                                 Z* gobj0, Z* einc) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    zd a = zmk(0.0, 0.0), b = zmk(0.0, 0.0);
+    Z a = zc<Z>(0, 0), b = zc<Z>(0, 0);
     for (int i = 0; i < np; i++) {
         const int p = port[i];
         a = zadd(a, G[(size_t)n * ldN + p]);
@@ -2089,27 +2381,32 @@ __global__ void port_sums_kernel(int N, int ldN, const zd* __restrict__ G, const
     einc[n] = b;
 }
 
-void launch_port_sums(hipStream_t st, int N, int ldN, const zd* Gcm, const int32_t* port, int np, zd* gobj0,
-                      zd* einc) {
-    hipLaunchKernelGGL(port_sums_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, Gcm, port, np, gobj0, einc);
+template <class Z>
+void launch_port_sums(hipStream_t st, int N, int ldN, const Z* Gcm, const int32_t* port, int np, Z* gobj0,
+                      Z* einc) {
+    hipLaunchKernelGGL(port_sums_kernel<Z>, dim3((N + 255) / 256), dim3(256), 0, st, N, ldN, Gcm, port, np, gobj0,
+                       einc);
 }
 
-__global__ void pe_kernel(int N, const int8_t* __restrict__ P, const zd* __restrict__ einc, zd* pe) {
+template <class Z>
+__global__ void pe_kernel(int N, const int8_t* __restrict__ P, const Z* __restrict__ einc, Z* pe) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    zd e = einc[n];
-    pe[n] = P[n] ? zmk(-e.x, -e.y) : zmk(0.0, 0.0);
+    Z e = einc[n];
+    pe[n] = P[n] ? zc<Z>(-e.x, -e.y) : zc<Z>(0, 0);
 }
 
-void launch_pe(hipStream_t st, int N, const int8_t* P, const zd* einc, zd* pe) {
-    hipLaunchKernelGGL(pe_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, P, einc, pe);
+template <class Z>
+void launch_pe(hipStream_t st, int N, const int8_t* P, const Z* einc, Z* pe) {
+    hipLaunchKernelGGL(pe_kernel<Z>, dim3((N + 255) / 256), dim3(256), 0, st, N, P, einc, pe);
 }
 
 // out[r] = sum_n A[r][n] x[n]   (rows of length N at stride ldN; fixed-order tree)
-__global__ void dot_rows_kernel(int N, int ldN, const zd* __restrict__ A, const zd* __restrict__ x, zd* out) {
-    __shared__ zd red[256];
+template <class Z>
+__global__ void dot_rows_kernel(int N, int ldN, const Z* __restrict__ A, const Z* __restrict__ x, zd* out) {
+    __shared__ Z red[256];
     const int r = blockIdx.x, tid = threadIdx.x;
-    zd s = zmk(0.0, 0.0);
+    Z s = zc<Z>(0, 0);
     for (int n = tid; n < N; n += 256) s = zfma(A[(size_t)r * ldN + n], x[n], s);
     red[tid] = s;
     __syncthreads();
@@ -2117,49 +2414,102 @@ __global__ void dot_rows_kernel(int N, int ldN, const zd* __restrict__ A, const 
         if (tid < off) red[tid] = zadd(red[tid], red[tid + off]);
         __syncthreads();
     }
-    if (tid == 0) out[r] = red[0];
+    if (tid == 0) out[r] = zwide(red[0]);
 }
 
-void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const zd* A, const zd* x, zd* out) {
-    hipLaunchKernelGGL(dot_rows_kernel, dim3(rows), dim3(256), 0, st, N, ldN, A, x, out);
+template <class Z>
+void launch_dot_rows(hipStream_t st, int N, int ldN, int rows, const Z* A, const Z* x, zd* out) {
+    hipLaunchKernelGGL(dot_rows_kernel<Z>, dim3(rows), dim3(256), 0, st, N, ldN, A, x, out);
+}
+
+// out = (complex64) in, rounded to nearest, over the N x N column-major matrix (ld ldN)
+__global__ void cast_c64_kernel(int N, int ldN, const zd* __restrict__ in, zf* out) {
+    for (int col = blockIdx.y; col < N; col += gridDim.y)
+        for (int r = blockIdx.x * 256 + threadIdx.x; r < N; r += gridDim.x * 256) {
+            const zd v = in[(size_t)col * ldN + r];
+            out[(size_t)col * ldN + r] = zmkf((float)v.x, (float)v.y);
+        }
+}
+
+void launch_cast_c64(hipStream_t st, int N, int ldN, const zd* in, zf* out) {
+    const int gx = (N + 255) / 256 < 8 ? (N + 255) / 256 : 8;
+    dim3 grid(gx, N < 4096 ? N : 4096);
+    hipLaunchKernelGGL(cast_c64_kernel, grid, dim3(256), 0, st, N, ldN, in, out);
 }
 
 
 // Current W over an arbitrary group's components (flips outside the driver's
 // group table): D[a][b] = W0[c_a][c_b] - sum_K' A_all[c_a][K'] M_all[K'][c_b].
-__global__ void lr_block_current_kernel(Dims d, DevTabs tb, GroupTab gt, int g, int K, zd* Dblk) {
+template <class Z>
+__global__ void lr_block_current_kernel(Dims d, DevTabsT<Z> tb, GroupTab gt, int g, int K, Z* Dblk) {
     const int f = blockIdx.y;
     const int nc = gt.ncomp[g];
     const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
-    const zd* W0 = tb.W0[f];
-    const zd* A = tb.Aall[f];
-    zd* D = Dblk + ((size_t)f * d.ngroups + g) * gt.gmax * gt.gmax;
+    const Z* W0 = tb.W0[f];
+    const Z* A = tb.Aall[f];
+    Z* D = Dblk + ((size_t)f * d.ngroups + g) * gt.gmax * gt.gmax;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nc * nc; e += gridDim.x * blockDim.x) {
         const int a = e / nc, b = e - a * nc;
-        zd s = W0[(size_t)cl[a] * d.ldN + cl[b]];
+        Z s = W0[(size_t)cl[a] * d.ldN + cl[b]];
         for (int kk = 0; kk < K; kk++) s = zfms(A[(size_t)cl[a] * d.Kmax + kk], m_entry(tb, d, f, kk, cl[b]), s);
         D[a * gt.gmax + b] = s;
     }
 }
 
-void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabs& tb, GroupTab gt, int g, int K, zd* Dblk) {
-    hipLaunchKernelGGL(lr_block_current_kernel, dim3(36, d.nf), dim3(256), 0, st, d, tb, gt, g, K, Dblk);
+template <class Z>
+void launch_lr_block_current(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, GroupTab gt, int g, int K, Z* Dblk) {
+    hipLaunchKernelGGL(lr_block_current_kernel<Z>, dim3(36, d.nf), dim3(256), 0, st, d, tb, gt, g, K, Dblk);
 }
 
 // D[g][a][b] = W0[comp_g[a]][comp_g[b]]
-__global__ void extract_blocks_kernel(Dims d, const zd* __restrict__ W0, GroupTab gt, zd* Dblk) {
+template <class Z>
+__global__ void extract_blocks_kernel(Dims d, const Z* __restrict__ W0, GroupTab gt, Z* Dblk) {
     const int g = blockIdx.x;
     const int nc = gt.ncomp[g];
     const int32_t* cl = gt.comps + (size_t)g * gt.gmax;
-    zd* D = Dblk + (size_t)g * gt.gmax * gt.gmax;
+    Z* D = Dblk + (size_t)g * gt.gmax * gt.gmax;
     for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
         const int a = e / nc, b = e - a * nc;
         D[a * gt.gmax + b] = W0[(size_t)cl[a] * d.ldN + cl[b]];
     }
 }
 
-void launch_extract_blocks(hipStream_t st, const Dims& d, const zd* W0, GroupTab gt, zd* Dblk_f) {
-    hipLaunchKernelGGL(extract_blocks_kernel, dim3(d.ngroups), dim3(256), 0, st, d, W0, gt, Dblk_f);
+template <class Z>
+void launch_extract_blocks(hipStream_t st, const Dims& d, const Z* W0, GroupTab gt, Z* Dblk_f) {
+    hipLaunchKernelGGL(extract_blocks_kernel<Z>, dim3(d.ngroups), dim3(256), 0, st, d, W0, gt, Dblk_f);
 }
+
+// ------------------------------------------------- explicit instantiations -----
+// complex128 (every method) and complex64 (the LOWRANK state of the single-precision mode)
+#define PNGF_INST(Z)                                                                                             \
+    template void launch_lr_eval<Z>(hipStream_t, const Dims&, const DevVecsT<Z>&, GroupTab, const int32_t*,        \
+                                    const int8_t*, const int32_t*, const int32_t*, int, zd*, int32_t*, Z*,        \
+                                    const int32_t*);                                                              \
+    template void launch_lr_accept_prep<Z>(hipStream_t, const Dims&, const DevVecsT<Z>&, int, const int32_t*,      \
+                                           const int32_t*, const int32_t*, int, Z*, int, int32_t*, Z*,            \
+                                           const int32_t*, const int32_t*);                                       \
+    template void launch_lr_gather<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, const int32_t*, int, int, Z*, \
+                                      Z*, int, const int32_t*);                                                   \
+    template void launch_lr_build_m<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, int, int, int, Z*, int);      \
+    template void launch_apply_capinv<Z>(hipStream_t, const Dims&, const Z*, int, const Z*, size_t, int, int,      \
+                                         const int32_t*, Z* const*, Z*, size_t, int, int, const Z* const*, int);   \
+    template void launch_lr_small<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, const DevVecsT<Z>&, const Z*,  \
+                                     int, int, int);                                                              \
+    template void launch_lr_finish<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, const DevVecsT<Z>&, GroupTab, \
+                                      const Z*, int, const Z*, size_t, const int32_t*, int, int, const int32_t*,  \
+                                      const int32_t*, int, int, FlipUpdate);                                      \
+    template void launch_lr_blocks<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, const DevVecsT<Z>&, GroupTab, \
+                                      int, int);                                                                  \
+    template void launch_build_C<Z>(hipStream_t, int, int, const int8_t*, const Z*, Z*);                          \
+    template void launch_add_diag<Z>(hipStream_t, int, int, Z*, double);                                          \
+    template void launch_symmetrize<Z>(hipStream_t, int, int, Z*);                                                 \
+    template void launch_port_sums<Z>(hipStream_t, int, int, const Z*, const int32_t*, int, Z*, Z*);               \
+    template void launch_pe<Z>(hipStream_t, int, const int8_t*, const Z*, Z*);                                     \
+    template void launch_dot_rows<Z>(hipStream_t, int, int, int, const Z*, const Z*, zd*);                         \
+    template void launch_lr_block_current<Z>(hipStream_t, const Dims&, const DevTabsT<Z>&, GroupTab, int, int, Z*); \
+    template void launch_extract_blocks<Z>(hipStream_t, const Dims&, const Z*, GroupTab, Z*);
+PNGF_INST(zd)
+PNGF_INST(zf)
+#undef PNGF_INST
 
 }  // namespace pngf

@@ -75,6 +75,10 @@ constexpr int kPinSel = 512, kPinBatch = 1024, kPinInfo = 16384, kPinInts = 6553
 
 struct pngf_handle {
     pngf_config cfg{};
+    // complex64 state (PNGF_PREC_C64): every zd*-typed buffer of the LOWRANK state below then holds
+    // complex64 entries, esz bytes each; Eobj, eobj_out and the objective stay complex128
+    bool c64 = false;
+    size_t esz = sizeof(zd);
     Layout L;
     Groups grp;
     int ngroups_drv = 0, adhoc = -1;
@@ -218,6 +222,8 @@ int dalloc(pngf_handle* h, void** p, size_t bytes) {
 
 template <class T>
 int dalloc_t(pngf_handle* h, T** p, size_t n) { return dalloc(h, (void**)p, n * sizeof(T)); }
+// n complex entries of the handle's state precision (esz bytes each)
+int zalloc(pngf_handle* h, zd** p, size_t n) { return dalloc(h, (void**)p, n * h->esz); }
 
 void dfree(pngf_handle* h, void* p) {
     if (!p) return;
@@ -301,6 +307,20 @@ GroupTab gtab(pngf_handle* h) {
     return g;
 }
 
+// The LOWRANK state of a handle seen with the complex type of its precision (Z = zf for
+// PNGF_PREC_C64, whose buffers hold complex64 entries behind the zd-typed pointers).
+template <class Z>
+struct LRView {
+    DevTabsT<Z> tb;
+    DevVecsT<Z> v;
+    Z *acc, *lu, *Mc, *Ar, *Bt, *mchunk;
+};
+template <class Z>
+LRView<Z> lrview(pngf_handle* h) {
+    return LRView<Z>{tabs_as<Z>(h->tabs), vecs_as<Z>(h->vecs), (Z*)h->d_acc, (Z*)h->d_lu,
+                     (Z*)h->d_Mc,         (Z*)h->d_Ar,         (Z*)h->d_Bt,  (Z*)h->d_mchunk};
+}
+
 // Host change set of group g against the host mirrors (same rule as the device).
 int host_change_set(pngf_handle* h, int g, int* t, int* s, int* lp) {
     const Groups& G = h->grp;
@@ -342,8 +362,12 @@ int set_adhoc(pngf_handle* h, const int32_t* tiles, int nt) {
     HC(h, hipMemcpyAsync(h->d_gcomps + (size_t)g * G.gmax, &G.comps[(size_t)g * G.gmax], G.gmax * sizeof(int32_t),
                          hipMemcpyHostToDevice, h->st));
     HC(h, hipStreamSynchronize(h->st));
-    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->inited)
-        launch_lr_block_current(h->st, h->d, h->tabs, gtab(h), g, h->K, h->vecs.Dblk);
+    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->inited) {
+        if (h->c64)
+            launch_lr_block_current(h->st, h->d, tabs_as<zf>(h->tabs), gtab(h), g, h->K, (zf*)h->vecs.Dblk);
+        else
+            launch_lr_block_current(h->st, h->d, h->tabs, gtab(h), g, h->K, h->vecs.Dblk);
+    }
     return PNGF_OK;
 }
 
@@ -430,25 +454,135 @@ int alloc_gcm(pngf_handle* h, int f) {
     return PNGF_OK;
 }
 
+// A NaN or an infinity in G gives NaN scores, which never compare below F_best: the run would
+// stop as "converged" on garbage.  G is refused instead (the reference does not check).
+int check_finite(pngf_handle* h, int f) {
+    hipStream_t st = h->st;
+    HC(h, hipMemsetAsync(h->d_flag, 0, sizeof(int32_t), st));
+    launch_nonfinite(st, h->d.N, h->d.ldN, h->Gcm[f], h->d_flag);
+    HC(h, hipMemcpyAsync(h->h_pin_i, h->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    int bad = h->h_pin_i[0];
+    for (const zd& v : h->gobj_extra[f])
+        if (!std::isfinite(v.x) || !std::isfinite(v.y)) bad = 1;
+    if (bad) return fail(h, PNGF_ERR_NONFINITE, "G of frequency slot " + std::to_string(f) + " holds NaN or Inf");
+    return PNGF_OK;
+}
+
+// *asym_f = max|G_ij - G_ji| / max|G_ij| of slot f's (complex128) G; 1 unless the symmetric accept may apply
+int measure_asym(pngf_handle* h, int f, double* asym_f) {
+    *asym_f = 1.0;
+    if (h->cfg.symmetry != PNGF_SYM_AUTO) return PNGF_OK;
+    hipStream_t st = h->st;
+    unsigned long long m[2] = {0, 0};
+    HC(h, hipMemsetAsync(h->d_asym, 0, 2 * sizeof(unsigned long long), st));
+    launch_asym(st, h->d.N, h->d.ldN, h->Gcm[f], h->d_asym);
+    HC(h, hipMemcpyAsync(m, h->d_asym, sizeof(m), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    double dm, am;
+    std::memcpy(&dm, &m[0], sizeof(double));
+    std::memcpy(&am, &m[1], sizeof(double));
+    *asym_f = am > 0.0 ? dm / am : 0.0;
+    h->asym = std::max(h->asym, *asym_f);
+    return PNGF_OK;
+}
+
+// complex64 state (the reference's USE_SINGLE_PRECISION): G cast to c64 as the reference casts the
+// file (pngfopt.hpp:128-156), then Gobj row 0 and Einc (pngfopt.hpp:584-595), C and its inverse by
+// cgetrf + cgetri (linalg.hpp:49-84), PEinc, R = Gobj C^-1, Eobj = R PEinc, W = (G - I) C^-1, q and the
+// blocks, all in c64 (Eobj widened to c128)
+int init_freq_c64(pngf_handle* h, int f) {
+    typedef rocblas_float_complex rbc;
+    const int N = h->d.N, ld = h->d.ldN, ne = h->d.ne;
+    hipStream_t st = h->st;
+    const rbc one = {1.f, 0.f}, zero = {0.f, 0.f};
+    CK(check_finite(h, f));
+    double asym_f = 1.0;
+    CK(measure_asym(h, f, &asym_f));
+    zd* Gbuf = nullptr;
+    CK(zalloc(h, &Gbuf, (size_t)ld * N));
+    zf* Gf = (zf*)Gbuf;
+    launch_cast_c64(st, N, ld, h->Gcm[f], Gf);
+    HC(h, hipStreamSynchronize(st));
+    dfree(h, h->Gcm[f]);
+    h->Gcm[f] = nullptr;
+    zd* Cbuf = nullptr;
+    CK(zalloc(h, &Cbuf, (size_t)ld * N));
+    zf* C = (zf*)Cbuf;
+    launch_build_C(st, N, ld, h->d_P, (const zf*)Gf, C);
+    zf* gobj = (zf*)h->vecs.Gobj + (size_t)f * ne * ld;
+    zf* einc = (zf*)h->vecs.Einc + (size_t)f * ld;
+    launch_port_sums(st, N, ld, (const zf*)Gf, h->d_small + 400, (int)h->L.port.size(), gobj, einc);
+    std::vector<zf> row(N);
+    for (int r = 1; r < ne; r++) {
+        for (int n = 0; n < N; n++) {
+            const zd v = h->gobj_extra[f][(size_t)(r - 1) * N + n];
+            row[n] = zmkf((float)v.x, (float)v.y);
+        }
+        HC(h, hipMemcpy(gobj + (size_t)r * ld, row.data(), N * sizeof(zf), hipMemcpyHostToDevice));
+    }
+    int* ipiv = nullptr;
+    CK(dalloc_t(h, &ipiv, N));
+    RB(h, rocsolver_cgetrf(h->rb, N, N, (rbc*)C, ld, ipiv, h->d_info));
+    int info = 0;
+    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    if (info != 0) { dfree(h, ipiv); dfree(h, Cbuf); dfree(h, Gbuf); return fail(h, PNGF_ERR_SINGULAR, "C is singular (cgetrf info " + std::to_string(info) + ")"); }
+    RB(h, rocsolver_cgetri(h->rb, N, (rbc*)C, ld, ipiv, h->d_info));
+    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    dfree(h, ipiv);
+    if (info != 0) { dfree(h, Cbuf); dfree(h, Gbuf); return fail(h, PNGF_ERR_SINGULAR, "C is singular (cgetri info " + std::to_string(info) + ")"); }
+    zf* pe = (zf*)h->d_pe;
+    launch_pe(st, N, h->d_P, (const zf*)einc, pe);
+    zf* R = (zf*)h->vecs.R + (size_t)f * ne * ld;
+    RB(h, rocblas_cgemm(h->rb, rocblas_operation_transpose, rocblas_operation_none, N, ne, N, &one, (rbc*)C, ld,
+                        (rbc*)gobj, ld, &zero, (rbc*)R, ld));
+    launch_dot_rows(st, N, ld, ne, (const zf*)R, (const zf*)pe, h->vecs.Eobj + (size_t)f * ne);
+    launch_add_diag(st, N, ld, Gf, -1.0);
+    zd* Wbuf = nullptr;
+    CK(zalloc(h, &Wbuf, (size_t)ld * N));
+    zf* W = (zf*)Wbuf;
+    RB(h, rocblas_cgemm(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, N, N, N, &one, (rbc*)C, ld,
+                        (rbc*)Gf, ld, &zero, (rbc*)W, ld));
+    if (asym_f <= kSymTol) launch_symmetrize(st, N, ld, W);
+    RB(h, rocblas_cgemv(h->rb, rocblas_operation_transpose, N, N, &one, (rbc*)W, ld, (rbc*)pe, 1, &zero,
+                        (rbc*)((zf*)h->vecs.q + (size_t)f * ld), 1));
+    launch_extract_blocks(st, h->d, (const zf*)W, gtab(h), (zf*)h->vecs.Dblk + (size_t)f * h->d.ngroups * h->d.gmax * h->d.gmax);
+    HC(h, hipStreamSynchronize(st));
+    dfree(h, Gbuf);
+    dfree(h, Cbuf);
+    h->W0[f] = Wbuf;
+    HC(h, hipGetLastError());
+    // Vinc, LN1inc, LN2inc summed in single precision (pngfopt.hpp:599-617 in c64), kept as c128
+    std::vector<zf> e(h->L.port.size());
+    for (size_t i = 0; i < h->L.port.size(); i++)
+        HC(h, hipMemcpyAsync(&e[i], einc + h->L.port[i], sizeof(zf), hipMemcpyDeviceToHost, st));
+    HC(h, hipStreamSynchronize(st));
+    zf vinc = zmkf(0.f, 0.f), ln1 = zmkf(0.f, 0.f), ln2 = zmkf(0.f, 0.f);
+    for (size_t i = 0; i < e.size(); i++) vinc = zadd(vinc, e[i]);
+    if (ne > 2)
+        for (int p : h->L.port) {
+            const zd a = h->gobj_extra[f][p], b = h->gobj_extra[f][(size_t)N + p];
+            ln1 = zadd(ln1, zmkf((float)a.x, (float)a.y));
+            ln2 = zadd(ln2, zmkf((float)b.x, (float)b.y));
+        }
+    FreqConst fcv;
+    fcv.vinc = zwide(vinc);
+    fcv.ln1 = zwide(ln1);
+    fcv.ln2 = zwide(ln2);
+    fcv.k0 = 2.0 * kPi * h->freqs[f] / kC0;
+    h->fc[f] = fcv;
+    return PNGF_OK;
+}
+
 int init_freq(pngf_handle* h, int f) {
+    if (h->c64) return init_freq_c64(h, f);
     const int N = h->d.N, ld = h->d.ldN, ne = h->d.ne;
     hipStream_t st = h->st;
     const rbz one = {1.0, 0.0}, zero = {0.0, 0.0};
     zd* G = h->Gcm[f];
-    // A NaN or an infinity in G gives NaN scores, which never compare below F_best: the run would
-    // stop as "converged" on garbage.  G is refused here instead (the reference does not check).
-    {
-        int bad = 0;
-        HC(h, hipMemsetAsync(h->d_flag, 0, sizeof(int32_t), st));
-        launch_nonfinite(st, N, ld, G, h->d_flag);
-        HC(h, hipMemcpyAsync(h->h_pin_i, h->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        HC(h, hipStreamSynchronize(st));
-        bad = h->h_pin_i[0];
-        for (const zd& v : h->gobj_extra[f])
-            if (!std::isfinite(v.x) || !std::isfinite(v.y)) bad = 1;
-        if (bad)
-            return fail(h, PNGF_ERR_NONFINITE, "G of frequency slot " + std::to_string(f) + " holds NaN or Inf");
-    }
+    CK(check_finite(h, f));
     zd* C = nullptr;
     CK(dalloc_t(h, &C, (size_t)ld * N));
     launch_build_C(st, N, ld, h->d_P, G, C);
@@ -490,18 +624,7 @@ int init_freq(pngf_handle* h, int f) {
         h->Cinv[f] = C;
     } else {
         double asym_f = 1.0;
-        if (h->cfg.symmetry == PNGF_SYM_AUTO) {
-            unsigned long long m[2] = {0, 0};
-            HC(h, hipMemsetAsync(h->d_asym, 0, 2 * sizeof(unsigned long long), st));
-            launch_asym(st, N, ld, G, h->d_asym);
-            HC(h, hipMemcpyAsync(m, h->d_asym, sizeof(m), hipMemcpyDeviceToHost, st));
-            HC(h, hipStreamSynchronize(st));
-            double dm, am;
-            std::memcpy(&dm, &m[0], sizeof(double));
-            std::memcpy(&am, &m[1], sizeof(double));
-            asym_f = am > 0.0 ? dm / am : 0.0;
-            h->asym = std::max(h->asym, asym_f);
-        }
+        CK(measure_asym(h, f, &asym_f));
         // W = (G - I) C^-1, stored row-major: W^T = C^-T (G - I)^T
         launch_add_diag(st, N, ld, G, -1.0);
         zd* W = nullptr;
@@ -555,7 +678,8 @@ int autotune_skinny(pngf_handle* h) {
     const int K = std::max(0, std::min(h->Kmax - k, 64 * k));
     for (int i = 0; i < k; i++) h->h_pin_i[i] = h->grp.comps[(size_t)g0 * d.gmax + i];
     HC(h, hipMemcpy(h->d_small, h->h_pin_i, k * sizeof(int32_t), hipMemcpyHostToDevice));
-    const int nv = skinny_variants(k);
+    const int nv = skinny_variants(k, h->c64);
+    const LRView<zf> w32 = lrview<zf>(h);
     hipEvent_t e0, e1;
     HC(h, hipEventCreate(&e0));
     HC(h, hipEventCreate(&e1));
@@ -564,6 +688,13 @@ int autotune_skinny(pngf_handle* h) {
     for (int v = 0; v < nv; v++) {
         for (int rep = 0; rep < 3; rep++) {
             if (rep == 1) HC(h, hipEventRecord(e0, h->st));
+            if (h->c64) {
+                launch_lr_colblk(h->st, d, w32.tb, h->d_small, k, K, w32.Mc, (int)h->small_stride, v);
+                if (!h->sym)
+                    launch_lr_rowblk(h->st, d, w32.tb, h->d_small, k, K, w32.Ar, (int)h->small_stride, w32.Bt,
+                                     h->bt_stride, v);
+                continue;
+            }
             launch_lr_colblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Mc, (int)h->small_stride, v);
             if (!h->sym)
                 launch_lr_rowblk(h->st, d, h->tabs, h->d_small, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt,
@@ -593,9 +724,13 @@ int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
     double bytes = 0, flops = 0;
     const double kk = (double)d.gmax;
     if (h->cfg.method == PNGF_METHOD_LOWRANK) {
-        launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb, h->d_eobj_out, h->d_info,
-                       h->d_lu);
-        bytes = (double)nb * d.nf * 16.0 * (kk * kk + kk * (d.ne + 2));
+        if (h->c64)
+            launch_lr_eval(st, d, vecs_as<zf>(h->vecs), gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb,
+                           h->d_eobj_out, h->d_info, (zf*)h->d_lu);
+        else
+            launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, nb, h->d_eobj_out,
+                           h->d_info, h->d_lu);
+        bytes = (double)nb * d.nf * (double)h->esz * (kk * kk + kk * (d.ne + 2));
         flops = (double)nb * d.nf * ((8.0 / 3.0) * kk * kk * kk + 16.0 * kk * kk);
     } else {
         const int items = nb * d.nf;
@@ -615,7 +750,27 @@ int eval_launch(pngf_handle* h, const int32_t* groups, int nb) {
 // ---------------------------------------------------------------- accept ---
 // out -= A_all[:, :K] M[:K, :] for frequency f (out: N x ldN row-major, W0 or a copy of it);
 // with the factors kept as A and Lam, M is rebuilt kMChunk rows at a time.
+int fold_into_c64(pngf_handle* h, int f, zf* out) {
+    typedef rocblas_float_complex rbc;
+    const rbc one = {1.f, 0.f}, mone = {-1.f, 0.f};
+    const Dims& d = h->d;
+    const LRView<zf> w = lrview<zf>(h);
+    if (!h->lam) {
+        RB(h, rocblas_cgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, h->K, &mone,
+                            (rbc*)h->Mall[f], d.ldN, (rbc*)h->Aall[f], d.Kmax, &one, (rbc*)out, d.ldN));
+        return PNGF_OK;
+    }
+    for (int kk0 = 0; kk0 < h->K; kk0 += kMChunk) {
+        const int nr = std::min(kMChunk, h->K - kk0);
+        launch_lr_build_m(h->st, d, w.tb, f, kk0, nr, w.mchunk, d.ldN);
+        RB(h, rocblas_cgemm(h->rb, rocblas_operation_none, rocblas_operation_none, d.N, d.N, nr, &mone,
+                            (rbc*)w.mchunk, d.ldN, (rbc*)((zf*)h->Aall[f] + kk0), d.Kmax, &one, (rbc*)out, d.ldN));
+    }
+    return PNGF_OK;
+}
+
 int fold_into(pngf_handle* h, int f, zd* out) {
+    if (h->c64) return fold_into_c64(h, f, (zf*)out);
     const rbz one = {1.0, 0.0}, mone = {-1.0, 0.0};
     const Dims& d = h->d;
     if (!h->lam) {
@@ -639,12 +794,73 @@ int flush_factors(pngf_handle* h) {
     prof_begin(h, tm);
     for (int f = 0; f < d.nf; f++) {
         CK(fold_into(h, f, h->W0[f]));
-        if (h->sym) launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
+        if (h->sym) {
+            if (h->c64) launch_symmetrize(h->st, d.N, d.ldN, (zf*)h->W0[f]);
+            else launch_symmetrize(h->st, d.N, d.ldN, h->W0[f]);
+        }
     }
-    prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (32.0 * d.N * (double)d.N + 32.0 * d.N * (double)h->K),
+    const double eb2 = 2.0 * (double)h->esz;
+    prof_end(h, tm, PNGF_K_RANKK, (double)d.nf * (eb2 * d.N * (double)d.N + eb2 * d.N * (double)h->K),
              (double)d.nf * 8.0 * h->K * (double)d.N * d.N);
     h->K = 0;
     h->flushes++;
+    return PNGF_OK;
+}
+
+// The LOWRANK accept of group g (change set t, s, lp already on the device) in the state's precision.
+template <class Z>
+int accept_lowrank(pngf_handle* h, int g, int k, int nt, const int32_t* dt, const int32_t* ds, const int32_t* dlp,
+                   const int32_t* dgt) {
+    hipStream_t st = h->st;
+    const Dims& d = h->d;
+    if (h->K + k > h->Kmax) CK(flush_factors(h));
+    const int K = h->K;
+    const LRView<Z> w = lrview<Z>(h);
+    // the capacitance solve (one workgroup per frequency) runs on the side stream while the
+    // strips are formed; the epilogue waits for both
+    HC(h, hipEventRecord(h->ev_fork, st));
+    HC(h, hipStreamWaitEvent(h->st2, h->ev_fork, 0));
+    Timed tp{};
+    prof_begin(h, tp, h->st2);
+    launch_lr_accept_prep(h->st2, d, w.v, g, dt, ds, dlp, k, w.acc, h->acc_stride, h->d_info, w.lu);
+    prof_end(h, tp, PNGF_K_ACC_SMALL, 0, 0, h->st2);
+    HC(h, hipEventRecord(h->ev_join, h->st2));
+    Timed tm{};
+    prof_begin(h, tm);
+    launch_lr_gather(st, d, w.tb, dt, k, K, w.Mc, h->sym ? (Z*)nullptr : w.Ar, (int)h->small_stride);
+    prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
+    const double N = d.N, kd = k, Kd = K, eb = (double)h->esz;
+    Timed t1{};
+    prof_begin(h, t1);
+    launch_lr_colblk(st, d, w.tb, dt, k, K, w.Mc, (int)h->small_stride, h->skv);
+    prof_end(h, t1, PNGF_K_COLBLK, d.nf * eb * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+    if (!h->sym) {
+        Timed t2{};
+        prof_begin(h, t2);
+        launch_lr_rowblk(st, d, w.tb, dt, k, K, w.Ar, (int)h->small_stride, w.Bt, h->bt_stride, h->skv);
+        prof_end(h, t2, PNGF_K_ROWBLK, d.nf * eb * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
+    }
+    HC(h, hipStreamWaitEvent(st, h->ev_join, 0));
+    Timed t3{};
+    prof_begin(h, t3);
+    // M_new = diag(s) cap^-1 W[t,:]; W[t,:]^T is the row strip Bt, or, for symmetric W, the
+    // column strip just written to A_all[:, K:K+k]
+    if (d.gmax <= kLdsGmax) {
+        launch_lr_finish(st, d, w.tb, w.v, gtab(h), (const Z*)w.acc, h->acc_stride, (const Z*)w.Bt, h->bt_stride, ds,
+                         k, K, nullptr, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
+    } else {
+        if (h->sym)
+            launch_apply_capinv<Z>(st, d, w.acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, w.tb.Mall, nullptr, 0, K,
+                                   d.ldN, w.tb.Aall, K);
+        else
+            launch_apply_capinv<Z>(st, d, w.acc, h->acc_stride, w.Bt, h->bt_stride, d.gmax, k, ds, w.tb.Mall,
+                                   nullptr, 0, K, d.ldN);
+        launch_lr_small(st, d, w.tb, w.v, (const Z*)w.acc, h->acc_stride, k, K);
+        launch_lr_blocks(st, d, w.tb, w.v, gtab(h), k, K);
+    }
+    launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
+    prof_end(h, t3, PNGF_K_ACC_SMALL, 0, 0);
+    h->K += k;
     return PNGF_OK;
 }
 
@@ -666,53 +882,8 @@ int accept_group(pngf_handle* h, int g) {
     const int32_t* dgt = h->d_small + 288;
     auto t0 = std::chrono::high_resolution_clock::now();
     if (k > 0 && h->cfg.method == PNGF_METHOD_LOWRANK) {
-        if (h->K + k > h->Kmax) CK(flush_factors(h));
-        const int K = h->K;
-        // the capacitance solve (one workgroup per frequency) runs on the side stream while the
-        // strips are formed; the epilogue waits for both
-        HC(h, hipEventRecord(h->ev_fork, st));
-        HC(h, hipStreamWaitEvent(h->st2, h->ev_fork, 0));
-        Timed tp{};
-        prof_begin(h, tp, h->st2);
-        launch_lr_accept_prep(h->st2, d, h->vecs, g, dt, ds, dlp, k, h->d_acc, h->acc_stride, h->d_info, h->d_lu);
-        prof_end(h, tp, PNGF_K_ACC_SMALL, 0, 0, h->st2);
-        HC(h, hipEventRecord(h->ev_join, h->st2));
-        Timed tm{};
-        prof_begin(h, tm);
-        launch_lr_gather(st, d, h->tabs, dt, k, K, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride);
-        prof_end(h, tm, PNGF_K_ACC_SMALL, 0, 0);
-        const double N = d.N, kd = k, Kd = K;
-        Timed t1{};
-        prof_begin(h, t1);
-        launch_lr_colblk(st, d, h->tabs, dt, k, K, h->d_Mc, (int)h->small_stride, h->skv);
-        prof_end(h, t1, PNGF_K_COLBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
-        if (!h->sym) {
-            Timed t2{};
-            prof_begin(h, t2);
-            launch_lr_rowblk(st, d, h->tabs, dt, k, K, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv);
-            prof_end(h, t2, PNGF_K_ROWBLK, d.nf * 16.0 * (N * Kd + 2 * N * kd + Kd * kd), d.nf * 8.0 * N * Kd * kd);
-        }
-        HC(h, hipStreamWaitEvent(st, h->ev_join, 0));
-        Timed t3{};
-        prof_begin(h, t3);
-        // M_new = diag(s) cap^-1 W[t,:]; W[t,:]^T is the row strip Bt, or, for symmetric W, the
-        // column strip just written to A_all[:, K:K+k]
-        if (d.gmax <= kLdsGmax) {
-            launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, k,
-                             K, nullptr, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv);
-        } else {
-            if (h->sym)
-                launch_apply_capinv(st, d, h->d_acc, h->acc_stride, nullptr, 0, d.Kmax, k, ds, h->tabs.Mall, nullptr,
-                                    0, K, d.ldN, h->tabs.Aall, K);
-            else
-                launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, d.gmax, k, ds,
-                                    h->tabs.Mall, nullptr, 0, K, d.ldN);
-            launch_lr_small(st, d, h->tabs, h->vecs, h->d_acc, h->acc_stride, k, K);
-            launch_lr_blocks(st, d, h->tabs, h->vecs, gtab(h), k, K);
-        }
-        launch_mask_update(st, h->d_P, h->d_tile, dt, ds, k, dgt, nt);
-        prof_end(h, t3, PNGF_K_ACC_SMALL, 0, 0);
-        h->K += k;
+        if (h->c64) CK(accept_lowrank<zf>(h, g, k, nt, dt, ds, dlp, dgt));
+        else CK(accept_lowrank<zd>(h, g, k, nt, dt, ds, dlp, dgt));
     } else if (k > 0) {
         // PLAIN: re-score the flip keeping its factors, then pngfopt.hpp:256-367
         const rbz one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
@@ -735,7 +906,7 @@ int accept_group(pngf_handle* h, int g) {
         prof_end(h, t1, PNGF_K_VCINV, d.nf * 16.0 * (N * N + 2 * N * kd), d.nf * 8.0 * kd * N * N);
         Timed t2{};
         prof_begin(h, t2);
-        launch_apply_capinv(st, d, h->d_acc, h->acc_stride, h->d_VC, h->v_stride, k, k, nullptr, nullptr, h->d_M,
+        launch_apply_capinv<zd>(st, d, h->d_acc, h->acc_stride, h->d_VC, h->v_stride, k, k, nullptr, nullptr, h->d_M,
                             h->v_stride, 0, d.ldN);
         launch_snapshot_cinvu(st, d, h->tabs, dt, ds, k, h->d_CU, h->v_stride);
         prof_end(h, t2, PNGF_K_ACC_SMALL, 0, 0);
@@ -785,7 +956,8 @@ double host_objective(pngf_handle* h, const zd* eobj) { return objective_of(h->o
 int ranker_din(const pngf_handle* h) { return 51 + 8 * h->d.nf; }
 
 int ensure_feat(pngf_handle* h) {
-    if (h->cfg.method != PNGF_METHOD_LOWRANK) return fail(h, PNGF_ERR_STATE, "ranker inputs need the LOWRANK method");
+    if (h->cfg.method != PNGF_METHOD_LOWRANK || h->c64)
+        return fail(h, PNGF_ERR_STATE, "ranker inputs need the LOWRANK method in complex128");
     if (!h->d_feat) {
         CK(dalloc_t(h, &h->d_feat, (size_t)h->d.ngroups * ranker_din(h)));
         CK(dalloc_t(h, &h->d_lab, (size_t)h->d.ngroups * h->d.nf));
@@ -966,10 +1138,14 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
     if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6 ||
         (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF) ||
-        c.factors < PNGF_FACTORS_AUTO || c.factors > PNGF_FACTORS_REBUILT) {
+        c.factors < PNGF_FACTORS_AUTO || c.factors > PNGF_FACTORS_REBUILT ||
+        (c.precision != PNGF_PREC_C128 && c.precision != PNGF_PREC_C64) ||
+        (c.precision == PNGF_PREC_C64 && c.method != PNGF_METHOD_LOWRANK)) {
         delete h;
         return PNGF_ERR_ARG;
     }
+    h->c64 = c.precision == PNGF_PREC_C64;
+    h->esz = h->c64 ? sizeof(zf) : sizeof(zd);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) {
         (void)hipGetLastError();
@@ -1020,16 +1196,16 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
 
     CK(dalloc_t(h, &h->d_Fbest, 4));
     CK(dalloc_t(h, &h->d_Fsel, 4));
-    CK(dalloc_t(h, &h->d_pe, ld));
+    CK(zalloc(h, &h->d_pe, ld));
     CK(dalloc_t(h, &h->d_tabs, 6 * (size_t)nf));
-    CK(dalloc_t(h, &h->vecs.q, nf * ld));
-    CK(dalloc_t(h, &h->vecs.S, nf * ld));
-    CK(dalloc_t(h, &h->vecs.R, nf * ne * ld));
-    CK(dalloc_t(h, &h->vecs.Gobj, nf * ne * ld));
-    CK(dalloc_t(h, &h->vecs.Einc, nf * ld));
+    CK(zalloc(h, &h->vecs.q, nf * ld));
+    CK(zalloc(h, &h->vecs.S, nf * ld));
+    CK(zalloc(h, &h->vecs.R, nf * ne * ld));
+    CK(zalloc(h, &h->vecs.Gobj, nf * ne * ld));
+    CK(zalloc(h, &h->vecs.Einc, nf * ld));
     CK(dalloc_t(h, &h->vecs.Eobj, (size_t)nf * ne));
     CK(dalloc_t(h, &h->vecs.fc, nf));
-    HC(h, hipMemset(h->vecs.Gobj, 0, nf * ne * ld * sizeof(zd)));
+    HC(h, hipMemset(h->vecs.Gobj, 0, nf * ne * ld * h->esz));
     HC(h, hipMemcpy(h->d_owners, h->owners.data(), h->owners.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HC(h, hipMemcpy(h->d_pos, h->pos.data(), h->pos.size() * sizeof(double), hipMemcpyHostToDevice));
     HC(h, hipHostMalloc((void**)&h->h_pin_i, kPinInts * sizeof(int32_t), hipHostMallocDefault));
@@ -1305,10 +1481,10 @@ int pngf_init_state(pngf_handle* h) {
         return fail(h, PNGF_ERR_ARG, "design too large for the staging buffers");
     h->acc_stride = 2 * gmax * gmax + 2 * gmax + 4 + ne * gmax;
     if (gmax > kLdsGmax)
-        CK(dalloc_t(h, &h->d_lu, (size_t)std::max(G.n, 1) * nf * gmax * (2 * gmax + 1)));
-    CK(dalloc_t(h, &h->d_acc, (size_t)nf * h->acc_stride));
+        CK(zalloc(h, &h->d_lu, (size_t)std::max(G.n, 1) * nf * gmax * (2 * gmax + 1)));
+    CK(zalloc(h, &h->d_acc, (size_t)nf * h->acc_stride));
     if (h->cfg.method == PNGF_METHOD_LOWRANK)
-        CK(dalloc_t(h, &h->vecs.Dblk, (size_t)nf * G.n * gmax * gmax));
+        CK(zalloc(h, &h->vecs.Dblk, (size_t)nf * G.n * gmax * gmax));
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
     h->asym = 0.0;
     h->sym = false;
@@ -1323,8 +1499,8 @@ int pngf_init_state(pngf_handle* h) {
         size_t freeb = 0, totb = 0;
         HC(h, hipMemGetInfo(&freeb, &totb));
         // keep room for the scratch strips and for one checkpoint copy of W0
-        const size_t scratch = (size_t)nf * N * gmax * sizeof(zd) * 2 + (size_t)nf * ld * N * sizeof(zd) +
-                               (size_t)kMChunk * ld * sizeof(zd) + (64u << 20);
+        const size_t scratch = (size_t)nf * N * gmax * h->esz * 2 + (size_t)nf * ld * N * h->esz +
+                               (size_t)kMChunk * ld * h->esz + (64u << 20);
         size_t avail = freeb > scratch ? freeb - scratch : 0;
         if (h->cfg.mem_limit > 0) {
             size_t lim = (size_t)h->cfg.mem_limit > h->mem_used + scratch ? (size_t)h->cfg.mem_limit - h->mem_used - scratch : 0;
@@ -1339,12 +1515,12 @@ int pngf_init_state(pngf_handle* h) {
         // k x k Lam of their accept: a factor column then costs one A column (twice the capacity;
         // the gather that rebuilds M[:, t] costs ~15 us more per accept than reading stored M).
         const size_t want = (size_t)maxdef * gmax;
-        const size_t per_col_m = (size_t)nf * 2 * ld * sizeof(zd);
+        const size_t per_col_m = (size_t)nf * 2 * ld * h->esz;
         const bool capped = (size_t)(0.85 * avail) / per_col_m < want;
         h->lam = h->sym && gmax <= kLdsGmax && h->cfg.factors != PNGF_FACTORS_STORED &&
                  (capped || h->cfg.factors == PNGF_FACTORS_REBUILT);
         h->d.lam = h->lam ? 1 : 0;
-        const size_t per_col = h->lam ? (size_t)nf * (ld + gmax) * sizeof(zd) : per_col_m;
+        const size_t per_col = h->lam ? (size_t)nf * (ld + gmax) * h->esz : per_col_m;
         size_t cols = std::min<size_t>(want, (size_t)(0.85 * avail) / per_col);
         cols = cols / gmax * gmax;
         if (cols < (size_t)gmax) return fail(h, PNGF_ERR_NOMEM, "no device memory for the deferred factors");
@@ -1352,29 +1528,29 @@ int pngf_init_state(pngf_handle* h) {
         h->d.Kmax = h->Kmax;
         if (h->lam) h->Lam.assign(nf, nullptr);
         for (int f = 0; f < nf; f++) {
-            CK(dalloc_t(h, &h->Aall[f], (size_t)N * h->Kmax));
-            if (h->lam) CK(dalloc_t(h, &h->Lam[f], (size_t)h->Kmax * gmax));
-            else CK(dalloc_t(h, &h->Mall[f], (size_t)h->Kmax * ld));
+            CK(zalloc(h, &h->Aall[f], (size_t)N * h->Kmax));
+            if (h->lam) CK(zalloc(h, &h->Lam[f], (size_t)h->Kmax * gmax));
+            else CK(zalloc(h, &h->Mall[f], (size_t)h->Kmax * ld));
         }
         if (h->lam) {
             CK(dalloc_t(h, &h->d_fstart, h->Kmax));
             CK(dalloc_t(h, &h->d_flen, h->Kmax));
-            CK(dalloc_t(h, &h->d_mchunk, (size_t)kMChunk * ld));
+            CK(zalloc(h, &h->d_mchunk, (size_t)kMChunk * ld));
             HC(h, hipMemset(h->d_fstart, 0, sizeof(int32_t) * h->Kmax));
             HC(h, hipMemset(h->d_flen, 0, sizeof(int32_t) * h->Kmax));
         }
         h->small_stride = (size_t)h->Kmax * gmax;
-        CK(dalloc_t(h, &h->d_Mc, nf * h->small_stride));
-        CK(dalloc_t(h, &h->d_Ar, nf * h->small_stride));
+        CK(zalloc(h, &h->d_Mc, nf * h->small_stride));
+        CK(zalloc(h, &h->d_Ar, nf * h->small_stride));
         h->bt_stride = (size_t)N * gmax;
-        CK(dalloc_t(h, &h->d_Bt, nf * h->bt_stride));
+        CK(zalloc(h, &h->d_Bt, nf * h->bt_stride));
         for (int f = 0; f < nf; f++) {
-            HC(h, hipMemset(h->Aall[f], 0, sizeof(zd) * (size_t)N * h->Kmax));
-            if (h->lam) HC(h, hipMemset(h->Lam[f], 0, sizeof(zd) * (size_t)h->Kmax * gmax));
-            else HC(h, hipMemset(h->Mall[f], 0, sizeof(zd) * (size_t)h->Kmax * ld));
+            HC(h, hipMemset(h->Aall[f], 0, h->esz * (size_t)N * h->Kmax));
+            if (h->lam) HC(h, hipMemset(h->Lam[f], 0, h->esz * (size_t)h->Kmax * gmax));
+            else HC(h, hipMemset(h->Mall[f], 0, h->esz * (size_t)h->Kmax * ld));
         }
-        HC(h, hipMemset(h->d_Mc, 0, sizeof(zd) * nf * h->small_stride));
-        HC(h, hipMemset(h->d_Ar, 0, sizeof(zd) * nf * h->small_stride));
+        HC(h, hipMemset(h->d_Mc, 0, h->esz * nf * h->small_stride));
+        HC(h, hipMemset(h->d_Ar, 0, h->esz * nf * h->small_stride));
     } else {
         h->d.Kmax = 0;
         h->v_stride = (size_t)gmax * ld;
@@ -1565,11 +1741,18 @@ int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out) {
     const int N = h->d.N;
     const size_t ld = h->d.ldN;
     zd* tmp = nullptr;
-    CK(dalloc_t(h, &tmp, ld * N));
-    HC(h, hipMemcpyAsync(tmp, h->W0[f], ld * N * sizeof(zd), hipMemcpyDeviceToDevice, h->st));
+    CK(zalloc(h, &tmp, ld * N));
+    HC(h, hipMemcpyAsync(tmp, h->W0[f], ld * N * h->esz, hipMemcpyDeviceToDevice, h->st));
     if (h->K > 0) CK(fold_into(h, f, tmp));
     HC(h, hipStreamSynchronize(h->st));
-    HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
+    if (h->c64) {
+        // complex64 state, returned widened to complex128
+        std::vector<zf> w((size_t)N * N);
+        HC(h, hipMemcpy2D(w.data(), N * sizeof(zf), tmp, ld * sizeof(zf), N * sizeof(zf), N, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < w.size(); i++) { out[i].re = w[i].x; out[i].im = w[i].y; }
+    } else {
+        HC(h, hipMemcpy2D(out, N * sizeof(zd), tmp, ld * sizeof(zd), N * sizeof(zd), N, hipMemcpyDeviceToHost));
+    }
     dfree(h, tmp);
     return PNGF_OK;
 }
@@ -1601,21 +1784,23 @@ int pngf_checkpoint(pngf_handle* h) {
     const size_t ld = d.ldN, big = ld * d.N;
     const bool lr = h->cfg.method == PNGF_METHOD_LOWRANK;
     if (lr) CK(flush_factors(h));
+    // the snapshot's vector part: q (or S), R in the state's precision, then Eobj in complex128
+    const size_t qb = (size_t)d.nf * ld * h->esz, rb = (size_t)d.nf * d.ne * ld * h->esz;
     if (!h->ckpt) {
         h->ck_big.assign(d.nf, nullptr);
-        for (int f = 0; f < d.nf; f++) CK(dalloc_t(h, &h->ck_big[f], big));
-        CK(dalloc_t(h, &h->ck_vec, (size_t)d.nf * ld * (2 + d.ne) + (size_t)d.nf * d.ne));
-        if (lr) CK(dalloc_t(h, &h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax));
+        for (int f = 0; f < d.nf; f++) CK(zalloc(h, &h->ck_big[f], big));
+        CK(dalloc(h, (void**)&h->ck_vec, qb + rb + (size_t)d.nf * d.ne * sizeof(zd)));
+        if (lr) CK(zalloc(h, &h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax));
         CK(dalloc_t(h, &h->ck_P, d.N));
     }
     hipStream_t st = h->st;
     for (int f = 0; f < d.nf; f++)
-        HC(h, hipMemcpyAsync(h->ck_big[f], lr ? h->W0[f] : h->Cinv[f], big * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    zd* v = h->ck_vec;
-    HC(h, hipMemcpyAsync(v, lr ? h->vecs.q : h->vecs.S, d.nf * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    HC(h, hipMemcpyAsync(v + d.nf * ld, h->vecs.R, d.nf * d.ne * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    HC(h, hipMemcpyAsync(v + d.nf * ld * (1 + d.ne), h->vecs.Eobj, d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    if (lr) HC(h, hipMemcpyAsync(h->ck_D, h->vecs.Dblk, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * sizeof(zd), hipMemcpyDeviceToDevice, st));
+        HC(h, hipMemcpyAsync(h->ck_big[f], lr ? h->W0[f] : h->Cinv[f], big * h->esz, hipMemcpyDeviceToDevice, st));
+    char* v = (char*)h->ck_vec;
+    HC(h, hipMemcpyAsync(v, lr ? h->vecs.q : h->vecs.S, qb, hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(v + qb, h->vecs.R, rb, hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(v + qb + rb, h->vecs.Eobj, d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    if (lr) HC(h, hipMemcpyAsync(h->ck_D, h->vecs.Dblk, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * h->esz, hipMemcpyDeviceToDevice, st));
     HC(h, hipMemcpyAsync(h->ck_P, h->d_P, d.N, hipMemcpyDeviceToDevice, st));
     HC(h, hipStreamSynchronize(st));
     h->ck_tile = h->tile;
@@ -1632,13 +1817,14 @@ int pngf_restore(pngf_handle* h) {
     const size_t ld = d.ldN, big = ld * d.N;
     const bool lr = h->cfg.method == PNGF_METHOD_LOWRANK;
     hipStream_t st = h->st;
+    const size_t qb = (size_t)d.nf * ld * h->esz, rb = (size_t)d.nf * d.ne * ld * h->esz;
     for (int f = 0; f < d.nf; f++)
-        HC(h, hipMemcpyAsync(lr ? h->W0[f] : h->Cinv[f], h->ck_big[f], big * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    zd* v = h->ck_vec;
-    HC(h, hipMemcpyAsync(lr ? h->vecs.q : h->vecs.S, v, d.nf * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    HC(h, hipMemcpyAsync(h->vecs.R, v + d.nf * ld, d.nf * d.ne * ld * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    HC(h, hipMemcpyAsync(h->vecs.Eobj, v + d.nf * ld * (1 + d.ne), d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
-    if (lr) HC(h, hipMemcpyAsync(h->vecs.Dblk, h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * sizeof(zd), hipMemcpyDeviceToDevice, st));
+        HC(h, hipMemcpyAsync(lr ? h->W0[f] : h->Cinv[f], h->ck_big[f], big * h->esz, hipMemcpyDeviceToDevice, st));
+    const char* v = (const char*)h->ck_vec;
+    HC(h, hipMemcpyAsync(lr ? h->vecs.q : h->vecs.S, v, qb, hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->vecs.R, v + qb, rb, hipMemcpyDeviceToDevice, st));
+    HC(h, hipMemcpyAsync(h->vecs.Eobj, v + qb + rb, d.nf * d.ne * sizeof(zd), hipMemcpyDeviceToDevice, st));
+    if (lr) HC(h, hipMemcpyAsync(h->vecs.Dblk, h->ck_D, (size_t)d.nf * d.ngroups * d.gmax * d.gmax * h->esz, hipMemcpyDeviceToDevice, st));
     HC(h, hipMemcpyAsync(h->d_P, h->ck_P, d.N, hipMemcpyDeviceToDevice, st));
     h->tile = h->ck_tile;
     h->P = h->ck_Ph;
@@ -1735,7 +1921,9 @@ int device_loop_prepare(pngf_handle* h, const pngf_dbs_params* p, double F_best,
 
 // Iteration i of a chunk.  With profiling on, events bracket the column strip and the control
 // kernel records (pending, k, K) for device_loop_collect.
-void device_loop_iteration(pngf_handle* h, int i) {
+extern "C++" template <class Z>
+void device_loop_iteration_t(pngf_handle* h, int i) {
+    const LRView<Z> w = lrview<Z>(h);
     hipStream_t st = h->st;
     const bool tm = h->prof && h->d_iterlog;
     const Dims& d = h->d;
@@ -1759,24 +1947,22 @@ void device_loop_iteration(pngf_handle* h, int i) {
     c.acc_group = h->d_accg;
     c.acc_eval = h->d_acce;
     c.acc_F = h->d_accF;
-    launch_lr_eval(st, d, h->vecs, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, ncl, h->d_eobj_out,
-                   h->d_info, h->d_lu, &ls->nb);
+    launch_lr_eval(st, d, w.v, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, ncl, h->d_eobj_out,
+                   h->d_info, w.lu, &ls->nb);
     launch_loop_control(st, d, gtab(h), h->d_tile, h->d_P, h->d_owners, h->d_batch, h->d_eobj_out, h->d_info, c,
                         tm ? h->d_iterlog + 3 * i : nullptr);
     (void)hipEventRecord(h->ev_fork, st);
     (void)hipStreamWaitEvent(h->st2, h->ev_fork, 0);
-    launch_lr_accept_prep(h->st2, d, h->vecs, 0, dt, ds, dlp, d.gmax, h->d_acc, h->acc_stride, h->d_info, h->d_lu, dyn,
-                          &ls->g);
+    launch_lr_accept_prep(h->st2, d, w.v, 0, dt, ds, dlp, d.gmax, w.acc, h->acc_stride, h->d_info, w.lu, dyn, &ls->g);
     (void)hipEventRecord(h->ev_join, h->st2);
     // host k = gmax selects the column-tile count; the kernels read the actual k and K
-    launch_lr_gather(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, h->sym ? nullptr : h->d_Ar, (int)h->small_stride, dyn);
+    launch_lr_gather(st, d, w.tb, dt, d.gmax, 0, w.Mc, h->sym ? (Z*)nullptr : w.Ar, (int)h->small_stride, dyn);
     if (tm) (void)hipEventRecord(h->lev[2 * i], st);
-    launch_lr_colblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Mc, (int)h->small_stride, h->skv, dyn);
+    launch_lr_colblk(st, d, w.tb, dt, d.gmax, 0, w.Mc, (int)h->small_stride, h->skv, dyn);
     if (tm) (void)hipEventRecord(h->lev[2 * i + 1], st);
     if (!h->sym) {
         if (tm) (void)hipEventRecord(h->levr[2 * i], st);
-        launch_lr_rowblk(st, d, h->tabs, dt, d.gmax, 0, h->d_Ar, (int)h->small_stride, h->d_Bt, h->bt_stride, h->skv,
-                         dyn);
+        launch_lr_rowblk(st, d, w.tb, dt, d.gmax, 0, w.Ar, (int)h->small_stride, w.Bt, h->bt_stride, h->skv, dyn);
         if (tm) (void)hipEventRecord(h->levr[2 * i + 1], st);
     }
     (void)hipStreamWaitEvent(st, h->ev_join, 0);
@@ -1785,8 +1971,13 @@ void device_loop_iteration(pngf_handle* h, int i) {
     fu.tile = h->d_tile;
     fu.t = dt;
     fu.gtiles = dgt;
-    launch_lr_finish(st, d, h->tabs, h->vecs, gtab(h), h->d_acc, h->acc_stride, h->d_Bt, h->bt_stride, ds, d.gmax, 0,
-                     dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv, fu);
+    launch_lr_finish(st, d, w.tb, w.v, gtab(h), (const Z*)w.acc, h->acc_stride, (const Z*)w.Bt, h->bt_stride, ds,
+                     d.gmax, 0, dyn, h->part ? h->d_left : nullptr, h->nleft, h->ngroups_drv, fu);
+}
+
+void device_loop_iteration(pngf_handle* h, int i) {
+    if (h->c64) device_loop_iteration_t<zf>(h, i);
+    else device_loop_iteration_t<zd>(h, i);
 }
 
 // Captures kLoopChunk iterations into h->loop_exec once (after device_loop_prepare); a capture
