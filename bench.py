@@ -196,16 +196,60 @@ def cpu_baseline(wl, g0, cinv0, budget_s, gpu_run):
 
 
 def copy_probe():
-    """Three device-to-device copies of 1 GiB: the reference reading for the HBM byte
-    counters (FETCH_SIZE / WRITE_SIZE) that scale the kernels' measured traffic."""
-    import torch
-    n = 1 << 27
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+    """Three device-to-device copies of 1 GiB (the runtime's copy kernel): the reference reading
+    for the HBM byte counters (FETCH_SIZE / WRITE_SIZE) that scale the kernels' measured traffic."""
+    import ctypes as ct
+    try:
+        hip = ct.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ct.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    hip.hipMemset.argtypes = [ct.c_void_p, ct.c_int, ct.c_size_t]
+    hip.hipFree.argtypes = [ct.c_void_p]
+    n = 1 << 30
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(a), n) == 0 and hip.hipMalloc(ct.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 8, "copies": 3}))
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
+
+
+def counter_probe(steps=600):
+    """The workload's column-strip launches in a process without rocSOLVER work, for HBM and MFMA
+    counter passes (rocprofv3 --pmc ... --kernel-include-regex lr_skinny): the 21x21-tile,
+    9-frequency problem from the empty design (no metal component: C = I, nothing to factor), whose
+    init runs the autotuner's strips at factor depth K = 1536 and whose DBS (the first `steps`
+    evaluations) accepts flips at K = 24 j.  A strip's work depends on N, K, k and the frequency
+    count only.  Prints one JSON line: the dispatches' (K, algorithmic bytes, flops) in order."""
+    wl = WORKLOADS["antenna21x21_9f"]
+    T, freqs = wl["T"], wl["freqs"]
+    opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=P.METHOD_LOWRANK)
+    for f, hz in enumerate(freqs):
+        opt.set_frequency(f, hz)
+        opt.synth_g(f)
+    mid = (T - 1) // 2
+    opt.set_tile_map(np.zeros(T * T, dtype=np.int32))
+    opt.set_source_port(mid, mid, 1, 2)
+    opt.init_state()
+    tune = opt.autotune_result()
+    # host-driven loop: one strip dispatch per accept, no graph replay under the counter passes
+    r = opt.dbs_run(max_evals=steps, loop=P.LOOP_HOST)
+    N, nf, k = opt.N, len(freqs), 24
+    tune_K = min(opt.factor_capacity()[0] - k, 64 * k)
+
+    def algo(K):
+        return dict(K=K, bytes=16.0 * nf * (N * K + 2 * N * k + K * k), flops=8.0 * nf * N * K * k)
+    # the autotuner times every variant with 3 dispatches each, in variant order; the run then
+    # dispatches the chosen one once per accept
+    out = dict(N=N, freqs=nf, k=k, autotune_variants=len(tune["ms"]), autotune_chosen=tune["chosen"],
+               autotune_K=tune_K, autotune=algo(tune_K), accepts=[algo(k * j) for j in range(r["accepts"])],
+               flip_count=r["flip_count"])
+    print(json.dumps(out))
 
 
 def bench_multi(workers=4):
@@ -264,6 +308,8 @@ def main():
                     help="skinny-product tile shape: 0 = autotune at init, v+1 = use variant v")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
+    ap.add_argument("--counter-probe", action="store_true",
+                    help="only run the column strips of the 21x21 x 9 workload from the empty design (counter passes)")
     ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
@@ -276,6 +322,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.counter_probe:
+        counter_probe()
         return
     if args.workload == "multi16":
         bench_multi(args.workers)

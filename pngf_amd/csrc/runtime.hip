@@ -592,19 +592,23 @@ int init_freq(pngf_handle* h, int f) {
     for (int r = 1; r < ne; r++)
         HC(h, hipMemcpyAsync(gobj + (size_t)r * ld, h->gobj_extra[f].data() + (size_t)(r - 1) * N, N * sizeof(zd),
                              hipMemcpyHostToDevice, st));
-    // C^-1 in place (linalg.hpp:49-84: zgetrf + zgetri)
-    int* ipiv = nullptr;
-    CK(dalloc_t(h, &ipiv, N));
-    RB(h, rocsolver_zgetrf(h->rb, N, N, (rbz*)C, ld, ipiv, h->d_info));
-    int info = 0;
-    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-    HC(h, hipStreamSynchronize(st));
-    if (info != 0) { dfree(h, ipiv); dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetrf info " + std::to_string(info) + ")"); }
-    RB(h, rocsolver_zgetri(h->rb, N, (rbz*)C, ld, ipiv, h->d_info));
-    HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-    HC(h, hipStreamSynchronize(st));
-    dfree(h, ipiv);
-    if (info != 0) { dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetri info " + std::to_string(info) + ")"); }
+    // C^-1 in place (linalg.hpp:49-84: zgetrf + zgetri).  A design with no metal component has
+    // C = I (build_C wrote the identity), which is its own inverse: nothing to factor.
+    const bool all_void = std::find(h->P.begin(), h->P.end(), (int8_t)1) == h->P.end();
+    if (!all_void) {
+        int* ipiv = nullptr;
+        CK(dalloc_t(h, &ipiv, N));
+        RB(h, rocsolver_zgetrf(h->rb, N, N, (rbz*)C, ld, ipiv, h->d_info));
+        int info = 0;
+        HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+        HC(h, hipStreamSynchronize(st));
+        if (info != 0) { dfree(h, ipiv); dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetrf info " + std::to_string(info) + ")"); }
+        RB(h, rocsolver_zgetri(h->rb, N, (rbz*)C, ld, ipiv, h->d_info));
+        HC(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+        HC(h, hipStreamSynchronize(st));
+        dfree(h, ipiv);
+        if (info != 0) { dfree(h, C); return fail(h, PNGF_ERR_SINGULAR, "C is singular (zgetri info " + std::to_string(info) + ")"); }
+    }
     // PEinc, R = Gobj C^-1 (row-major ne x ld: R^T = C^-T Gobj^T), Eobj = R PEinc
     launch_pe(st, N, h->d_P, einc, h->d_pe);
     zd* R = h->vecs.R + (size_t)f * ne * ld;

@@ -231,6 +231,27 @@ def test_dbs_ill_conditioned_identical_sequence(method):
     assert rel(g["acc_F"], o["acc_F"]) < RTOL_ILL
 
 
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_empty_design_init(method):
+    """No metal component: C = I is its own inverse and init_state skips the factorisation.  The
+    state is the closed form (Eobj = 0: PEinc = 0) and scores and DBS follow the oracle, which
+    factors the identity."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, method, tm=np.zeros(T * T, dtype=np.int32))
+    assert np.all(gpu.state()["Eobj"] == 0)
+    cands = candidates(P.LAYOUT_DISJOINT, T)
+    F, e = gpu.eval_batch([[t] for t in cands])
+    for i, t in enumerate(cands):
+        ref = np.array([o.compute_updated_Eobj([t]) for o in ors])
+        assert rel(e[i], ref) < RTOL
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11, max_evals=150)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    gpu.close()
+
+
 def test_checkpoint_restore_repeats_run():
     T = 7
     freqs = [30e9]
