@@ -36,6 +36,14 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
+HEADLINE_ACCEPTS = 183  # accepts of the default workload's run (deterministic)
+# rocprofv3 --pmc counts of the column strip over the default run's 183 launches (python3 bench.py
+# --counter-probe under FETCH_SIZE / WRITE_SIZE / MfmaUtil passes, summarised by --counter-summary;
+# profiles/colblk_counters_antenna21x21_9f.txt)
+STRIP_COUNTED = dict(N=10590, freqs=9, variant=4, launches=HEADLINE_ACCEPTS, alg_bytes_per_launch=3411258624.0,
+                     bytes_per_launch=3507488159.6, mfma_util=0.7615, clock_ghz=2.223,
+                     source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (copy-calibrated), MfmaUtil over the same "
+                            "183 strip launches (profiles/colblk_counters_antenna21x21_9f.txt)")
 
 
 def log(*a):
@@ -59,7 +67,7 @@ def make(wl, method, max_deferred=0, symmetry=P.SYM_AUTO, tile_variant=0, pertur
     return opt, time.perf_counter() - t0
 
 
-def roofline(opt, gmax=24):
+def roofline(opt, variant=None):
     """Dominant kernel category by event-timed duration -> roofline record."""
     n_comp = opt.N
     best = None
@@ -84,13 +92,16 @@ def roofline(opt, gmax=24):
                    traffic=None)
     rec.update(kernel=P.K_NAMES[k], avg_ms=avg_ms, launches=s["launches"], algo_bytes=bytes_l,
                algo_flops=flops_l)
-    if P.K_NAMES[k] in ("colblk", "rowblk"):
-        # HBM bytes per launch: the algorithmic bytes plus the staged K x k factor block that the
-        # 7 other XCD L2 caches fetch again, a factor 1 + 7k/N that matches rocprofv3 FETCH_SIZE /
-        # WRITE_SIZE counts (copy-calibrated) at N = 10590 and N = 2910
-        # (profiles/colblk_hbm_traffic_antenna21x21.txt); not counted in this run
-        rec["traffic"] = bytes_l * (1.0 + 7.0 * gmax / n_comp)
-        rec["traffic_source"] = "model 1 + 7k/N fitted to rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE"
+    c = STRIP_COUNTED
+    if (P.K_NAMES[k] == "colblk" and n_comp == c["N"] and opt.nf == c["freqs"] and variant == c["variant"]
+            and s["launches"] % c["launches"] == 0 and abs(bytes_l / c["alg_bytes_per_launch"] - 1.0) < 1e-9):
+        # the same launches (N, frequencies, k, K = 0, 24, ..., 24 (L - 1), tile variant) as the
+        # counted strip probe: HBM bytes per launch from its copy-calibrated FETCH_SIZE + WRITE_SIZE,
+        # and its MFMA-busy fraction (MfmaUtil) and clock beside the event-timed fraction of peak
+        rec["traffic"] = c["bytes_per_launch"]
+        rec["traffic_source"] = c["source"]
+        rec["mfma_busy"] = c["mfma_util"]
+        rec["clock_ghz"] = c["clock_ghz"]
     allk = {P.K_NAMES[i]: opt.kernel_stats(i) for i in range(6)}
     return rec, allk
 
@@ -219,13 +230,14 @@ def copy_probe():
     print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
-def counter_probe(steps=600):
+def counter_probe(steps=1500):
     """The workload's column-strip launches in a process without rocSOLVER work, for HBM and MFMA
     counter passes (rocprofv3 --pmc ... --kernel-include-regex lr_skinny): the 21x21-tile,
     9-frequency problem from the empty design (no metal component: C = I, nothing to factor), whose
     init runs the autotuner's strips at factor depth K = 1536 and whose DBS (the first `steps`
-    evaluations) accepts flips at K = 24 j.  A strip's work depends on N, K, k and the frequency
-    count only.  Prints one JSON line: the dispatches' (K, algorithmic bytes, flops) in order."""
+    evaluations) accepts flips at K = 24 j, the factor depths of the benchmark's run.
+    A strip's work depends on N, K, k and the frequency count only.  Prints one JSON line: the
+    dispatches' (K, algorithmic bytes, flops) in order."""
     wl = WORKLOADS["antenna21x21_9f"]
     T, freqs = wl["T"], wl["freqs"]
     opt = P.Optimizer(wl["layout"], T, num_freq=len(freqs), method=P.METHOD_LOWRANK)
@@ -240,6 +252,13 @@ def counter_probe(steps=600):
     # host-driven loop: one strip dispatch per accept, no graph replay under the counter passes
     r = opt.dbs_run(max_evals=steps, loop=P.LOOP_HOST)
     N, nf, k = opt.N, len(freqs), 24
+    # the run converges before the benchmark's accept count: further explicit accepts (single
+    # tiles in raster order, the port tile skipped) continue the factor depths K = 24 j
+    extra = max(0, HEADLINE_ACCEPTS + 8 - r["accepts"])
+    tiles = [t for t in range(T * T) if t != mid * T + mid]
+    for i in range(extra):
+        opt.accept([tiles[i % len(tiles)]])
+    opt.state()  # a read back: the stream has drained
     tune_K = min(opt.factor_capacity()[0] - k, 64 * k)
 
     def algo(K):
@@ -247,9 +266,92 @@ def counter_probe(steps=600):
     # the autotuner times every variant with 3 dispatches each, in variant order; the run then
     # dispatches the chosen one once per accept
     out = dict(N=N, freqs=nf, k=k, autotune_variants=len(tune["ms"]), autotune_chosen=tune["chosen"],
-               autotune_K=tune_K, autotune=algo(tune_K), accepts=[algo(k * j) for j in range(r["accepts"])],
-               flip_count=r["flip_count"])
+               autotune_K=tune_K, autotune=algo(tune_K), accepts=[algo(k * j) for j in range(r["accepts"] + extra)],
+               flip_count=r["flip_count"], dbs_accepts=r["accepts"])
     print(json.dumps(out))
+
+
+def _pmc_csv(path):
+    """rocprofv3 --pmc ... -f csv output under `path`: per dispatch (in order) its kernel name and
+    counter values (summed over the rows of one counter)."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)
+    if len(files) != 1:
+        raise SystemExit("expected one counter_collection.csv under %s, found %d" % (path, len(files)))
+    per = {}
+    with open(files[0]) as fh:
+        for row in csv.DictReader(fh):
+            d = per.setdefault(int(row["Dispatch_Id"]), {
+                "name": row["Kernel_Name"], "ns": int(row["End_Timestamp"]) - int(row["Start_Timestamp"])})
+            d[row["Counter_Name"]] = d.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    return [per[i] for i in sorted(per)]
+
+
+def counter_summary(path):
+    """Summary of the counter passes over --counter-probe and --copy-probe, read from `path`:
+    probe.json (the probe's stdout) and the rocprofv3 CSV folders pmc_copy_fetch, pmc_copy_write,
+    pmc_strip_fetch, pmc_strip_write and pmc_strip_mfma.  HBM bytes = FETCH_SIZE / WRITE_SIZE
+    (KiB) x 1024 x the factor that makes the 1 GiB copies read and write 1 GiB.  Prints a text
+    table and, last, one JSON line with the per-launch figures over the first L dispatches of the
+    run (the benchmark's L = 183 launches and all of the probe's)."""
+    with open(os.path.join(path, "probe.json")) as fh:
+        probe = json.load(fh)
+    cf, cw = _pmc_csv(os.path.join(path, "pmc_copy_fetch")), _pmc_csv(os.path.join(path, "pmc_copy_write"))
+    gib_kib = float(1 << 20)
+    f_fetch = gib_kib * len(cf) / sum(d["FETCH_SIZE"] for d in cf)
+    f_write = gib_kib * len(cw) / sum(d["WRITE_SIZE"] for d in cw)
+    print("copy calibration (%d x 1 GiB %s): FETCH_SIZE %.1f KiB per copy -> factor %.3f; WRITE_SIZE %.1f KiB "
+          "-> factor %.3f" % (len(cf), cf[0]["name"], gib_kib / f_fetch, f_fetch, gib_kib / f_write, f_write))
+    sf, sw, sm = (_pmc_csv(os.path.join(path, n)) for n in ("pmc_strip_fetch", "pmc_strip_write", "pmc_strip_mfma"))
+    nv, N, nf, k = probe["autotune_variants"], probe["N"], probe["freqs"], probe["k"]
+    acc = probe["accepts"]
+    if not (len(sf) == len(sw) == len(sm) == 3 * nv + len(acc)):
+        raise SystemExit("dispatch counts %d/%d/%d, expected %d" % (len(sf), len(sw), len(sm), 3 * nv + len(acc)))
+    rows = []
+    for i, (a, b, c) in enumerate(zip(sf, sw, sm)):
+        al = probe["autotune"] if i < 3 * nv else acc[i - 3 * nv]
+        rows.append(dict(K=al["K"], rd=a["FETCH_SIZE"] * 1024 * f_fetch, wr=b["WRITE_SIZE"] * 1024 * f_write,
+                         alg=al["bytes"], algf=al["flops"], mfma_flops=c["SQ_INSTS_VALU_MFMA_MOPS_F64"] * 512,
+                         busy=c["SQ_VALU_MFMA_BUSY_CYCLES"], active=c["GRBM_GUI_ACTIVE"], util=c["MfmaUtil"] / 100,
+                         ns=c["ns"], name=a["name"]))
+    wr_alg = 16.0 * nf * N * k
+    print("\ncolumn strip, N = %d, %d frequencies, k = %d\nautotuner: 3 dispatches per variant at K = %d "
+          "(algorithmic read %.0f KiB, write %.0f KiB)" % (N, nf, k, probe["autotune_K"],
+                                                         (probe["autotune"]["bytes"] - wr_alg) / 1024, wr_alg / 1024))
+    print("  variant  read/alg  write/alg  MFMA flops/alg  MfmaUtil  kernel")
+    for i in range(3 * nv):
+        r = rows[i]
+        nm = r["name"]
+        nm = nm[nm.find("<"):nm.find(">") + 1] if "<" in nm else nm
+        print("  %d.%d      %.4f    %.4f     %.4f          %.3f     %s" % (
+            i // 3, i % 3, r["rd"] / (r["alg"] - wr_alg), r["wr"] / wr_alg, r["mfma_flops"] / r["algf"], r["util"], nm))
+    dbs = rows[3 * nv:]
+    print("\nDBS from the empty design, chosen variant %d: %d strips at K = 0, 24, ..., %d" % (
+        probe["autotune_chosen"], len(dbs), dbs[-1]["K"]))
+    print("  K      (read+write)/alg  MFMA flops/alg  MfmaUtil  us     GHz")
+    for r in dbs[::max(1, len(dbs) // 16)]:
+        print("  %5d  %.4f            %.4f          %.3f     %6.1f  %.3f" % (
+            r["K"], (r["rd"] + r["wr"]) / r["alg"], r["mfma_flops"] / max(r["algf"], 1.0), r["util"], r["ns"] * 1e-3,
+            r["active"] / 8.0 / r["ns"]))
+    fit = dict(N=N, freqs=nf, k=k, variant=probe["autotune_chosen"])
+    # the benchmark's run launches the strip once per accept at K = 0, 24, ..., 24 (L - 1): the
+    # probe's first L dispatches are the same launches
+    fit["prefix"] = {}
+    for L in sorted({L for L in (HEADLINE_ACCEPTS, len(dbs)) if L <= len(dbs)}):
+        sel = dbs[:L]
+        t = sum(r["rd"] + r["wr"] for r in sel)
+        u = sum(r["util"] * r["active"] for r in sel) / sum(r["active"] for r in sel)
+        alg = sum(r["alg"] for r in sel)
+        fl = sum(r["mfma_flops"] for r in sel) / sum(r["algf"] for r in sel)
+        # GRBM_GUI_ACTIVE counts the cycles of each of the 8 XCDs: / 8 / dispatch time = clock
+        ghz = sum(r["active"] for r in sel) / 8.0 / sum(r["ns"] for r in sel)
+        fit["prefix"][str(L)] = dict(bytes_per_launch=t / L, alg_bytes_per_launch=alg / L, mfma_util=u,
+                                     mfma_flops_per_alg=fl, clock_ghz=ghz)
+        print("first %d strips (K = 0 .. %d): HBM bytes %.4f x algorithmic (%.6g B per launch), MfmaUtil %.3f "
+              "(active-cycle weighted), MFMA flops %.4f x algorithmic, clock %.3f GHz (GRBM_GUI_ACTIVE / 8 / "
+              "dispatch time)" % (L, sel[-1]["K"], t / alg, t / L, u, fl, ghz))
+    print(json.dumps(fit))
 
 
 def bench_multi(workers=4):
@@ -310,6 +412,8 @@ def main():
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration)")
     ap.add_argument("--counter-probe", action="store_true",
                     help="only run the column strips of the 21x21 x 9 workload from the empty design (counter passes)")
+    ap.add_argument("--counter-summary", metavar="DIR",
+                    help="only summarise the counter passes of --counter-probe / --copy-probe found under DIR")
     ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
@@ -325,6 +429,9 @@ def main():
         return
     if args.counter_probe:
         counter_probe()
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
         return
     if args.workload == "multi16":
         bench_multi(args.workers)
@@ -375,7 +482,7 @@ def main():
     last = runs[-1]
     roof, allk, strip_share = None, {}, None
     if profile:
-        roof, allk = roofline(opt)
+        roof, allk = roofline(opt, tune["chosen"] if tune else None)
         opt.profile(False)
         if roof:
             strip_share = roof["avg_ms"] * roof["launches"] * 1e-3 / loop_secs
