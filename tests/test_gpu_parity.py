@@ -346,12 +346,12 @@ def test_sweep_is_first_improvement_of_sequential_evals(driver):
 
 
 @pytest.mark.parametrize("layout", [P.LAYOUT_DISJOINT, P.LAYOUT_SHARED])
-def test_wave_scoring_matches_workgroup_scoring(layout):
-    """All 225 candidates of 15x15 tiles at 10 frequencies in one batch (2250 pairs: scored one
-    pair per wave, the capacitance matrix in registers) and in two batches of 113 and 112
-    (1130 pairs or fewer: the two-wave workgroup kernel, matrix in LDS): bitwise the same
-    scores, on the default synthetic G (cond(C) ~ 1e5, where any change in rounding shows)."""
-    T, nf = 15, 10
+@pytest.mark.parametrize("T,nf", [(15, 10), (17, 9)])
+def test_wave_scoring_matches_workgroup_scoring(layout, T, nf):
+    """All candidates of T x T tiles at 10 or 9 frequencies in one batch (2240 or 2592 pairs:
+    scored one pair per wave, the capacitance matrix in registers) and in two batches under 2048
+    pairs each (the two-wave workgroup kernel, matrix in LDS): bitwise the same scores, on the
+    default synthetic G (cond(C) ~ 1e5, where any change in rounding shows)."""
     gpu = P.Optimizer(layout, T, num_freq=nf, method=P.METHOD_LOWRANK)
     for f, hz in enumerate(freq_list(nf, 26e9, 34e9)):
         gpu.set_frequency(f, hz)
@@ -360,10 +360,11 @@ def test_wave_scoring_matches_workgroup_scoring(layout):
     gpu.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
     gpu.init_state()
     cands = [[t] for t in range(T * T) if t != (T - 1) // 2 * (T + 1)]
+    h = len(cands) // 2
     F1, e1 = gpu.eval_batch(cands)
-    F2a, e2a = gpu.eval_batch(cands[:113])
-    F2b, e2b = gpu.eval_batch(cands[113:])
-    assert len(cands) * nf >= 2048 and 113 * nf < 2048
+    F2a, e2a = gpu.eval_batch(cands[:h])
+    F2b, e2b = gpu.eval_batch(cands[h:])
+    assert len(cands) * nf >= 2048 and (len(cands) - h) * nf < 2048
     assert np.array_equal(F1, np.concatenate([F2a, F2b]))
     assert np.array_equal(e1, np.concatenate([e2a, e2b]))
     assert np.all(np.isfinite(F1))
