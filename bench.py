@@ -11,8 +11,10 @@ value = committed evaluations (the reference's flip_count) / seconds of the time
 each measured on the host around the DBS call (the device is idle when it starts and the
 call returns after synchronising its stream).  Printed as one JSON line with, beside the
 required fields: roofline (the column-strip kernel, timed by HIP events recorded on its
-stream around every launch of the timed steps; traffic from the copy-calibrated HBM counter
-model), cpu_baseline (the oracle on a bounded sample),
+stream around every launch of the timed steps; HBM traffic, MFMA busy and clock from rocprofv3
+counter passes over the same launches), cpu_baseline (the oracle on a bounded sample,
+projected to the run's counts), config_7x7_1f (the reference's CPU-runnable 7x7 case: the GPU
+beside the oracle),
 plain_fp64_gpu (the reference formulation on the GPU, same start), speedup_vs_plain,
 best_improvement_variant, time_to_solution_s, per-kernel-category statistics, autotuner
 choice and the symmetric-accept decision.
@@ -204,6 +206,65 @@ def cpu_baseline(wl, g0, cinv0, budget_s, gpu_run):
                        "run's %d evaluations and %d accepts at those times x %d frequencies (%.0f s)"
                        % (O.num_threads(), nf, r["flip_count"], r["accepts"], r["loop_s"], 1e3 * t_eval,
                           1e3 * t_acc, gpu_run["flip_count"], gpu_run["accepts"], nf, run_s))
+
+
+def config_small(steps=5):
+    """The reference's CPU-runnable case (7x7 tiles, N = 1182, one frequency at 30 GHz, the
+    benchmark's start convention): DBS to convergence on the GPU (LOWRANK, device-driven loop;
+    one warmup, then `steps` timed runs from a device checkpoint) and on the CPU (the oracle on
+    the device's G, read back through a PLAIN handle, one run).  Same accepted sequence expected; both rates in
+    evaluated flips/s."""
+    import oracle as O
+    wl = WORKLOADS["antenna7x7_1f"]
+    T, freqs = wl["T"], wl["freqs"]
+    mid = (T - 1) // 2
+    tm = initial_pattern(T, 0)
+    # the device's G for the oracle, read back from a PLAIN handle (the method that keeps G)
+    pl = P.Optimizer(wl["layout"], T, num_freq=1, method=P.METHOD_PLAIN)
+    try:
+        pl.set_frequency(0, freqs[0])
+        pl.synth_g(0)
+        pl.set_tile_map(tm)
+        pl.set_source_port(mid, mid, 1, 2)
+        pl.init_state()
+        G = pl.g(0)
+    finally:
+        pl.close()
+    opt = P.Optimizer(wl["layout"], T, num_freq=1, method=P.METHOD_LOWRANK)
+    try:
+        opt.set_frequency(0, freqs[0])
+        opt.synth_g(0)
+        opt.set_tile_map(tm)
+        opt.set_source_port(mid, mid, 1, 2)
+        opt.init_state()
+        opt.checkpoint()
+        opt.dbs_run()
+        secs, runs = 0.0, []
+        for _ in range(steps):
+            opt.restore()
+            t0 = time.perf_counter()
+            runs.append(opt.dbs_run())
+            secs += time.perf_counter() - t0
+    finally:
+        opt.close()
+    g = runs[-1]
+    o = O.Optimizer(wl["layout"], T, G, ne=1)
+    assert o.init_state(tm, (mid, mid, 1, 2)) == 0
+    c = O.dbs_run([o], O.OBJ_S11, freqs, DX, DY, DZ)
+    out = dict(workload="antenna7x7_1f", N=P.layout_size(wl["layout"], T),
+               gpu=dict(value=sum(r["flip_count"] for r in runs) / secs, seconds=secs / steps,
+                        flip_count=g["flip_count"], accepts=g["accepts"], F_final=g["F_final"]),
+               cpu_oracle=dict(value=c["flip_count"] / c["loop_s"], seconds=c["loop_s"], flip_count=c["flip_count"],
+                               accepts=c["accepts"], F_final=c["F_final"], cores=O.num_threads(),
+                               kind="port (C++ restatement of pngfopt.hpp, FP64, OpenMP, no BLAS library)"),
+               same_accept_sequence=bool(np.array_equal(g["acc_tiles"], c["acc_tiles"]) and
+                                         np.array_equal(g["acc_eval"], c["acc_eval"])),
+               F_final_rel_diff=abs(g["F_final"] - c["F_final"]) / c["F_final"])
+    log("[bench] 7x7 x 1: GPU %d evals %d accepts %.2f ms (%.0f flips/s); CPU oracle %.1f ms (%.0f flips/s, %d "
+        "threads); same sequence %s" % (g["flip_count"], g["accepts"], 1e3 * out["gpu"]["seconds"], out["gpu"]["value"],
+                                        1e3 * c["loop_s"], out["cpu_oracle"]["value"], O.num_threads(),
+                                        out["same_accept_sequence"]))
+    return out
 
 
 def copy_probe():
@@ -555,10 +616,12 @@ def main():
         popt.close()
         del popt
     cpu = None
+    small = None
     if not args.no_cpu:
         log("[bench] cpu baseline (%.0f s budget)" % args.cpu_seconds)
         cpu = cpu_baseline(wl, g0, cinv0, args.cpu_seconds, last)
         del g0, cinv0
+        small = config_small()
 
     out = {
         "metric": METRIC,
@@ -588,6 +651,7 @@ def main():
         "autotune": tune,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "config_7x7_1f": small,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "speedup_vs_plain_ceiling": (value / plain["hbm_ceiling_value"]) if plain else None,
