@@ -1731,8 +1731,11 @@ __device__ inline void gstore2(zf* base, size_t i, float x, float y) {
 __device__ inline zd ldgz(const zd* p, size_t i) { return ldg(p, i); }
 __device__ inline zf ldgz(const zf* p, size_t i) { return ldgf(p, i); }
 
-template <class Z, int G>
-__global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt,
+// SYMA (symmetric W): the M_new rows take the capacitance block's LDS once it is dead, so a
+// workgroup holds two G x (G + 1) arrays instead of three, and 8 workgroups of 4 waves fit a CU
+// (19.3 KB of LDS and 64 VGPRs at G = 24) instead of 5.
+template <class Z, int G, bool SYMA>
+__global__ void __launch_bounds__(256, 8) lr_finish_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt,
                                                         const Z* __restrict__ acc, int acc_stride,
                                                         const Z* __restrict__ Bt, size_t bt_stride,
                                                         const int32_t* __restrict__ s, int k, int K, int nchunk,
@@ -1750,7 +1753,10 @@ __global__ void __launch_bounds__(256) lr_finish_kernel(Dims d, DevTabsT<Z> tb, 
     }
     __shared__ Z Xs[G][G + 1];
     __shared__ Z As[G][G + 1];
-    __shared__ Z Ss[G][G + 1];  // row strip, then M_new[c][row]
+    __shared__ Z Ss2[SYMA ? 1 : G][G + 1];
+    // row strip (two-strip path), then M_new[c][row]; in the symmetric path the storage of Xs,
+    // which nothing reads after the first product
+    Z (*Ss)[G + 1] = SYMA ? Xs : Ss2;
     __shared__ int rows[G];
     const int f = blockIdx.y, tid = threadIdx.x, bx = blockIdx.x;
     bool do_rows, do_block;
@@ -1873,13 +1879,21 @@ void launch_lr_finish(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, cons
     if (d.gmax <= 24) {
         const int nchunk = (d.N + 23) / 24;
         const int nblk = left ? ngroups_part + (nleft + 23) / 24 : nchunk + d.ngroups;
-        hipLaunchKernelGGL((lr_finish_kernel<Z, 24>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
-                           acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        if (d.sym)
+            hipLaunchKernelGGL((lr_finish_kernel<Z, 24, true>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                               acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        else
+            hipLaunchKernelGGL((lr_finish_kernel<Z, 24, false>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                               acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     } else {
         const int nchunk = (d.N + 31) / 32;
         const int nblk = left ? ngroups_part + (nleft + 31) / 32 : nchunk + d.ngroups;
-        hipLaunchKernelGGL((lr_finish_kernel<Z, 32>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
-                           acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        if (d.sym)
+            hipLaunchKernelGGL((lr_finish_kernel<Z, 32, true>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                               acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
+        else
+            hipLaunchKernelGGL((lr_finish_kernel<Z, 32, false>), dim3(nblk, d.nf), dim3(256), 0, st, d, tb, v, gt, acc,
+                               acc_stride, Bt, bt_stride, s, k, K, nchunk, dyn, left, nleft, ngroups_part, fu);
     }
 }
 
