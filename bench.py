@@ -415,6 +415,62 @@ def counter_summary(path):
     print(json.dumps(fit))
 
 
+def trace_summary(path):
+    """Summary of a rocprofv3 --kernel-trace CSV of `bench.py --steps 1 --warmup 0 --no-cpu`:
+    the timed step (from its first loop_batch_kernel through the device loop's kernels, up to the
+    first copy of 50 us or more: the checkpoint restore that follows the step) and the PLAIN
+    phase (every dispatch after the last rocSOLVER dispatch, the PLAIN handle's init).  Per
+    kernel: dispatches, counted dispatches (column strips under 10 us are iterations without an
+    accept, which return at once), average / min / max us and total ms."""
+    import collections
+    import csv
+    rows = []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            n = r["Kernel_Name"].replace("void ", "").replace("pngf::", "").replace("(anonymous namespace)::", "")
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), n.split("(")[0][:60]))
+    rows.sort()
+    loop = ("lr_eval_kernel", "loop_control_kernel", "lr_prep_kernel", "lr_gather", "lr_skinny_mfma_kernel",
+            "lr_finish_kernel", "loop_batch_kernel", "__amd_rocclr_copyBuffer", "mask_update_kernel")
+
+    def table(sel, title):
+        st = collections.OrderedDict()
+        for s0, e0, n in sel:
+            d = (e0 - s0) / 1e3
+            v = st.setdefault(n, [0, 0, 0.0, 1e30, 0.0])
+            v[0] += 1
+            if n.startswith("lr_skinny") and d < 10.0:
+                continue
+            v[1] += 1
+            v[2] += d
+            v[3] = min(v[3], d)
+            v[4] = max(v[4], d)
+        print(title)
+        print("    %-60s %10s %8s %10s %9s %9s %10s" % ("kernel", "dispatches", "counted", "avg_us", "min_us",
+                                                       "max_us", "total_ms"))
+        for n, v in sorted(st.items(), key=lambda kv: -kv[1][2]):
+            if v[1]:
+                print("    %-60s %10d %8d %10.1f %9.1f %9.1f %10.2f" % (n, v[0], v[1], v[2] / v[1], v[3], v[4],
+                                                                       v[2] / 1e3))
+        print()
+
+    i0 = next(i for i, r in enumerate(rows) if r[2].startswith("loop_batch_kernel"))
+    i1 = i0 + 1
+    while (i1 < len(rows) and any(rows[i1][2].startswith(x) for x in loop) and
+           not (rows[i1][2].startswith("__amd_rocclr_copyBuffer") and rows[i1][1] - rows[i1][0] >= 50000)):
+        i1 += 1
+    win = rows[i0:i1]
+    print("timed step window: %d dispatches, %.1f ms (first scoring launch to last accept kernel)" % (
+        len(win), (win[-1][1] - win[0][0]) / 1e6))
+    table(win, "## Timed step")
+    solver = [i for i, r in enumerate(rows) if "getr" in r[2] or "rocsolver" in r[2].lower() or "getf2" in r[2]]
+    if solver:
+        plain = rows[solver[-1] + 1:]
+        print("PLAIN phase: %d dispatches after the PLAIN handle's init, %.1f ms of kernel time" % (
+            len(plain), sum(e0 - s0 for s0, e0, n in plain) / 1e6))
+        table(plain, "## PLAIN runs (two DBS runs: one for the rate, one event-timed)")
+
+
 def bench_multi(workers=4):
     """16 DBS instances of uneven size (11..25 tiles, 5 frequencies) on one GPU at once.
     Prints its own JSON line: total committed evaluations / wall seconds of the concurrent
@@ -475,6 +531,8 @@ def main():
                     help="only run the column strips of the 21x21 x 9 workload from the empty design (counter passes)")
     ap.add_argument("--counter-summary", metavar="DIR",
                     help="only summarise the counter passes of --counter-probe / --copy-probe found under DIR")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="only summarise a rocprofv3 kernel_trace.csv of bench.py --steps 1 --warmup 0 --no-cpu")
     ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
@@ -493,6 +551,9 @@ def main():
         return
     if args.counter_summary:
         counter_summary(args.counter_summary)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
         return
     if args.workload == "multi16":
         bench_multi(args.workers)
