@@ -645,7 +645,13 @@ def main():
     if not args.no_variants and method == P.METHOD_LOWRANK:
         variants = {"symmetry_off": variant(wl, "symmetry off", P.SYM_OFF, 0.0, last),
                     "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last),
+                    "nonreciprocal_1e-7_projected": variant(wl, "G perturbed 1e-7, projected", P.SYM_PROJECT,
+                                                            1e-7, last),
                     "precision_c64": variant(wl, "complex64", P.SYM_AUTO, 0.0, last, P.PREC_C64)}
+        # the projected run against the exact (two-strip) run on the same perturbed G
+        pr, ex = variants["nonreciprocal_1e-7_projected"], variants["nonreciprocal_1e-7"]
+        pr["F_final_vs_unprojected"] = pr["F_final"] / ex["F_final"]
+        pr["speedup_vs_unprojected"] = pr["value"] / ex["value"]
 
     plain = None
     g0 = cinv0 = None

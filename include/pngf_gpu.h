@@ -61,6 +61,11 @@ typedef struct { double re, im; } pngf_c128;
    and one of the two N x K x k products is skipped */
 #define PNGF_SYM_AUTO 0 /* use it when every G is symmetric to 1e-14 of its largest entry */
 #define PNGF_SYM_OFF  1 /* always form both strips */
+#define PNGF_SYM_PROJECT 2 /* opt-in, not the reference's results: a G whose max|G - G^T| is at most
+                              1e-5 of its largest entry (acmefdfd's single-precision MUMPS output,
+                              widened, is symmetric to ~1e-7) is replaced by (G + G^T) / 2 before
+                              the state is built, and the accept then uses W = W^T; a G further from
+                              symmetric keeps both strips.  The results are those of the projected G. */
 
 /* LOWRANK deferred factors W = W0 - A M: how the M side is held */
 #define PNGF_FACTORS_AUTO    0 /* stored M, unless device memory caps its capacity below the wanted
@@ -110,7 +115,7 @@ typedef struct {
     int64_t mem_limit;    /* refuse device allocations beyond this many bytes (0 = no limit) */
     double synth_soft;    /* synthetic G only: softening radius in cells (0 = 1.0); smaller values
                              give better-conditioned C */
-    int32_t symmetry;     /* LOWRANK: PNGF_SYM_AUTO or PNGF_SYM_OFF */
+    int32_t symmetry;     /* LOWRANK: PNGF_SYM_AUTO, PNGF_SYM_OFF or PNGF_SYM_PROJECT */
     int32_t factors;      /* LOWRANK: PNGF_FACTORS_* */
     int32_t precision;    /* PNGF_PREC_* (PNGF_PREC_C64 needs PNGF_METHOD_LOWRANK) */
 } pngf_config;
@@ -350,8 +355,9 @@ int pngf_profile(pngf_handle* h, int32_t enable);
 int pngf_autotune_result(pngf_handle* h, int32_t* chosen, int32_t* nvariants, double* ms);
 int pngf_kernel_stats(pngf_handle* h, int32_t which, pngf_kstat* out);
 /* After pngf_init_state: *used = 1 if the LOWRANK accept uses W = W^T (PNGF_SYM_AUTO and G
-   symmetric), *rel_asym = max over frequencies of max|G_ij - G_ji| / max|G_ij| (|z| =
-   |Re z| + |Im z|).  Either pointer may be null. */
+   symmetric, or PNGF_SYM_PROJECT and every G projected), *rel_asym = max over frequencies of
+   max|G_ij - G_ji| / max|G_ij| (|z| = |Re z| + |Im z|) of G as given, before any projection.
+   Either pointer may be null. */
 int pngf_symmetry(pngf_handle* h, int32_t* used, double* rel_asym);
 /* After pngf_init_state (LOWRANK): *columns = factor columns the handle holds before it folds them
    into W0, *rebuilt = 1 if M rows are rebuilt from A and Lam' (PNGF_FACTORS_REBUILT in effect).

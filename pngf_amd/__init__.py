@@ -19,7 +19,7 @@ OBJ_REF, OBJ_S11 = 0, 1
 METHOD_PLAIN, METHOD_LOWRANK = 0, 1
 DRIVER_REF, DRIVER_RASTER = 0, 1
 ORDER_FIRST, ORDER_BEST = 0, 1
-SYM_AUTO, SYM_OFF = 0, 1
+SYM_AUTO, SYM_OFF, SYM_PROJECT = 0, 1, 2
 FACTORS_AUTO, FACTORS_STORED, FACTORS_REBUILT = 0, 1, 2
 LOOP_AUTO, LOOP_HOST = 0, 1
 PREC_C128, PREC_C64 = 0, 1

@@ -131,6 +131,7 @@ struct pngf_handle {
     int K = 0, Kmax = 0, flushes = 0;
     int skv = 0;                     // tile-shape variant of the skinny products (autotuned)
     hipStream_t st2 = nullptr;       // side stream: the capacitance solve of an accept
+    bool sym_proj_ok = true;         // PNGF_SYM_PROJECT: every frequency's G was projected
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool sym = false;                // LOWRANK accept takes W[t,:] = W[:,t]^T (G symmetric)
     double asym = 0.0;               // max over frequencies of max|G_ij - G_ji| / max|G_ij|
@@ -393,6 +394,8 @@ const double kEpsR = 3.5;
 const double kSoftDefault = 1.0;  // softening radius in cells
 // largest relative asymmetry of G for which the LOWRANK accept treats W as symmetric
 const double kSymTol = 1e-14;
+// PNGF_SYM_PROJECT: the largest asymmetry (relative to G's largest entry) that is projected away
+const double kSymProjectTol = 1e-5;
 
 void synth_positions(const Layout& L, double dx, double dy, std::vector<double>& pos) {
     pos.resize((size_t)L.N * 3);
@@ -472,7 +475,7 @@ int check_finite(pngf_handle* h, int f) {
 // *asym_f = max|G_ij - G_ji| / max|G_ij| of slot f's (complex128) G; 1 unless the symmetric accept may apply
 int measure_asym(pngf_handle* h, int f, double* asym_f) {
     *asym_f = 1.0;
-    if (h->cfg.symmetry != PNGF_SYM_AUTO) return PNGF_OK;
+    if (h->cfg.symmetry == PNGF_SYM_OFF) return PNGF_OK;
     hipStream_t st = h->st;
     unsigned long long m[2] = {0, 0};
     HC(h, hipMemsetAsync(h->d_asym, 0, 2 * sizeof(unsigned long long), st));
@@ -487,6 +490,21 @@ int measure_asym(pngf_handle* h, int f, double* asym_f) {
     return PNGF_OK;
 }
 
+// PNGF_SYM_PROJECT: G measured as given (the reported asymmetry) and, within kSymProjectTol,
+// replaced in place by (G + G^T) / 2 before anything is built from it; *asym_f is then the
+// projected G's asymmetry (0).  A frequency left unprojected keeps both strips for the handle.
+int project_g(pngf_handle* h, int f, double* asym_f) {
+    CK(measure_asym(h, f, asym_f));
+    if (*asym_f <= kSymProjectTol) {
+        launch_symmetrize(h->st, h->d.N, h->d.ldN, h->Gcm[f]);
+        HC(h, hipGetLastError());
+        *asym_f = 0.0;
+    } else {
+        h->sym_proj_ok = false;
+    }
+    return PNGF_OK;
+}
+
 // complex64 state (the reference's USE_SINGLE_PRECISION): G cast to c64 as the reference casts the
 // file (pngfopt.hpp:128-156), then Gobj row 0 and Einc (pngfopt.hpp:584-595), C and its inverse by
 // cgetrf + cgetri (linalg.hpp:49-84), PEinc, R = Gobj C^-1, Eobj = R PEinc, W = (G - I) C^-1, q and the
@@ -498,7 +516,8 @@ int init_freq_c64(pngf_handle* h, int f) {
     const rbc one = {1.f, 0.f}, zero = {0.f, 0.f};
     CK(check_finite(h, f));
     double asym_f = 1.0;
-    CK(measure_asym(h, f, &asym_f));
+    if (h->cfg.symmetry == PNGF_SYM_PROJECT) CK(project_g(h, f, &asym_f));
+    else CK(measure_asym(h, f, &asym_f));
     zd* Gbuf = nullptr;
     CK(zalloc(h, &Gbuf, (size_t)ld * N));
     zf* Gf = (zf*)Gbuf;
@@ -583,6 +602,8 @@ int init_freq(pngf_handle* h, int f) {
     const rbz one = {1.0, 0.0}, zero = {0.0, 0.0};
     zd* G = h->Gcm[f];
     CK(check_finite(h, f));
+    double asym_f = 1.0;
+    if (h->cfg.symmetry == PNGF_SYM_PROJECT) CK(project_g(h, f, &asym_f));
     zd* C = nullptr;
     CK(dalloc_t(h, &C, (size_t)ld * N));
     launch_build_C(st, N, ld, h->d_P, G, C);
@@ -627,8 +648,7 @@ int init_freq(pngf_handle* h, int f) {
         h->Grow[f] = Gr;
         h->Cinv[f] = C;
     } else {
-        double asym_f = 1.0;
-        CK(measure_asym(h, f, &asym_f));
+        if (h->cfg.symmetry != PNGF_SYM_PROJECT) CK(measure_asym(h, f, &asym_f));
         // W = (G - I) C^-1, stored row-major: W^T = C^-T (G - I)^T
         launch_add_diag(st, N, ld, G, -1.0);
         zd* W = nullptr;
@@ -1141,7 +1161,8 @@ int pngf_create(const pngf_config* cfg, pngf_handle** out) {
     if (!(h->cfg.synth_soft > 0.0)) h->cfg.synth_soft = kSoftDefault;
     h->L.setup(c.layout, c.tiles_x, c.tiles_y, c.cells_x, c.cells_y);
     if (h->L.per_tile * c.max_flips > 96 || c.tile_variant < 0 || c.tile_variant > 6 ||
-        (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF) ||
+        (c.symmetry != PNGF_SYM_AUTO && c.symmetry != PNGF_SYM_OFF && c.symmetry != PNGF_SYM_PROJECT) ||
+        (c.symmetry == PNGF_SYM_PROJECT && c.method != PNGF_METHOD_LOWRANK) ||
         c.factors < PNGF_FACTORS_AUTO || c.factors > PNGF_FACTORS_REBUILT ||
         (c.precision != PNGF_PREC_C128 && c.precision != PNGF_PREC_C64) ||
         (c.precision == PNGF_PREC_C64 && c.method != PNGF_METHOD_LOWRANK)) {
@@ -1493,10 +1514,13 @@ int pngf_init_state(pngf_handle* h) {
     h->asym = 0.0;
     h->sym = false;
     h->d.sym = 0;
-    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && !h->d_asym)
+    if (h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry != PNGF_SYM_OFF && !h->d_asym)
         CK(dalloc_t(h, &h->d_asym, 2));
+    h->sym_proj_ok = true;
     for (int f = 0; f < nf; f++) CK(init_freq(h, f));
-    h->sym = h->cfg.method == PNGF_METHOD_LOWRANK && h->cfg.symmetry == PNGF_SYM_AUTO && h->asym <= kSymTol;
+    h->sym = h->cfg.method == PNGF_METHOD_LOWRANK &&
+             ((h->cfg.symmetry == PNGF_SYM_AUTO && h->asym <= kSymTol) ||
+              (h->cfg.symmetry == PNGF_SYM_PROJECT && h->sym_proj_ok));
     h->d.sym = h->sym ? 1 : 0;
     HC(h, hipMemcpy(h->vecs.fc, h->fc.data(), nf * sizeof(FreqConst), hipMemcpyHostToDevice));
     if (h->cfg.method == PNGF_METHOD_LOWRANK) {

@@ -73,7 +73,8 @@ def test_create_rejects_bad_config():
     with pytest.raises(P.PNGFError):
         P.Optimizer(P.LAYOUT_SHARED, 8, num_eobj=3, max_flips=4, driver=P.DRIVER_REF, objective=P.OBJ_REF)
     # enumerations outside their range are refused before any device is touched
-    for kw in (dict(symmetry=2), dict(factors=3), dict(factors=-1), dict(tile_variant=7)):
+    for kw in (dict(symmetry=3), dict(symmetry=P.SYM_PROJECT, method=P.METHOD_PLAIN), dict(factors=3),
+               dict(factors=-1), dict(tile_variant=7)):
         with pytest.raises(P.PNGFError) as e:
             P.Optimizer(P.LAYOUT_DISJOINT, 7, **kw)
         assert e.value.code == -1, kw

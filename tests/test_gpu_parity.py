@@ -408,6 +408,71 @@ def test_asymmetric_g_forms_both_strips():
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
+def _perturbed_gs(T, freqs, rel_noise):
+    N = P.layout_size(P.LAYOUT_DISJOINT, T)
+    return [P.perturb_block(P.synth_g_host(P.LAYOUT_DISJOINT, T, 1, hz, soft=SOFT_WELL), N, rel_noise, seed=f + 1)
+            for f, hz in enumerate(freqs)]
+
+
+def _oracles(Gs, T, tm):
+    ors = []
+    for G in Gs:
+        o = O.Optimizer(P.LAYOUT_DISJOINT, T, G)
+        assert o.init_state(tm, ((T - 1) // 2, (T - 1) // 2, 1, 2)) == 0
+        ors.append(o)
+    return ors
+
+
+def test_projection_runs_the_projected_g():
+    """PNGF_SYM_PROJECT on a G perturbed non-reciprocally at 1e-7 of its largest entry (the level
+    of acmefdfd's single-precision Schur complement): the reported asymmetry is that of G as
+    given, the accept uses W = W^T, and init state, scores and the DBS run are the oracle's on
+    (G + G^T) / 2 (formed on the host with the same sums, so the same matrix bitwise).  Against
+    the oracle on the unprojected G the objective of the run moves by the projection only:
+    printed, and bounded by 1e-5."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    Gs = _perturbed_gs(T, freqs, 1e-7)
+    gpu, ors_given, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, Gs=Gs,
+                                    symmetry=P.SYM_PROJECT)
+    used, asym = gpu.symmetry()
+    assert used and 1e-8 < asym < 5e-7
+    ors = _oracles([np.asfortranarray(0.5 * (G + G.T)) for G in Gs], T, tm)
+    E, _ = oracle_state(ors)
+    assert rel(gpu.state()["Eobj"], E) < RTOL
+    cands = candidates(P.LAYOUT_DISJOINT, T)
+    F, e = gpu.eval_batch([[t] for t in cands])
+    for i, t in enumerate(cands):
+        assert rel(e[i], np.array([o.compute_updated_Eobj([t]) for o in ors])) < RTOL
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    ref = O.dbs_run(ors_given, O.OBJ_S11, freqs, DX, DY, DZ)
+    dF = abs(g["F_final"] - ref["F_final"]) / ref["F_final"]
+    print("[projection] asym %.2e: %d accepts (unprojected %d, same sequence %s), F %.10g vs %.10g (rel %.2e)" % (
+        asym, g["accepts"], ref["accepts"], np.array_equal(g["acc_tiles"], ref["acc_tiles"]), g["F_final"],
+        ref["F_final"], dF))
+    assert dF < 1e-5
+    gpu.close()
+
+
+def test_projection_declined_beyond_its_tolerance():
+    """A G non-reciprocal at 1e-4 of its largest entry is beyond PNGF_SYM_PROJECT's 1e-5: it is
+    not projected, the accept forms both strips, and the run is the oracle's on G as given."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    Gs = _perturbed_gs(T, freqs, 1e-4)
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK, Gs=Gs, symmetry=P.SYM_PROJECT)
+    used, asym = gpu.symmetry()
+    assert not used and 1e-5 < asym < 1e-3
+    g, o = run_both(gpu, ors, freqs, O.OBJ_S11)
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+    gpu.close()
+
+
 def test_device_perturbation_matches_host_and_oracle():
     """pngf_perturb_g at 1e-7 of the largest entry (the rounding level of acmefdfd's single-
     precision Schur complement): the device's G equals perturb_block's on the host bitwise, the
