@@ -29,11 +29,13 @@ public:
     //            num_eobj, max_flips, dx, dy, dz) -- here for all frequencies at once.
     Optimizer(int layout, int tiles_x, int tiles_y, int cells_x, int cells_y, int num_eobj, int max_flips,
               int num_freq, double dx, double dy, double dz, int method = PNGF_METHOD_LOWRANK,
-              int objective = PNGF_OBJ_REF, int driver = PNGF_DRIVER_REF, int device = 0) {
+              int objective = PNGF_OBJ_REF, int driver = PNGF_DRIVER_REF, int device = 0,
+              int symmetry = PNGF_SYM_AUTO) {
         pngf_config c = {};
         c.layout = layout; c.tiles_x = tiles_x; c.tiles_y = tiles_y; c.cells_x = cells_x; c.cells_y = cells_y;
         c.num_eobj = num_eobj; c.max_flips = max_flips; c.num_freq = num_freq; c.method = method;
         c.objective = objective; c.driver = driver; c.device = device; c.dx = dx; c.dy = dy; c.dz = dz;
+        c.symmetry = symmetry;
         c.z0 = 50.0;
         nf_ = num_freq; ne_ = num_eobj; tiles_ = tiles_x * tiles_y; max_flips_ = max_flips;
         int rc = pngf_create(&c, &h_);

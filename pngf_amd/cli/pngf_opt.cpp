@@ -6,6 +6,8 @@
 //     --freqs f1,f2,...  frequencies in GHz (25,27.5,30,32.5,35 as main.cpp:116-117)
 //     --gdir DIR         read DIR/Gmat_sub_XX.bin (main.cpp:235); default: synthetic G
 //     --driver ref|raster, --objective ref|s11, --method lowrank|plain
+//     --symmetry auto|off|project   LOWRANK accept (project: a G symmetric to 1e-5, such as
+//                        acmefdfd's widened single-precision output, replaced by (G + G^T) / 2)
 //     --seed S           MagicHat seed (-514213645, main.cpp:154)
 //     --init empty|random (main.cpp:155, 187-219)
 //     --log INDEX        write the optimization history to log_INDEX.txt in the reference
@@ -53,7 +55,7 @@ static void save_history(const std::string& fname, int seed, const std::vector<i
 
 int main(int argc, char** argv) {
     int T = 21, layout = PNGF_LAYOUT_SHARED, driver = PNGF_DRIVER_REF, objective = PNGF_OBJ_REF;
-    int method = PNGF_METHOD_LOWRANK, seed = -514213645;
+    int method = PNGF_METHOD_LOWRANK, seed = -514213645, symmetry = PNGF_SYM_AUTO;
     bool random_init = false;
     int log_index = -1;
     std::string gdir;
@@ -66,6 +68,10 @@ int main(int argc, char** argv) {
         else if (a == "--driver") driver = next() == "raster" ? PNGF_DRIVER_RASTER : PNGF_DRIVER_REF;
         else if (a == "--objective") objective = next() == "s11" ? PNGF_OBJ_S11 : PNGF_OBJ_REF;
         else if (a == "--method") method = next() == "plain" ? PNGF_METHOD_PLAIN : PNGF_METHOD_LOWRANK;
+        else if (a == "--symmetry") {
+            const std::string v = next();
+            symmetry = v == "off" ? PNGF_SYM_OFF : v == "project" ? PNGF_SYM_PROJECT : PNGF_SYM_AUTO;
+        }
         else if (a == "--seed") seed = std::atoi(next().c_str());
         else if (a == "--init") random_init = next() == "random";
         else if (a == "--gdir") gdir = next();
@@ -88,7 +94,8 @@ int main(int argc, char** argv) {
     const int max_flips = driver == PNGF_DRIVER_REF ? 4 : 1;
     const double lambda = 0.01, dx = lambda / 60.0, dy = lambda / 60.0, dz = lambda / 180.0;  // main.cpp:157-160
     try {
-        pngf::Optimizer opt(layout, T, T, 3, 3, ne, max_flips, nf, dx, dy, dz, method, objective, driver);
+        pngf::Optimizer opt(layout, T, T, 3, 3, ne, max_flips, nf, dx, dy, dz, method, objective, driver, 0,
+                            symmetry);
         // initial design (main.cpp:187-219): mirrored random quadrant or the dipole
         std::vector<int> tm(T * T, 0);
         if (random_init) {
