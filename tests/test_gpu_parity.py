@@ -171,6 +171,64 @@ def test_dbs_reference_driver(method, symmetry):
     assert rel(g["acc_F"], o["acc_F"]) < RTOL
 
 
+@pytest.mark.timeout(900)
+def test_reference_default_configuration():
+    """The reference program's own configuration at its own size (main.cpp:116-117, 154-160,
+    181-241): 21x21 tiles on the shared Yee grid (N = 8064), 5 frequencies 25..35 GHz, ne = 3
+    (port row and two NF2FF rows), the directivity objective, MagicHat order with mirror-
+    symmetric 2/4-tile flips (k up to 96), seed -514213645, port at tile (10, 10); softening 0.4.
+    The LOWRANK run's first 60 evaluations against the oracle's: same accepted flips at the same
+    evaluations, objective to RTOL.  The oracle starts from the C^-1 that a PLAIN handle formed
+    on the device (the oracle's own inversion at this N takes minutes on the host) and then
+    runs the reference's Woodbury updates from it."""
+    T, ne, mid, nev = 21, 3, 10, 60
+    freqs = [25e9, 27.5e9, 30e9, 32.5e9, 35e9]
+    tm = initial_pattern(T, 1)
+    Gs = [P.synth_g_host(P.LAYOUT_SHARED, T, ne, hz, soft=SOFT_WELL) for hz in freqs]
+
+    def handle(method):
+        h = P.Optimizer(P.LAYOUT_SHARED, T, num_freq=len(freqs), num_eobj=ne, max_flips=4, method=method,
+                        objective=P.OBJ_REF, driver=P.DRIVER_REF)
+        for f, (hz, G) in enumerate(zip(freqs, Gs)):
+            h.set_frequency(f, hz)
+            h.set_g(f, G)
+        h.set_tile_map(tm)
+        h.set_source_port(mid, mid, 1, 2)
+        h.init_state()
+        return h
+
+    pl = handle(P.METHOD_PLAIN)
+    try:
+        assert pl.N == 8064
+        cinvs = [pl.cinv(f) for f in range(len(freqs))]
+    finally:
+        pl.close()
+    ors = []
+    for G, ci in zip(Gs, cinvs):
+        o = O.Optimizer(P.LAYOUT_SHARED, T, G, ne=ne, max_flips=4)
+        o.set_tile_map(tm)
+        o.set_source_port(mid, mid, 1, 2)
+        o.set_cinv(ci)
+        o.compute_PEinc()
+        o.compute_R_matrix(1)
+        ors.append(o)
+    del cinvs
+    gpu = handle(P.METHOD_LOWRANK)
+    try:
+        E, V = oracle_state(ors)
+        assert rel(gpu.state()["Eobj"], E) < RTOL and rel(gpu.state()["Vinc"], V) < RTOL
+        g, o = run_both(gpu, ors, freqs, O.OBJ_REF, seed=-514213645, max_evals=nev)
+    finally:
+        gpu.close()
+    print("[reference config] %d evaluations, %d accepts, F %.10g (oracle %.10g)" % (
+        g["flip_count"], g["accepts"], g["F_final"], o["F_final"]))
+    assert g["flip_count"] == o["flip_count"] == nev
+    assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+    assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+    assert np.array_equal(g["acc_eval"], o["acc_eval"])
+    assert rel(g["acc_F"], o["acc_F"]) < RTOL
+
+
 @pytest.mark.parametrize("case", ["raster_s11_lowrank", "raster_s11_plain", "ref_lowrank"])
 def test_history_matches_oracle(case):
     """The reference Logger's records (start state, then one per evaluation: tile, decision,
