@@ -14,7 +14,8 @@ required fields: roofline (the column-strip kernel, timed by HIP events recorded
 stream around every launch of the timed steps; HBM traffic, MFMA busy and clock from rocprofv3
 counter passes over the same launches), cpu_baseline (the oracle on a bounded sample,
 projected to the run's counts), config_7x7_1f (the reference's CPU-runnable 7x7 case: the GPU
-beside the oracle),
+beside the oracle), reference_config (the reference program's own default configuration:
+LOWRANK and PLAIN on the GPU),
 plain_fp64_gpu (the reference formulation on the GPU, same start), speedup_vs_plain,
 best_improvement_variant, time_to_solution_s, per-kernel-category statistics, autotuner
 choice and the symmetric-accept decision.
@@ -206,6 +207,44 @@ def cpu_baseline(wl, g0, cinv0, budget_s, gpu_run):
                        "run's %d evaluations and %d accepts at those times x %d frequencies (%.0f s)"
                        % (O.num_threads(), nf, r["flip_count"], r["accepts"], r["loop_s"], 1e3 * t_eval,
                           1e3 * t_acc, gpu_run["flip_count"], gpu_run["accepts"], nf, run_s))
+
+
+def reference_config(max_evals=3000):
+    """The reference program's own default (main.cpp:116-117, 154-160, 181-241): 21x21 tiles on the
+    shared Yee grid (N = 8064), 5 frequencies 25..35 GHz, ne = 3, the directivity objective,
+    MagicHat order with mirror-symmetric 2/4-tile flips, seed -514213645, port at tile (10, 10),
+    synthetic G at the default softening, start initial_pattern(21, 0).  LOWRANK and PLAIN on the
+    GPU for the first `max_evals` evaluations (one warmup run each, restored from a checkpoint):
+    evaluated flips/s, accepts, and whether the two take the same flips."""
+    T, ne, mid = 21, 3, 10
+    freqs = [25e9, 27.5e9, 30e9, 32.5e9, 35e9]
+    out = dict(workload="reference default: 21x21 shared grid, N = 8064, 5 frequencies, ne = 3, directivity "
+                        "objective, MagicHat 2/4-tile flips", max_evals=max_evals)
+    runs = {}
+    for name, method in (("lowrank", P.METHOD_LOWRANK), ("plain", P.METHOD_PLAIN)):
+        opt = P.Optimizer(P.LAYOUT_SHARED, T, num_freq=len(freqs), num_eobj=ne, max_flips=4, method=method,
+                          objective=P.OBJ_REF, driver=P.DRIVER_REF)
+        try:
+            for f, hz in enumerate(freqs):
+                opt.set_frequency(f, hz)
+                opt.synth_g(f)
+            opt.set_tile_map(initial_pattern(T, 0))
+            opt.set_source_port(mid, mid, 1, 2)
+            opt.init_state()
+            opt.checkpoint()
+            opt.dbs_run(max_evals=max_evals, seed=-514213645)
+            opt.restore()
+            r = opt.dbs_run(max_evals=max_evals, seed=-514213645)
+        finally:
+            opt.close()
+        runs[name] = r
+        out[name] = dict(value=r["flip_count"] / r["seconds"], seconds=r["seconds"], flip_count=r["flip_count"],
+                         accepts=r["accepts"], F_final=r["F_final"])
+        log("[bench] reference config, %s: %d evals %d accepts %.3f s (%.0f flips/s) F %.6g" % (
+            name, r["flip_count"], r["accepts"], r["seconds"], out[name]["value"], r["F_final"]))
+    out["speedup_vs_plain"] = out["lowrank"]["value"] / out["plain"]["value"]
+    out["same_accept_sequence"] = bool(np.array_equal(runs["lowrank"]["acc_tiles"], runs["plain"]["acc_tiles"]))
+    return out
 
 
 def config_small(steps=5):
@@ -653,6 +692,10 @@ def main():
         pr["F_final_vs_unprojected"] = pr["F_final"] / ex["F_final"]
         pr["speedup_vs_unprojected"] = pr["value"] / ex["value"]
 
+    refcfg = None
+    if not args.no_variants and method == P.METHOD_LOWRANK:
+        refcfg = reference_config()
+
     plain = None
     g0 = cinv0 = None
     if not args.no_plain or not args.no_cpu:
@@ -719,6 +762,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "config_7x7_1f": small,
+        "reference_config": refcfg,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "speedup_vs_plain_ceiling": (value / plain["hbm_ceiling_value"]) if plain else None,
