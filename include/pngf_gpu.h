@@ -128,18 +128,21 @@ const char* pngf_last_error(const pngf_handle* h);
 /* Number of design components N for a layout (pngfopt.hpp:86-94 for SHARED). */
 int pngf_layout_size(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x, int32_t cells_y);
 
-/* Frequency of slot f in Hz (used by the directivity term and the synthetic G). */
+/* Frequency of slot f in Hz: k0 = 2 pi f / c0 of the directivity term (main.cpp:162-164, one
+   PNGF_Optimizer per frequency there) and the synthetic G's frequency. */
 int pngf_set_frequency(pngf_handle* h, int32_t f, double hz);
 
 /* Reads acmefdfd's raw G file for frequency slot f: M x M complex128, column-major,
    M = N + num_eobj - 1, no header.  G = file[0:N,0:N], Gobj rows 1.. = file[N.., 0:N].
    Replaces PNGF_Optimizer::init's fread (pngfopt.hpp:121-184); adds a size check. */
 int pngf_load_g(pngf_handle* h, int32_t f, const char* path);
-/* Same contents from a host buffer in the file layout. */
+/* Same contents from a host buffer in the file layout.  No counterpart in the reference, which
+   reads files only. */
 int pngf_set_g(pngf_handle* h, int32_t f, const pngf_c128* file_buf);
 /* Synthetic stand-in for acmefdfd's G (dyadic Green's function of an eps_r = 3.5
    medium sampled at the component positions, softened self term), generated on the
-   device for slot f at its frequency.  See pngf_synth_g_host for the definition. */
+   device for slot f at its frequency.  See pngf_synth_g_host for the definition.  No
+   counterpart in the reference, where acmefdfd (FDFD + MUMPS, host) produces G. */
 int pngf_synth_g(pngf_handle* h, int32_t f);
 /* Adds a deterministic non-reciprocal perturbation to the G of slot f (before pngf_init_state):
    G(i, j) += rel * max|G| * (u1 + i u2), with u1, u2 uniform in [-1, 1) drawn from a hash of
@@ -149,7 +152,8 @@ int pngf_synth_g(pngf_handle* h, int32_t f);
    counterpart in the reference. */
 int pngf_perturb_g(pngf_handle* h, int32_t f, double rel, int32_t seed);
 /* Host version of the synthetic generator, written in the acmefdfd file layout
-   (M x M column-major).  The device version evaluates the same formula. */
+   (M x M column-major).  The device version evaluates the same formula.  No counterpart in
+   the reference. */
 int pngf_synth_g_host(int32_t layout, int32_t tiles_x, int32_t tiles_y, int32_t cells_x,
                       int32_t cells_y, int32_t num_eobj, double hz, double soft_cells,
                       pngf_c128* file_buf);
@@ -169,7 +173,9 @@ int pngf_init_state(pngf_handle* h);
    objective of main.cpp:401-423.  new_eobj: [num_freq][num_eobj] (may be null); F may be null. */
 int pngf_eval(pngf_handle* h, const int32_t* tiles, int32_t ntiles, pngf_c128* new_eobj, double* F);
 /* Scores ncand flips against the same state in one batch.  tiles: [ncand][max_flips],
-   ntiles: [ncand].  F_out: [ncand]; eobj_out: [ncand][num_freq][num_eobj] (may be null). */
+   ntiles: [ncand].  F_out: [ncand]; eobj_out: [ncand][num_freq][num_eobj] (may be null).
+   The batched form of compute_updated_Eobj (pngfopt.hpp:379-539) and main.cpp:401-423, which
+   the reference calls one flip and one frequency at a time. */
 int pngf_eval_batch(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int32_t ncand,
                     double* F_out, pngf_c128* eobj_out);
 /* First improvement over a visit sequence against the current state: the sequential loop of
@@ -186,16 +192,21 @@ int pngf_sweep(pngf_handle* h, const int32_t* tiles, const int32_t* ntiles, int3
 int pngf_accept(pngf_handle* h, const int32_t* tiles, int32_t ntiles);
 
 /* Current state.  Any pointer may be null.  eobj: [num_freq][num_eobj]; vinc: [num_freq];
-   ln: [num_freq][2] (LN1inc, LN2inc); s11: [num_freq]; F: objective. */
+   ln: [num_freq][2] (LN1inc, LN2inc); s11: [num_freq]; F: objective.  Replaces main.cpp's
+   reads of the public members Eobj, Vinc, LN1inc, LN2inc (main.cpp:271-305, pngfopt.hpp:21-50). */
 int pngf_get_state(pngf_handle* h, pngf_c128* eobj, pngf_c128* vinc, pngf_c128* ln,
                    pngf_c128* s11, double* F);
+/* reference: the public member tile_map (pngfopt.hpp:21-50), T x T, x fastest. */
 int pngf_get_tile_map(pngf_handle* h, int32_t* tile_map);
-/* C^-1 of slot f, column-major N x N (PLAIN only: LOWRANK does not keep it). */
+/* C^-1 of slot f, column-major N x N (PLAIN only: LOWRANK does not keep it).  The reference's
+   public member C, which holds C^-1 after main.cpp:246. */
 int pngf_get_cinv(pngf_handle* h, int32_t f, pngf_c128* out);
-/* Current W = (G-I) C^-1 of slot f, row-major N x N (LOWRANK only; folds pending factors). */
+/* Current W = (G-I) C^-1 of slot f, row-major N x N (LOWRANK only; folds pending factors).  No
+   counterpart in the reference, which keeps C^-1 instead. */
 int pngf_get_w(pngf_handle* h, int32_t f, pngf_c128* out);
 
-/* G of slot f, column-major N x N (PLAIN only: LOWRANK releases G after init). */
+/* G of slot f, column-major N x N (PLAIN only: LOWRANK releases G after init).  The reference's
+   member G (pngfopt.hpp:21-50). */
 int pngf_get_g(pngf_handle* h, int32_t f, pngf_c128* out);
 
 /* Device-side snapshot of the mutable optimizer state (C^-1 or W, S or q, R, Eobj,
@@ -285,6 +296,7 @@ int pngf_dbs_run_pool(pngf_handle** hs, int32_t n, const pngf_dbs_params* ps, pn
                       int32_t workers);
 
 /* ---- learned flip ranker (BF16 MLP, hand-written HIP forward / backward / Adam) ----
+   (this block, pngf_mlp_* through pngf_dbs_run_ranked, has no counterpart in the reference)
    A din -> dh -> dh -> dout network with GELU, FP32 master weights, BF16 MFMA products
    with FP32 accumulation.  No counterpart in the reference: it ranks candidate flips so
    that exact scoring runs on the top-K only. */
@@ -335,7 +347,8 @@ typedef struct {
 int pngf_dbs_run_ranked(pngf_handle* h, pngf_mlp* m, const pngf_rank_params* rp, const pngf_dbs_params* p,
                         pngf_dbs_result* r);
 
-/* Per-kernel timing collected with HIP events on the handle's stream (enable first). */
+/* Per-kernel timing collected with HIP events on the handle's stream (enable first).  No
+   counterpart in the reference (pngf_profile, pngf_kernel_stats, pngf_autotune_result). */
 #define PNGF_K_EVAL       0 /* candidate scoring (both methods) */
 #define PNGF_K_COLBLK     1 /* LOWRANK accept: W[:,t] from W0 and the factors */
 #define PNGF_K_ROWBLK     2 /* LOWRANK accept: W[t,:] from W0 and the factors */
