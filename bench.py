@@ -579,7 +579,7 @@ def main():
                     help="skip the two-strip and non-reciprocal-G runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)",
+        print("bench.py measures one GPU only: the library has no multi-GPU path (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:

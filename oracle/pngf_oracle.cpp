@@ -1,7 +1,7 @@
 // CPU oracle for the pngf-opt direct-binary-search (DBS) Woodbury path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
-// compared against, and the CPU baseline that bench.py times beside it.  It is
+// TEST INFRASTRUCTURE ONLY.  This file is what the tests compare the HIP library
+// against, and the CPU baseline that bench.py times beside it.  It is
 // never linked into, loaded by, or called from the product library.
 //
 // It restates the reference optimizer (pngf-opt/pngfopt.hpp, pngf-opt/linalg.hpp,

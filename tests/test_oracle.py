@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/pngf_oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/pngf_oracle.cpp) before the GPU tests trust it.
 
 Each check is one a plausible mistake in the restatement would fail: dense
 brute-force re-solves, numpy/scipy LAPACK, closed-form layout counts, the C++
