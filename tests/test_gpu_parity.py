@@ -276,6 +276,22 @@ def test_history_file_from_cli(tmp_path):
     assert [int(r[0]) for r in rec] == list(range(n))
 
 
+def test_cli_symmetry_project_is_a_no_op_on_a_symmetric_g(tmp_path):
+    """The synthetic G is exactly symmetric, so projecting it ((a + a) / 2 = a) changes nothing:
+    pngf_opt --symmetry project writes the same history as --symmetry auto, byte for byte, on
+    the default (ill-conditioned) synthetic G."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(P.__file__), "cli", "pngf_opt")
+    logs = []
+    for i, mode in enumerate(("auto", "project")):
+        cmd = [exe, "--tiles", "7", "--layout", "disjoint", "--driver", "raster", "--objective", "s11",
+               "--freqs", "29,31", "--symmetry", mode, "--log", str(i)]
+        out = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        logs.append((tmp_path / ("log_%d.txt" % i)).read_bytes())
+    assert len(logs[0]) > 100 and logs[0] == logs[1]
+
+
 @pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
 def test_dbs_ill_conditioned_identical_sequence(method):
     """Default synthetic G (cond(C) ~ 1e6): same accepted flips, objective to RTOL_ILL."""

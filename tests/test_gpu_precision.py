@@ -102,6 +102,44 @@ def test_c64_dbs_matches_c64_oracle(case):
     gpu.close()
 
 
+def test_c64_with_symmetric_projection():
+    """The complex64 mode with PNGF_SYM_PROJECT on a G perturbed non-reciprocally at 1e-7: the
+    projection is applied to the complex128 G before the cast, the accept uses W = W^T, and the
+    scores and the DBS run follow the complex64 oracle on (G + G^T) / 2 (formed on the host with
+    the same sums)."""
+    T = 7
+    freqs = freq_list(2, 29e9, 31e9)
+    N = P.layout_size(P.LAYOUT_DISJOINT, T)
+    Gs = [P.perturb_block(P.synth_g_host(P.LAYOUT_DISJOINT, T, 1, hz, soft=SOFT_WELL), N, 1e-7, seed=f + 1)
+          for f, hz in enumerate(freqs)]
+    tm = initial_pattern(T, 0)
+    gpu = P.Optimizer(P.LAYOUT_DISJOINT, T, num_freq=len(freqs), symmetry=P.SYM_PROJECT, precision=P.PREC_C64)
+    ors = []
+    for f, (hz, G) in enumerate(zip(freqs, Gs)):
+        gpu.set_frequency(f, hz)
+        gpu.set_g(f, G)
+        o = O.Optimizer(P.LAYOUT_DISJOINT, T, np.asfortranarray(0.5 * (G + G.T)), c64=True)
+        assert o.init_state(tm, (3, 3, 1, 2)) == 0
+        ors.append(o)
+    gpu.set_tile_map(tm)
+    gpu.set_source_port(3, 3, 1, 2)
+    gpu.init_state()
+    try:
+        used, asym = gpu.symmetry()
+        assert used and 1e-8 < asym < 5e-7
+        cands = [t for t in range(T * T) if t != 24]
+        F, e = gpu.eval_batch([[t] for t in cands])
+        worst = max(rel(e[i], np.array([o.compute_updated_Eobj([t]) for o in ors])) for i, t in enumerate(cands))
+        assert worst < RTOL32
+        g = gpu.dbs_run()
+        o = O.dbs_run(ors, O.OBJ_S11, freqs, DX, DY, DZ)
+        assert g["accepts"] == o["accepts"] and g["accepts"] > 0
+        assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
+        assert rel(g["acc_F"], o["acc_F"]) < RTOL32
+    finally:
+        gpu.close()
+
+
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("wl,soft,evals", [("antenna21x21_9f", SOFT_WELL, 1500), ("antenna31x31_7f", 0.0, 0)])
 def test_c64_error_against_complex128(wl, soft, evals):
