@@ -323,13 +323,22 @@ class Optimizer:
         self._ck(lib().pngf_eval(self.h, _p(t), len(t), _p(e), ct.byref(F)), "eval")
         return e, F.value
 
-    def eval_batch(self, cands):
+    def _pack(self, cands):
+        """[n][max_flips] tile array and per-candidate counts (a one-row buffer at n = 0, so the
+        library gets valid pointers and scores nothing); a flip of more tiles than max_flips is
+        refused as the library refuses it."""
         n = len(cands)
-        t = np.full((n, self.max_flips), -1, dtype=np.int32)
-        nt = np.zeros(n, dtype=np.int32)
+        t = np.full((max(n, 1), self.max_flips), -1, dtype=np.int32)
+        nt = np.zeros(max(n, 1), dtype=np.int32)
         for i, c in enumerate(cands):
+            if not 1 <= len(c) <= self.max_flips:
+                raise PNGFError(-1, "candidate %d has %d tiles (1..%d)" % (i, len(c), self.max_flips))
             t[i, :len(c)] = c
             nt[i] = len(c)
+        return n, t, nt
+
+    def eval_batch(self, cands):
+        n, t, nt = self._pack(cands)
         F = np.zeros(n)
         e = np.zeros((n, self.nf, self.ne), dtype=np.complex128)
         self._ck(lib().pngf_eval_batch(self.h, _p(t), _p(nt), n, _p(F), _p(e)), "eval_batch")
@@ -337,12 +346,7 @@ class Optimizer:
 
     def sweep(self, cands, F_best):
         """First improving flip of a visit sequence (not committed): (index or -1, F, flips scored)."""
-        n = len(cands)
-        t = np.full((max(n, 1), self.max_flips), -1, dtype=np.int32)
-        nt = np.zeros(max(n, 1), dtype=np.int32)
-        for i, c in enumerate(cands):
-            t[i, :len(c)] = c
-            nt[i] = len(c)
+        n, t, nt = self._pack(cands)
         idx, F, ev = ct.c_int32(), ct.c_double(), ct.c_int64()
         self._ck(lib().pngf_sweep(self.h, _p(t), _p(nt), n, float(F_best), ct.byref(idx), ct.byref(F), ct.byref(ev)),
                  "sweep")

@@ -214,7 +214,14 @@ def test_argument_and_order_errors():
         with pytest.raises(P.PNGFError) as e:
             opt.accept(bad)
         assert e.value.code == -1
+        with pytest.raises(P.PNGFError) as e:
+            opt.eval_batch([[5], bad])
+        assert e.value.code == -1
     assert np.array_equal(opt.tile_map(), finish)
+    # empty inputs: nothing scored, nothing accepted
+    F, e = opt.eval_batch([])
+    assert F.shape == (0,) and e.shape == (0, 1, 1)
+    assert opt.sweep([], 1.0) == (-1, 1.0, 0)
     with pytest.raises(P.PNGFError) as e:
         opt.dbs_run(order=P.ORDER_BEST, resume_from=np.zeros(P.DRIVER_STATE_WORDS, np.uint32))
     assert e.value.code == -1
