@@ -177,6 +177,13 @@ void launch_build_V(hipStream_t st, const Dims& d, const DevTabs& tb, const int3
                     size_t v_stride);
 void launch_snapshot_cinvu(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t,
                            const int32_t* s, int k, zd* CU, size_t cu_stride);
+// VC (k x N, ld k) = V C^-1 on FP64 MFMA, V (k x N, ld k); every frequency (grid.y).  `variant`
+// picks the tile shape (the column strip's shapes; k <= 24: 0 .. 5).
+void launch_plain_vcinv(hipStream_t st, const Dims& d, const DevTabs& tb, int k, const zd* V, size_t v_stride,
+                        zd* VC, size_t vc_stride, int variant = 1);
+// C^-1 -= CU M on FP64 MFMA, CU (N x k, ld ldN), M (k x N, row c at c * ldN); every frequency
+void launch_plain_rankk(hipStream_t st, const Dims& d, const DevTabs& tb, const zd* CU, size_t cu_stride,
+                        const zd* M, size_t m_stride, int k);
 void launch_plain_small(hipStream_t st, const Dims& d, const DevVecs& v, const zd* acc, int acc_stride,
                         const zd* CU, size_t cu_stride, const zd* M, size_t m_stride, const int32_t* t,
                         const int32_t* s, int k, zd* rut);

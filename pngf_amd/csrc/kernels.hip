@@ -1162,16 +1162,17 @@ __device__ inline zd ldg(const zd* p, size_t i) {
     return r;
 }
 
-template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
-__global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
-                                                                    int k, int K, const zd* __restrict__ small,
-                                                                    int small_stride, zd* Bt, size_t bt_stride,
-                                                                    const int32_t* dyn) {
-    if (dyn) {
-        if (!dyn[kDynActive]) return;
-        k = dyn[kDynK];
-        K = dyn[kDynKK];
-    }
+// MODE 0: colblk, 1: rowblk, 2: the PLAIN accept's V C^-1 (pngfopt.hpp:293-322), a colblk-shaped
+// product with A = C^-1 read column by column (A[n][i] = C^-1[i][n], n-th column contiguous),
+// inner index i over all N rows, B = V^T (V k x N, ld k: the staged-B layout of colblk) and the
+// result written as VC (k x N, ld k) without W0.
+enum { kSkinnyCol = 0, kSkinnyRow = 1, kSkinnyVC = 2 };
+
+template <int CT, int MODE, int KC, int DEPTH, int WAVES, bool DB, int RT>
+__device__ __forceinline__ void skinny_mfma(Dims d, DevTabs tb, const int32_t* __restrict__ t, int k,
+                                            int K, const zd* __restrict__ small, int small_stride, zd* Bt,
+                                            size_t bt_stride) {
+    constexpr bool ROW = MODE == kSkinnyRow;
     constexpr int BW = CT * 8;   // complex columns held (>= k)
     constexpr int NJ = CT * 16;  // real columns
     constexpr int KS = 8, NS = KC / KS, NT = WAVES * 64;
@@ -1187,7 +1188,8 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
     int nc[RT];  // clamped rows for loads
 #pragma unroll
     for (int r = 0; r < RT; r++) nc[r] = min(n0 + 16 * r + lr, d.N - 1);
-    const zd* A = tb.Aall[f];
+    const zd* A = MODE == kSkinnyVC ? tb.Cinv[f] : tb.Aall[f];
+    const int lda = MODE == kSkinnyVC ? d.ldN : d.Kmax;
     const zd* M = tb.Mall[f];
     const zd* W0 = tb.W0[f];
     const zd* sm = small + (size_t)f * small_stride;
@@ -1203,7 +1205,7 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int Kp = min(kb + 2 * lq + j, Kc);
-                a[r][j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc[r]) : ldg(A, (size_t)nc[r] * d.Kmax + Kp);
+                a[r][j] = ROW ? ldg(M, (size_t)Kp * d.ldN + nc[r]) : ldg(A, (size_t)nc[r] * lda + Kp);
             }
     };
     auto body = [&](const zd (&a)[RT][2], int kloc, int bsel) {
@@ -1304,7 +1306,9 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             if (row >= d.N) continue;
             const double a = acc[r][c][v];
             // global-space pointers: the tables' entries would otherwise be flat accesses
-            if (!ROW) {
+            if (MODE == kSkinnyVC) {
+                ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * k + cc) * 2 + q] = a;
+            } else if (!ROW) {
                 gdbl* w = (gdbl*)(W0 + (size_t)row * d.ldN + t[cc]);
                 ((gdbl_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
             } else {
@@ -1313,6 +1317,28 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevT
             }
         }
     }
+}
+
+template <int CT, bool ROW, int KC, int DEPTH, int WAVES, bool DB, int RT>
+__global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma_kernel(Dims d, DevTabs tb, const int32_t* __restrict__ t,
+                                                                    int k, int K, const zd* __restrict__ small,
+                                                                    int small_stride, zd* Bt, size_t bt_stride,
+                                                                    const int32_t* dyn) {
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+        K = dyn[kDynKK];
+    }
+    skinny_mfma<CT, ROW ? kSkinnyRow : kSkinnyCol, KC, DEPTH, WAVES, DB, RT>(d, tb, t, k, K, small, small_stride, Bt,
+                                                                             bt_stride);
+}
+
+// VC (k x N, ld k, per frequency at f * vc_stride) = V C^-1 with V (k x N, ld k, at f * v_stride)
+template <int CT, int KC, int DEPTH, int WAVES, bool DB, int RT>
+__global__ void __launch_bounds__(WAVES * 64) plain_vcinv_mfma_kernel(Dims d, DevTabs tb, int k,
+                                                                      const zd* __restrict__ V, int v_stride, zd* VC,
+                                                                      size_t vc_stride) {
+    skinny_mfma<CT, kSkinnyVC, KC, DEPTH, WAVES, DB, RT>(d, tb, nullptr, k, d.N, V, v_stride, VC, vc_stride);
 }
 
 // Tile-shape variants offered to the autotuner, per column-tile count CT: (KC inner entries
@@ -1356,6 +1382,187 @@ static void launch_skinny_mfma(hipStream_t st, const Dims& d, const DevTabs& tb,
 }
 
 int skinny_variants(int k, bool c64) { return c64 ? (k <= 24 ? 4 : 1) : (k <= 24 ? 6 : 2); }
+
+template <int CT, int KC, int DEPTH, int WAVES, bool DB, int RT>
+static void launch_vc_variant(hipStream_t st, const Dims& d, const DevTabs& tb, int k, const zd* V, int v_stride,
+                              zd* VC, size_t vc_stride) {
+    const int rows = 16 * WAVES * RT;
+    dim3 grid((d.N + rows - 1) / rows, d.nf);
+    hipLaunchKernelGGL((plain_vcinv_mfma_kernel<CT, KC, DEPTH, WAVES, DB, RT>), grid, dim3(WAVES * 64), 0, st, d, tb,
+                       k, V, v_stride, VC, vc_stride);
+}
+
+void launch_plain_vcinv(hipStream_t st, const Dims& d, const DevTabs& tb, int k, const zd* V, size_t v_stride,
+                        zd* VC, size_t vc_stride, int variant) {
+#define PNGF_VCV(CT, KC, DEPTH, W, DB, RT) \
+    launch_vc_variant<CT, KC, DEPTH, W, DB, RT>(st, d, tb, k, V, (int)v_stride, VC, vc_stride)
+    if (k <= 24) {
+        switch (variant) {
+            case 1: PNGF_VCV(3, 32, 2, 4, true, 2); break;
+            case 2: PNGF_VCV(3, 64, 2, 8, false, 1); break;
+            case 3: PNGF_VCV(3, 16, 2, 4, true, 2); break;
+            case 5: PNGF_VCV(3, 32, 2, 4, true, 1); break;
+            case 0: PNGF_VCV(3, 32, 2, 8, true, 1); break;
+            default: PNGF_VCV(3, 16, 2, 8, true, 1); break;
+        }
+    } else if (k <= 48) {
+        if (variant % 2) PNGF_VCV(6, 16, 2, 8, false, 1);
+        else PNGF_VCV(6, 16, 2, 8, true, 1);
+    } else {
+        if (variant % 2) PNGF_VCV(12, 16, 2, 4, false, 1);
+        else PNGF_VCV(12, 16, 2, 8, false, 1);
+    }
+#undef PNGF_VCV
+}
+
+// C^-1 -= CU M: the rank-k update of the PLAIN accept (pngfopt.hpp:335-358, zgemm with alpha = -1)
+// with CU = C^-1[:, t] diag(s) (N x k, column c at c * ldN) and M = cap^-1 V C^-1 (k x N, row c at
+// c * ldN), on FP64 MFMA (v_mfma_f64_16x16x4f64).  An output tile's 16 rows are 16 columns j of
+// C^-1 and its 16 real columns are 8 complex rows i (re/im interleaved, the storage order), so
+//   C^-1[i][j] -= sum_c M[c][j] CU[i][c]
+// runs as the real product A_real (16 x 2k) x B_real (2k x 16) with A_real[j][2c + p] = component p
+// of -M[c][j] and, for b = CU[i][c], B_real[2c][2i] = Re b, B_real[2c][2i+1] = Im b,
+// B_real[2c+1][2i] = -Im b, B_real[2c+1][2i+1] = Re b.  The accumulator starts as the C^-1 tile,
+// so the MFMAs leave C^-1 - CU M in it.  Lane l = lr + 16 lq: A entry (column lr, inner 4s + lq),
+// B entry (inner 4s + lq, real column lr), D element v = column j0 + 4v + lq, complex row
+// lr / 2, component lr % 2: each group of 16 lanes covers 128 contiguous bytes of one column.
+// A workgroup of W waves owns 16 W columns and a range of rows, walked 8 TT rows at a time.  The
+// step's CU rows are staged in LDS (double-buffered: one barrier per step; the row pitch KB + 1
+// keeps the 16 lanes of a read on distinct banks), the A entries stay in registers for the
+// whole range, and the next step's C^-1 tiles are loaded while the current ones are multiplied
+// (TT output tiles per wave and step: fewer for large k, whose A entries take more registers).
+template <int KB, int W, int TT>
+__global__ void __launch_bounds__(W * 64) plain_rankk_kernel(Dims d, DevTabs tb, const zd* __restrict__ CU,
+                                                             size_t cu_stride, const zd* __restrict__ M,
+                                                             size_t m_stride, int k, int rows_per_wg) {
+    constexpr int NS = KB / 2;  // MFMA steps of 4 real inner entries (2 complex)
+    constexpr int RS = 8 * TT, KP = KB + 1, NT = W * 64;
+    constexpr int PER = (RS * KB + NT - 1) / NT;
+    __shared__ zd Bs[2][RS][KP];
+    const int f = blockIdx.z;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int j0 = (blockIdx.x * W + wave) * 16;
+    const int ib = blockIdx.y * rows_per_wg, ie = min(d.N, ib + rows_per_wg);
+    const zd* CUf = CU + (size_t)f * cu_stride;
+    const zd* Mf = M + (size_t)f * m_stride;
+    gdbl_w* C = (gdbl_w*)tb.Cinv[f];
+    double a[NS];
+    {
+        const int j = min(j0 + lr, d.N - 1);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int c = 2 * s + (lq >> 1);
+            double v = 0.0;
+            if (c < k) {
+                const zd m = ldg(Mf, (size_t)c * d.ldN + j);
+                v = (lq & 1) ? m.y : m.x;
+            }
+            a[s] = -v;
+        }
+    }
+    // B entry of this lane: component (p ^ q) of CU[i][c], negated for p = 1, q = 0
+    const int bcomp = (lq ^ lr) & 1;
+    const int bsign = ((lq & 1) && !(lr & 1)) ? (int)0x80000000 : 0;
+    auto stage_load = [&](int i0, zd (&r)[PER]) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            const int rr = e % RS, c = e / RS;  // consecutive threads: consecutive rows of one CU column
+            r[u] = zmk(0.0, 0.0);
+            if (e < RS * KB && c < k && i0 + rr < ie) r[u] = ldg(CUf, (size_t)c * d.ldN + i0 + rr);
+        }
+    };
+    auto stage_store = [&](const zd (&r)[PER], int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = threadIdx.x + u * NT;
+            if (e < RS * KB) Bs[buf][e % RS][e / RS] = r[u];
+        }
+    };
+    // C^-1 entries of the TT tiles x 4 elements of this lane for the step at row i0
+    auto tile_index = [&](int i0, int tt, int v, bool& ok) {
+        const int row = i0 + 8 * tt + (lr >> 1), col = j0 + 4 * v + lq;
+        ok = row < ie && col < d.N;
+        return ((size_t)col * d.ldN + row) * 2 + (lr & 1);
+    };
+    auto tile_load = [&](int i0, d4v (&x)[TT]) {
+#pragma unroll
+        for (int tt = 0; tt < TT; tt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                bool ok;
+                const size_t idx = tile_index(i0, tt, v, ok);
+                x[tt][v] = ok ? C[idx] : 0.0;
+            }
+    };
+    d4v cur[TT];
+    tile_load(ib, cur);
+    {
+        zd r[PER];
+        stage_load(ib, r);
+        stage_store(r, 0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int i0 = ib; i0 < ie; i0 += RS) {
+        const bool more = i0 + RS < ie;
+        d4v nxt[TT];
+        zd r[PER];
+        if (more) {
+            tile_load(i0 + RS, nxt);
+            stage_load(i0 + RS, r);
+        }
+        // B entries one inner step ahead of the MFMAs that take them
+        const double* Bd = (const double*)&Bs[buf][0][0] + ((lr >> 1) * KP + (lq >> 1)) * 2 + bcomp;
+        double bn[TT];
+#pragma unroll
+        for (int tt = 0; tt < TT; tt++) bn[tt] = Bd[8 * tt * KP * 2];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            double bc[TT];
+#pragma unroll
+            for (int tt = 0; tt < TT; tt++) bc[tt] = bn[tt];
+            if (s + 1 < NS)
+#pragma unroll
+                for (int tt = 0; tt < TT; tt++) bn[tt] = Bd[(8 * tt * KP + 2 * (s + 1)) * 2];
+#pragma unroll
+            for (int tt = 0; tt < TT; tt++) {
+                const double b = __hiloint2double(__double2hiint(bc[tt]) ^ bsign, __double2loint(bc[tt]));
+                cur[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, cur[tt], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int tt = 0; tt < TT; tt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                bool ok;
+                const size_t idx = tile_index(i0, tt, v, ok);
+                if (ok) C[idx] = cur[tt][v];
+            }
+        if (more) stage_store(r, buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+        if (more)
+#pragma unroll
+            for (int tt = 0; tt < TT; tt++) cur[tt] = nxt[tt];
+    }
+}
+
+void launch_plain_rankk(hipStream_t st, const Dims& d, const DevTabs& tb, const zd* CU, size_t cu_stride,
+                        const zd* M, size_t m_stride, int k) {
+    constexpr int W = 8;
+    const int rows_per_wg = 1024;
+    dim3 grid((d.N + 16 * W - 1) / (16 * W), (d.N + rows_per_wg - 1) / rows_per_wg, d.nf);
+#define PNGF_RKV(KB, TT)                                                                                          \
+    hipLaunchKernelGGL((plain_rankk_kernel<KB, W, TT>), grid, dim3(W * 64), 0, st, d, tb, CU, cu_stride, M, m_stride, \
+                       k, rows_per_wg)
+    if (k <= 24) PNGF_RKV(24, 4);
+    else if (k <= 48) PNGF_RKV(48, 4);
+    else PNGF_RKV(96, 2);
+#undef PNGF_RKV
+}
 
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
                       const zd* Mc, int small_stride, int variant, const int32_t* dyn) {

@@ -909,8 +909,8 @@ int accept_group(pngf_handle* h, int g) {
         if (h->c64) CK(accept_lowrank<zf>(h, g, k, nt, dt, ds, dlp, dgt));
         else CK(accept_lowrank<zd>(h, g, k, nt, dt, ds, dlp, dgt));
     } else if (k > 0) {
-        // PLAIN: re-score the flip keeping its factors, then pngfopt.hpp:256-367
-        const rbz one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+        // PLAIN: re-score the flip keeping its factors, then pngfopt.hpp:256-367 (V C^-1 and the
+        // rank-k update on the library's FP64-MFMA kernels)
         h->h_pin_i[kPinBatch] = g;
         HC(h, hipMemcpyAsync(h->d_batch, h->h_pin_i + kPinBatch, sizeof(int32_t), hipMemcpyHostToDevice, st));
         const int nsplit = std::max(1, std::min(64, h->nsplit_items / d.nf));
@@ -923,10 +923,7 @@ int accept_group(pngf_handle* h, int g) {
         const double N = d.N, kd = k;
         Timed t1{};
         prof_begin(h, t1);
-        for (int f = 0; f < d.nf; f++)
-            RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_none, k, d.N, d.N, &one,
-                                (rbz*)(h->d_V + f * h->v_stride), k, (rbz*)h->Cinv[f], d.ldN, &zero,
-                                (rbz*)(h->d_VC + f * h->v_stride), k));
+        launch_plain_vcinv(st, d, h->tabs, k, h->d_V, h->v_stride, h->d_VC, h->v_stride);
         prof_end(h, t1, PNGF_K_VCINV, d.nf * 16.0 * (N * N + 2 * N * kd), d.nf * 8.0 * kd * N * N);
         Timed t2{};
         prof_begin(h, t2);
@@ -936,10 +933,7 @@ int accept_group(pngf_handle* h, int g) {
         prof_end(h, t2, PNGF_K_ACC_SMALL, 0, 0);
         Timed t3{};
         prof_begin(h, t3);
-        for (int f = 0; f < d.nf; f++)
-            RB(h, rocblas_zgemm(h->rb, rocblas_operation_none, rocblas_operation_transpose, d.N, d.N, k, &mone,
-                                (rbz*)(h->d_CU + f * h->v_stride), d.ldN, (rbz*)(h->d_M + f * h->v_stride), d.ldN,
-                                &one, (rbz*)h->Cinv[f], d.ldN));
+        launch_plain_rankk(st, d, h->tabs, h->d_CU, h->v_stride, h->d_M, h->v_stride, k);
         prof_end(h, t3, PNGF_K_RANKK, d.nf * (32.0 * N * N + 32.0 * N * kd), d.nf * 8.0 * kd * N * N);
         Timed t4{};
         prof_begin(h, t4);

@@ -114,6 +114,58 @@ def test_accept_sequence(method):
         assert rel(e[i], ref) < RTOL
 
 
+@pytest.mark.parametrize("layout,flips", [
+    # disjoint: 24 components a tile, so k = 24, 48, 96 (each tile-shape bucket of the MFMA
+    # kernels); N = 1182 is not a multiple of their 16-, 32- or 128-wide tiles
+    (P.LAYOUT_DISJOINT, [[3], [17, 30], [17, 30], [8, 9, 15, 16], [40], [1, 2, 45, 47], [1, 2, 45, 47]]),
+    # shared edges: k = 12 .. 24 for single tiles
+    (P.LAYOUT_SHARED, [[3], [17], [30], [30], [44], [8]]),
+])
+def test_plain_cinv_after_accepts(layout, flips):
+    """The PLAIN accept (V C^-1 and C^-1 -= CU M on the FP64-MFMA kernels) leaves the whole of C^-1
+    equal to the oracle's and to a dense inverse of the current C (rows of G on metal components,
+    identity rows elsewhere) after each accept of 1-, 2- and 4-tile flips at 3 frequencies; a
+    flip accepted twice in a row (the metal set back as it was) returns C^-1 to its value before
+    the pair.  Bars: 1e-10 until a flip undoes the one before it.  That update cancels the
+    previous correction on every entry it touched, and the oracle's own C^-1 then moves 2.4e-10
+    (first such flip) and 5.2e-10 (second) from the dense inverse on the disjoint sequence (1.2e-12
+    before), so from there both are held to 2e-9 of the dense inverse and of each other."""
+    T = 7
+    freqs = freq_list(3, 28e9, 32e9)
+    Gs = [P.synth_g_host(layout, T, 1, hz, soft=SOFT_WELL) for hz in freqs]
+    gpu, ors, tm = setup_pair(layout, T, freqs, P.METHOD_PLAIN, max_flips=4, Gs=Gs)
+    N = Gs[0].shape[0]
+    port = set(O.layout_port(layout, T, 3, 3).tolist())
+    tm = tm.copy()
+    hist = [[gpu.cinv(f) for f in range(len(freqs))]]
+    tol = RTOL
+    for n, fl in enumerate(flips):
+        tmf = np.zeros(T * T, dtype=np.int32)
+        tmf[fl] = 1
+        assert not set(np.nonzero(O.layout_mask(layout, T, tmf))[0].tolist()) & port
+        if n > 0 and flips[n - 1] == fl:
+            tol = 2e-9
+        tm[fl] ^= 1
+        mask = O.layout_mask(layout, T, tm).astype(bool)
+        for o in ors:
+            o.compute_updated_Eobj(fl)
+            o.accept()
+        gpu.accept(fl)
+        hist.append([gpu.cinv(f) for f in range(len(freqs))])
+        for f, o in enumerate(ors):
+            C = np.eye(N, dtype=np.complex128)
+            C[mask, :] = Gs[f][mask, :N]
+            dense = np.linalg.inv(C)
+            assert rel(hist[-1][f], o.cinv()) < tol, (n, fl, f)
+            assert rel(hist[-1][f], dense) < tol, (n, fl, f)
+            assert rel(o.cinv(), dense) < tol, (n, fl, f)
+        E, _ = oracle_state(ors)
+        assert rel(gpu.state()["Eobj"], E) < RTOL
+        if n > 0 and flips[n - 1] == fl:
+            for f in range(len(freqs)):
+                assert rel(hist[-1][f], hist[-3][f]) < tol, (n, fl, f)
+
+
 def run_both(gpu, ors, freqs, objective, seed=0, max_evals=0):
     g = gpu.dbs_run(max_evals=max_evals, seed=seed)
     o = O.dbs_run(ors, objective, freqs, DX, DY, DZ, seed=seed, max_evals=max_evals)
