@@ -53,10 +53,13 @@ def build(T, freqs, tm, method=P.METHOD_LOWRANK, soft=0.0, max_deferred=0):
 def test_drift_at_benchmark_conditioning():
     """21x21 x 9, default softening: DBS to convergence with each method (183 accepts), then the
     same tile map initialised afresh.  Both paths take the same accepted sequence.  Measured on
-    one MI355X: the maintained Eobj is 4.0e-4 (relative to its largest entry) from the fresh one
-    for both methods, the objective 6.7e-6 (LOWRANK) and 4.7e-5 (PLAIN) from the fresh
-    objective.  Asserted: Eobj within 1e-3, objective within 1e-4, LOWRANK's objective drift no
-    larger than PLAIN's and its Eobj drift within 5 % of PLAIN's."""
+    one MI355X: the maintained Eobj (ne x F values, relative to the largest entry) is 4.0e-4
+    from the fresh one for LOWRANK and 6.9e-4 for PLAIN; the objective 6.7e-6 (LOWRANK) and
+    1.1e-6 (PLAIN) from the fresh objective.  Which method lands nearer in the one scalar
+    objective follows the rounding of the accept's sums (PLAIN's rank-k update on rocBLAS zgemm
+    gave 4.7e-5 on the same run), so the comparison is made on the state.  Asserted: Eobj
+    within 1e-3 and objective within 1e-5 for both methods, LOWRANK's Eobj drift within 5 % of
+    PLAIN's or below it."""
     wl = WORKLOADS["antenna21x21_9f"]
     T, freqs = wl["T"], wl["freqs"]
     tm0 = initial_pattern(T, 0)
@@ -80,8 +83,7 @@ def test_drift_at_benchmark_conditioning():
     assert lo["r"]["converged"] and pl["r"]["converged"] and lo["r"]["accepts"] > 100
     assert np.array_equal(lo["r"]["acc_tiles"], pl["r"]["acc_tiles"])
     assert lo["eobj"] < 1e-3 and pl["eobj"] < 1e-3
-    assert lo["F"] < 1e-4 and pl["F"] < 1e-4
-    assert lo["F"] <= pl["F"]
+    assert lo["F"] < 1e-5 and pl["F"] < 1e-5
     assert lo["eobj"] <= 1.05 * pl["eobj"]
 
 
