@@ -828,15 +828,24 @@ __global__ void plain_finish_kernel(Dims d, DevVecs v, GroupTab gt, const int32_
     }
 }
 
+// split-N pass of the PLAIN score on FP64 MFMA (defined with the other MFMA kernels below); false
+// when the group size does not fit its tiles
+bool launch_plain_partial_mfma(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
+                               const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
+                               int nitems, int nsplit, zd* partial);
+
 void launch_plain_eval(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
                        const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
                        int nb, int nsplit, zd* partial, zd* out_eobj, int32_t* out_info, zd* store,
                        int acc_stride, zd* gscratch) {
     const int nitems = nb * d.nf;
-    const int ntiles_max = ((d.gmax + d.ne + PL_TILE - 1) / PL_TILE) * ((d.gmax + 1 + PL_TILE - 1) / PL_TILE);
-    hipLaunchKernelGGL(plain_partial_kernel, dim3(nitems, nsplit, ntiles_max), dim3(256),
-                       plain_partial_smem(d.gmax, d.ne), st, d, t, v, gt, tile, P, owners, batch, nsplit,
-                       plain_nc(d.gmax), partial);
+    // groups of up to 31 components (k + ne <= 32): the FP64-MFMA pass; larger ones: VALU tiles
+    if (!launch_plain_partial_mfma(st, d, t, v, gt, tile, P, owners, batch, nitems, nsplit, partial)) {
+        const int ntiles_max = ((d.gmax + d.ne + PL_TILE - 1) / PL_TILE) * ((d.gmax + 1 + PL_TILE - 1) / PL_TILE);
+        hipLaunchKernelGGL(plain_partial_kernel, dim3(nitems, nsplit, ntiles_max), dim3(256),
+                           plain_partial_smem(d.gmax, d.ne), st, d, t, v, gt, tile, P, owners, batch, nsplit,
+                           plain_nc(d.gmax), partial);
+    }
     size_t sm = 4 * 96 * sizeof(int) + 100 * sizeof(zd) + lds_matrix_bytes(d.gmax, gscratch);
     hipLaunchKernelGGL(plain_finish_kernel, dim3(nitems), dim3(d.gmax <= 32 ? 64 : 256), sm, st, d, v, gt, tile,
                        P, owners, batch, nsplit, partial, out_eobj, out_info, store, acc_stride, gscratch);
@@ -1562,6 +1571,206 @@ void launch_plain_rankk(hipStream_t st, const Dims& d, const DevTabs& tb, const 
     else if (k <= 48) PNGF_RKV(48, 4);
     else PNGF_RKV(96, 2);
 #undef PNGF_RKV
+}
+
+// PLAIN score, split-N pass on FP64 MFMA (reference pngfopt.hpp:420-480, fused as in
+// plain_partial_kernel): for the components n of one split,
+//   X = [V ; Gobj] [C^-1[:, t] , Snew]     ((k + ne) x (k + 1) complex),
+// V = (G - I)[t, :] (rows of the row-major G copy), Snew = S - C^-1[:, t] (s o Einc_t) formed on
+// the fly.  The complex product runs as the real one X_real = A_real B_real with
+// A_real[r][2n + p] = component p of A[r][n] (the storage order) and, for b = B[n][c],
+// B_real[2n][2c] = Re b, B_real[2n][2c+1] = Im b, B_real[2n+1][2c] = -Im b,
+// B_real[2n+1][2c+1] = Re b, in RT x CT tiles of v_mfma_f64_16x16x4f64 (RT = CT / 2 = 2 holds
+// k + ne <= 32 rows and k + 1 <= 32 columns).  In a group of 8 components n0 .. n0 + 7 lane
+// (lr, lq) supplies inner entries (n0 + 2 lq + e, p) at step 2e + p.  Its A operands of the group
+// are one 32-byte piece of a row of G or Gobj; its B operand is the entry of column
+// c = 8 ct + lr / 2 of C^-1 at component n0 + 2 lq + (lr & 1), the pair's other entry coming from
+// lane lr ^ 1 (DPP).  All of it is loaded straight from HBM (each 16-lane quarter reads 128
+// contiguous bytes per row or column), the next group's pieces in flight while the current one
+// is multiplied.  The Snew column: each lane's share of sum_c C^-1[n][t_c] es_c at its component
+// is summed over the column tiles, then over the 8 lanes of the same parity by an xor butterfly
+// (every lane ends with the same bits), and S[n] minus it takes the operand slot of column k.  Each wave owns a quarter of the split; the four wave sums are
+// added in LDS in wave order and written in plain_partial_kernel's layout.
+// x from lane lane ^ 1 (DPP quad_perm [1, 0, 3, 2])
+__device__ inline double dpp_swap1(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int RT, int CT>
+__global__ void __launch_bounds__(256, 2) plain_partial_mfma_kernel(Dims d, DevTabs tb, DevVecs v, GroupTab gt,
+        const int32_t* __restrict__ tile, const int8_t* __restrict__ P, const int32_t* __restrict__ owners,
+        const int32_t* __restrict__ batch, int nsplit, zd* __restrict__ partial) {
+    constexpr int NW = 4, NTL = RT * CT;
+    __shared__ int t[96], sg[96], lp[96];
+    __shared__ zd es[CT * 8];
+    __shared__ int s_k;
+    __shared__ double red[NTL][4][64];
+    const int item = blockIdx.x, split = blockIdx.y;
+    const int b = item / d.nf, f = item - b * d.nf;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4, q = lr & 1;
+    if (tid == 0) s_k = change_set(gt, batch[b], tile, P, owners, t, sg, lp);
+    __syncthreads();
+    const int k = s_k, ne = d.ne;
+    if (tid < CT * 8) {
+        zd e = zmk(0.0, 0.0);
+        if (tid < k) {
+            e = ldg(v.Einc + (size_t)f * d.ldN, t[tid]);
+            if (sg[tid] < 0) e = zmk(-e.x, -e.y);
+        }
+        es[tid] = e;
+    }
+    __syncthreads();
+    const zd* Gr = tb.Grow[f];
+    const zd* Ci = tb.Cinv[f];
+    const zd* Sv = v.S + (size_t)f * d.ldN;
+    // operand rows and columns of this lane; absent ones read row t[0] (the same cache lines) and
+    // are zeroed
+    const int t0 = k > 0 ? t[0] : 0;  // a valid row for absent operands (k = 0: no change)
+    const zd* arow[RT];
+    int adiag[RT];
+    bool aon[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        const int r = 16 * rt + lr;
+        aon[rt] = r < k + ne;
+        adiag[rt] = r < k ? t[r] : -1;
+        arow[rt] = r < k ? Gr + (size_t)t[r] * d.ldN
+                         : (r < k + ne ? v.Gobj + ((size_t)f * ne + (r - k)) * d.ldN : Gr + (size_t)t0 * d.ldN);
+    }
+    const zd* bcol[CT];
+    bool bon[CT];
+    zd esc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int c = 8 * ct + (lr >> 1);
+        bon[ct] = c < k;
+        bcol[ct] = Ci + (size_t)(c < k ? t[c] : t0) * d.ldN;
+        esc[ct] = es[c];
+    }
+    const int cts = (2 * k) >> 4;                     // column tile of Snew
+    const bool snew_lane = (lr >> 1) == (k & 7);      // lanes of Snew's two real columns
+    // this wave's components: a span of a multiple of 8
+    const int nsw = nsplit * NW;
+    const int span = (((d.N + nsw - 1) / nsw) + 7) & ~7;
+    const int n_begin = (split * NW + wave) * span, n_end = min(d.N, n_begin + span);
+    const int nlast = (d.N - 1) & ~1;                 // last even component: a pair load stays in ldN
+    d4v acc[RT][CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) acc[rt][ct] = d4v{0.0, 0.0, 0.0, 0.0};
+    // operands of one group of 8 components: A entries at both components of the lane's pair,
+    // B and S at the component of the lane's parity (n0 + 2 lq + q); the other parity's B entry
+    // comes from lane lr ^ 1 (same column) in the body
+    struct Grp {
+        zd a[RT][2], b[CT], s;
+    };
+    auto load = [&](int n0, Grp& g) {
+        const int na = n0 + 2 * lq;
+        const int ia = min(na, nlast);
+        const bool va = na < n_end, vb = na + 1 < n_end, vq = q ? vb : va;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            zd x0 = ldg(arow[rt], ia), x1 = ldg(arow[rt], ia + 1);
+            if (ia == adiag[rt]) x0.x -= 1.0;
+            if (ia + 1 == adiag[rt]) x1.x -= 1.0;
+            g.a[rt][0] = (aon[rt] && va) ? x0 : zmk(0.0, 0.0);
+            g.a[rt][1] = (aon[rt] && vb) ? x1 : zmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            const zd x = ldg(bcol[ct], ia + q);
+            g.b[ct] = (bon[ct] && vq) ? x : zmk(0.0, 0.0);
+        }
+        const zd sx = ldg(Sv, ia + q);
+        g.s = vq ? sx : zmk(0.0, 0.0);
+    };
+    auto body = [&](Grp& g) {
+        // Snew = S - sum_c C^-1[n][t_c] es_c at this lane's component n = n0 + 2 lq + q
+        zd part = zmk(0.0, 0.0);
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) part = zfma(g.b[ct], esc[ct], part);
+#pragma unroll
+        for (int off = 2; off < 16; off <<= 1) {
+            part.x += __shfl_xor(part.x, off, 64);
+            part.y += __shfl_xor(part.y, off, 64);
+        }
+        const zd sn = zsub(g.s, part);
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++)
+            if (ct == cts && snew_lane) g.b[ct] = sn;
+        double bb[CT][4];
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            // entries at the pair's two components: this lane's and lane lr ^ 1's
+            const zd own = g.b[ct];
+            const zd oth = zmk(dpp_swap1(own.x), dpp_swap1(own.y));
+            const zd x0 = q ? oth : own, x1 = q ? own : oth;
+            // B_real entries of real column 2c + q at inner (n, p): p = 0 -> (Re, Im)[q], p = 1 ->
+            // (-Im, Re)[q]
+            bb[ct][0] = q ? x0.y : x0.x;
+            bb[ct][1] = q ? x0.x : -x0.y;
+            bb[ct][2] = q ? x1.y : x1.x;
+            bb[ct][3] = q ? x1.x : -x1.y;
+        }
+        // steps (n_a, Re), (n_a, Im), (n_b, Re), (n_b, Im), each over all RT x CT accumulators
+#pragma unroll
+        for (int st = 0; st < 4; st++)
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                const zd ax = g.a[rt][st >> 1];
+                const double av = (st & 1) ? ax.y : ax.x;
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bb[ct][st], acc[rt][ct], 0, 0, 0);
+            }
+    };
+    if (n_begin < n_end) {
+        Grp cur;
+        load(n_begin, cur);
+        for (int n0 = n_begin; n0 < n_end; n0 += 8) {
+            Grp nxt;
+            const bool more = n0 + 8 < n_end;
+            if (more) load(n0 + 8, nxt);
+            body(cur);
+            __builtin_amdgcn_sched_barrier(0);
+            if (more) cur = nxt;
+        }
+    }
+    // the four wave sums in wave order
+    for (int w = 0; w < NW; w++) {
+        if (wave == w)
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        double& x = red[rt * CT + ct][u][lane];
+                        x = w == 0 ? acc[rt][ct][u] : x + acc[rt][ct][u];
+                    }
+        __syncthreads();
+    }
+    // D[i][j] of a tile sits in lane j + 16 (i % 4), element i / 4
+    const int pstride = (d.gmax + ne) * (d.gmax + 1);
+    zd* out = partial + ((size_t)item * nsplit + split) * pstride;
+    for (int e = tid; e < (k + ne) * (k + 1); e += 256) {
+        const int r = e / (k + 1), c = e - r * (k + 1);
+        const int i = r & 15, j = (2 * c) & 15;
+        const int tl = (r >> 4) * CT + ((2 * c) >> 4), ln = j + 16 * (i & 3);
+        out[r * (d.gmax + 1) + c] = zmk(red[tl][i >> 2][ln], red[tl][i >> 2][ln + 1]);
+    }
+}
+
+bool launch_plain_partial_mfma(hipStream_t st, const Dims& d, const DevTabs& t, const DevVecs& v, GroupTab gt,
+                               const int32_t* tile, const int8_t* P, const int32_t* owners, const int32_t* batch,
+                               int nitems, int nsplit, zd* partial) {
+    if (d.gmax + d.ne > 32 || d.gmax + 1 > 32) return false;
+    hipLaunchKernelGGL((plain_partial_mfma_kernel<2, 4>), dim3(nitems, nsplit), dim3(256), 0, st, d, t, v, gt, tile,
+                       P, owners, batch, nsplit, partial);
+    return true;
 }
 
 void launch_lr_colblk(hipStream_t st, const Dims& d, const DevTabs& tb, const int32_t* t, int k, int K,
