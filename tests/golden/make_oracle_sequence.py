@@ -1,0 +1,79 @@
+"""Writes tests/golden/oracle_antenna21x21_9f.npz: the oracle's whole DBS run on the benchmark's
+workload (21x21 tiles, disjoint layout, N = 10590, 9 frequencies 26..34 GHz, default synthetic G,
+Bernoulli(0.5) start from mt19937(0), raster first improvement of mean |S11|^2 to convergence).
+
+The oracle is the C++ restatement of pngf-opt/pngfopt.hpp (compute_updated_Eobj at
+pngfopt.hpp:379-539, update_C_inv at 256-367) driven as main.cpp:338-459 with the raster order.
+C^-1 of the start is formed as pngf-opt/linalg.hpp:34-85 does it, by LAPACK zgetrf + zgetri (here
+the OpenBLAS build that scipy ships), from C = I + P (G - I) (pngfopt.hpp:196-233).  G comes from
+the library's host synthesiser (pngf_synth_g_host, the formula of the device one).
+
+Stored: the accepted tiles, the evaluation index of each accept, the objective after each
+accept, the start and final objectives and the evaluation count.  At this conditioning
+(cond(C) ~ 1e5-1e6) the accepted sequence depends on the rounding of the sums, so the fixture
+records what the reference-structured arithmetic gives.
+
+Run (about 30 minutes and 40 GB of host memory on 8 cores):
+    python tests/golden/make_oracle_sequence.py
+"""
+import gc
+import os
+import sys
+import time
+
+import numpy as np
+import scipy.linalg.lapack as LA
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+import pngf_amd as P  # noqa: E402
+from pngf_amd.problem import WORKLOADS, initial_pattern  # noqa: E402
+
+DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
+
+
+def main():
+    wl = WORKLOADS["antenna21x21_9f"]
+    T, freqs, layout = wl["T"], wl["freqs"], wl["layout"]
+    tm = initial_pattern(T, 0)
+    mid = (T - 1) // 2
+    metal = O.layout_mask(layout, T, tm).astype(bool)
+    opts = []
+    t0 = time.time()
+    for f, hz in enumerate(freqs):
+        G = P.synth_g_host(layout, T, 1, hz)
+        o = O.Optimizer(layout, T, G, ne=1)
+        o.set_tile_map(tm)
+        o.set_source_port(mid, mid, 1, 2)
+        # C = I + P (G - I): rows of G at metal components, identity rows elsewhere
+        C = np.array(G, order="F")
+        del G
+        void = np.nonzero(~metal)[0]
+        C[void, :] = 0
+        C[void, void] = 1.0
+        lu, piv, info = LA.zgetrf(C, overwrite_a=True)
+        del C
+        assert info == 0
+        inv, info = LA.zgetri(lu, piv, overwrite_lu=True)
+        assert info == 0
+        o.set_cinv(inv)
+        del inv, lu
+        gc.collect()
+        o.compute_PEinc()
+        o.compute_R_matrix(1)
+        opts.append(o)
+        print("frequency %d initialised (%.0f s)" % (f, time.time() - t0), flush=True)
+    t1 = time.time()
+    r = O.dbs_run(opts, O.OBJ_S11, freqs, DX, DY, DZ)
+    print("DBS %.0f s: %d evaluations, %d accepts, F %.10g -> %.10g" % (
+        time.time() - t1, r["flip_count"], r["accepts"], r["F_start"], r["F_final"]), flush=True)
+    np.savez(os.path.join(HERE, "oracle_antenna21x21_9f.npz"), acc_tiles=r["acc_tiles"],
+             acc_eval=r["acc_eval"], acc_F=r["acc_F"], F_start=r["F_start"], F_final=r["F_final"],
+             flip_count=r["flip_count"])
+
+
+if __name__ == "__main__":
+    main()

@@ -235,6 +235,22 @@ def test_golden_fixture():
             assert rel(o.scalars()["Eobj"], want) < 1e-10, (name, f)
 
 
+def test_benchmark_sequence_fixture():
+    """The oracle's whole run on the benchmark's workload (tests/golden/make_oracle_sequence.py,
+    21x21 tiles x 9 frequencies) has the shape first-improvement raster DBS gives
+    (main.cpp:338-459): accepts at strictly increasing evaluations, each strictly improving the
+    objective (main.cpp:429), and the run ending after one full pass of the 441 tiles without an
+    improving flip."""
+    z = np.load(os.path.join(HERE, "golden", "oracle_antenna21x21_9f.npz"))
+    t, e, F = z["acc_tiles"], z["acc_eval"], z["acc_F"]
+    assert len(t) == len(e) == len(F) > 100
+    assert t.min() >= 0 and t.max() < 21 * 21
+    assert np.all(np.diff(e) > 0) and e[0] >= 0
+    assert F[0] < float(z["F_start"]) and np.all(np.diff(F) < 0)
+    assert F[-1] == float(z["F_final"])
+    assert int(z["flip_count"]) == e[-1] + 1 + 21 * 21
+
+
 # ---------------------------------------------------------------- objective --
 def test_objective_known_answers():
     # Zin = -(E0 + Vinc) dx/2 / (2 dy dz) = -4500 (E0 + Vinc) for the reference cell
