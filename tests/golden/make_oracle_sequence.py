@@ -1,20 +1,28 @@
-"""Writes tests/golden/oracle_antenna21x21_9f.npz: the oracle's whole DBS run on the benchmark's
-workload (21x21 tiles, disjoint layout, N = 10590, 9 frequencies 26..34 GHz, default synthetic G,
-Bernoulli(0.5) start from mt19937(0), raster first improvement of mean |S11|^2 to convergence).
+"""Writes the oracle's whole DBS runs on two full-size workloads as fixtures:
+
+* tests/golden/oracle_antenna21x21_9f.npz: the benchmark's workload (21x21 tiles, disjoint
+  layout, N = 10590, 9 frequencies 26..34 GHz, default synthetic G, Bernoulli(0.5) start from
+  mt19937(0), raster first improvement of mean |S11|^2 to convergence);
+* tests/golden/oracle_reference_default.npz: the reference program's own default configuration
+  (main.cpp:116-117, 154-160, 181-241): 21x21 tiles on the shared Yee grid (N = 8064), 5
+  frequencies 25..35 GHz, ne = 3, the directivity objective, MagicHat order with mirror-symmetric
+  2/4-tile flips, seed -514213645, port at tile (10, 10), default synthetic G, start
+  initial_pattern(21, 0), to convergence.
 
 The oracle is the C++ restatement of pngf-opt/pngfopt.hpp (compute_updated_Eobj at
-pngfopt.hpp:379-539, update_C_inv at 256-367) driven as main.cpp:338-459 with the raster order.
-C^-1 of the start is formed as pngf-opt/linalg.hpp:34-85 does it, by LAPACK zgetrf + zgetri (here
-the OpenBLAS build that scipy ships), from C = I + P (G - I) (pngfopt.hpp:196-233).  G comes from
-the library's host synthesiser (pngf_synth_g_host, the formula of the device one).
+pngfopt.hpp:379-539, update_C_inv at 256-367) driven as main.cpp:338-459.  C^-1 of the start is
+formed as pngf-opt/linalg.hpp:34-85 does it, by LAPACK zgetrf + zgetri (here the OpenBLAS build
+that scipy ships), from C = I + P (G - I) (pngfopt.hpp:196-233).  G comes from the library's host
+synthesiser (pngf_synth_g_host, the formula of the device one).
 
-Stored: the accepted tiles, the evaluation index of each accept, the objective after each
-accept, the start and final objectives and the evaluation count.  At this conditioning
-(cond(C) ~ 1e5-1e6) the accepted sequence depends on the rounding of the sums, so the fixture
-records what the reference-structured arithmetic gives.
+Stored per run: the accepted tiles (first tile of each flip), the evaluation index of each accept,
+the objective after each accept, the start and final objectives and the evaluation count.  At
+the default softening (cond(C) ~ 1e5-1e6) the accepted sequence depends on the rounding of the
+sums, so a fixture records what the reference-structured arithmetic gives.
 
-Run (about 30 minutes and 40 GB of host memory on 8 cores):
-    python tests/golden/make_oracle_sequence.py
+Run (8 host cores; the benchmark's workload takes about 45 minutes and 40 GB of host memory, the
+reference default about 10 minutes):
+    python tests/golden/make_oracle_sequence.py [antenna21x21_9f | reference_default]
 """
 import gc
 import os
@@ -34,22 +42,31 @@ from pngf_amd.problem import WORKLOADS, initial_pattern  # noqa: E402
 
 DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
 
+CASES = {
+    "antenna21x21_9f": dict(T=21, layout=WORKLOADS["antenna21x21_9f"]["layout"],
+                            freqs=WORKLOADS["antenna21x21_9f"]["freqs"], ne=1, max_flips=1,
+                            objective=O.OBJ_S11, seed=0),
+    "reference_default": dict(T=21, layout=O.LAYOUT_SHARED, freqs=[25e9, 27.5e9, 30e9, 32.5e9, 35e9], ne=3,
+                              max_flips=4, objective=O.OBJ_REF, seed=-514213645),
+}
 
-def main():
-    wl = WORKLOADS["antenna21x21_9f"]
-    T, freqs, layout = wl["T"], wl["freqs"], wl["layout"]
+
+def run(name):
+    c = CASES[name]
+    T, layout, ne = c["T"], c["layout"], c["ne"]
     tm = initial_pattern(T, 0)
     mid = (T - 1) // 2
     metal = O.layout_mask(layout, T, tm).astype(bool)
     opts = []
     t0 = time.time()
-    for f, hz in enumerate(freqs):
-        G = P.synth_g_host(layout, T, 1, hz)
-        o = O.Optimizer(layout, T, G, ne=1)
+    for f, hz in enumerate(c["freqs"]):
+        G = P.synth_g_host(layout, T, ne, hz)
+        o = O.Optimizer(layout, T, G, ne=ne, max_flips=c["max_flips"])
         o.set_tile_map(tm)
         o.set_source_port(mid, mid, 1, 2)
-        # C = I + P (G - I): rows of G at metal components, identity rows elsewhere
-        C = np.array(G, order="F")
+        # C = I + P (G - I) on the design block: rows of G at metal components, identity elsewhere
+        N = o.N
+        C = np.array(G[:N, :N], order="F")
         del G
         void = np.nonzero(~metal)[0]
         C[void, :] = 0
@@ -65,15 +82,15 @@ def main():
         o.compute_PEinc()
         o.compute_R_matrix(1)
         opts.append(o)
-        print("frequency %d initialised (%.0f s)" % (f, time.time() - t0), flush=True)
+        print("%s: frequency %d initialised (%.0f s)" % (name, f, time.time() - t0), flush=True)
     t1 = time.time()
-    r = O.dbs_run(opts, O.OBJ_S11, freqs, DX, DY, DZ)
-    print("DBS %.0f s: %d evaluations, %d accepts, F %.10g -> %.10g" % (
-        time.time() - t1, r["flip_count"], r["accepts"], r["F_start"], r["F_final"]), flush=True)
-    np.savez(os.path.join(HERE, "oracle_antenna21x21_9f.npz"), acc_tiles=r["acc_tiles"],
-             acc_eval=r["acc_eval"], acc_F=r["acc_F"], F_start=r["F_start"], F_final=r["F_final"],
-             flip_count=r["flip_count"])
+    r = O.dbs_run(opts, c["objective"], c["freqs"], DX, DY, DZ, seed=c["seed"])
+    print("%s: DBS %.0f s, %d evaluations, %d accepts, F %.10g -> %.10g" % (
+        name, time.time() - t1, r["flip_count"], r["accepts"], r["F_start"], r["F_final"]), flush=True)
+    np.savez(os.path.join(HERE, "oracle_%s.npz" % name), acc_tiles=r["acc_tiles"], acc_eval=r["acc_eval"],
+             acc_F=r["acc_F"], F_start=r["F_start"], F_final=r["F_final"], flip_count=r["flip_count"])
 
 
 if __name__ == "__main__":
-    main()
+    for name in (sys.argv[1:] or list(CASES)):
+        run(name)

@@ -118,6 +118,46 @@ def test_benchmark_run_follows_oracle():
     assert rel(r["acc_F"], gold["acc_F"]) < 1e-4
 
 
+GOLDEN_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_reference_default.npz")
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_reference_default_run_follows_oracle(method):
+    """The reference program's own default configuration (main.cpp:116-117, 154-160, 181-241) at
+    the default softening, whole run: 21x21 tiles on the shared Yee grid (N = 8064), 5 frequencies
+    25..35 GHz, ne = 3, the directivity objective, MagicHat order with mirror-symmetric 2/4-tile
+    flips (k up to 96), seed -514213645, start initial_pattern(21, 0).  Against the oracle's
+    whole run on it (tests/golden/make_oracle_sequence.py, C^-1 by LAPACK): both methods accept
+    the same flips at the same evaluations, the start objective agrees to 1e-10 and every
+    accepted objective to 1e-6 relative."""
+    gold = np.load(GOLDEN_REF)
+    T, ne, mid = 21, 3, 10
+    freqs = [25e9, 27.5e9, 30e9, 32.5e9, 35e9]
+    o = P.Optimizer(P.LAYOUT_SHARED, T, num_freq=len(freqs), num_eobj=ne, max_flips=4, method=method,
+                    objective=P.OBJ_REF, driver=P.DRIVER_REF)
+    try:
+        for f, hz in enumerate(freqs):
+            o.set_frequency(f, hz)
+            o.synth_g(f)
+        o.set_tile_map(initial_pattern(T, 0))
+        o.set_source_port(mid, mid, 1, 2)
+        o.init_state()
+        assert o.N == 8064
+        r = o.dbs_run(seed=-514213645)
+    finally:
+        o.close()
+    print("[reference default] %d evals %d accepts F %.10g; oracle %d evals %d accepts F %.10g" % (
+        r["flip_count"], r["accepts"], r["F_final"], int(gold["flip_count"]), len(gold["acc_tiles"]),
+        float(gold["F_final"])))
+    assert r["converged"]
+    assert abs(r["F_start"] - float(gold["F_start"])) < 1e-10 * abs(float(gold["F_start"]))
+    assert r["flip_count"] == int(gold["flip_count"])
+    assert np.array_equal(r["acc_tiles"], gold["acc_tiles"])
+    assert np.array_equal(r["acc_eval"], gold["acc_eval"])
+    assert rel(r["acc_F"], gold["acc_F"]) < 1e-6
+
+
 @pytest.mark.timeout(900)
 def test_31x31_7f_state_matches_fresh_init():
     """31x31 tiles x 7 frequencies 27..33 GHz (N = 23070, 8.5 GB of W per frequency), softening
