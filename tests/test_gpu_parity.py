@@ -378,16 +378,36 @@ def test_empty_design_init(method):
     gpu.close()
 
 
-def test_checkpoint_restore_repeats_run():
+@pytest.mark.parametrize("method", [P.METHOD_LOWRANK, P.METHOD_PLAIN])
+def test_checkpoint_restore_repeats_run(method):
+    """A device checkpoint taken mid-run (after 20 evaluations, accepts already in the state)
+    and restored after a second run went on to convergence: the run from the restored state
+    repeats the second one bitwise, its state is the oracle's at the checkpoint (Eobj to RTOL),
+    and it is the oracle's continuation from there (same accepted flips at the same
+    evaluations, objective to RTOL)."""
     T = 7
     freqs = [30e9]
-    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, P.METHOD_LOWRANK)
-    gpu.checkpoint()
-    a = gpu.dbs_run()
-    gpu.restore()
-    b = gpu.dbs_run()
+    gpu, ors, tm = setup_pair(P.LAYOUT_DISJOINT, T, freqs, method)
+    try:
+        g1 = gpu.dbs_run(max_evals=20)
+        o1 = O.dbs_run(ors, O.OBJ_S11, freqs, DX, DY, DZ, max_evals=20)
+        assert g1["accepts"] == o1["accepts"] > 0
+        assert np.array_equal(g1["acc_tiles"], o1["acc_tiles"])
+        gpu.checkpoint()
+        a = gpu.dbs_run()
+        gpu.restore()
+        E, _ = oracle_state(ors)
+        assert rel(gpu.state()["Eobj"], E) < RTOL
+        b = gpu.dbs_run()
+        o2 = O.dbs_run(ors, O.OBJ_S11, freqs, DX, DY, DZ)
+    finally:
+        gpu.close()
     assert np.array_equal(a["acc_tiles"], b["acc_tiles"])
     assert a["acc_F"].tolist() == b["acc_F"].tolist()
+    assert b["accepts"] == o2["accepts"] > 0
+    assert np.array_equal(b["acc_tiles"], o2["acc_tiles"])
+    assert np.array_equal(b["acc_eval"], o2["acc_eval"])
+    assert rel(b["acc_F"], o2["acc_F"]) < RTOL
 
 
 def test_dbs_run_many_matches_sequential():
