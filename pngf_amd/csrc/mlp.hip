@@ -30,7 +30,10 @@ __device__ inline short bf16_rne(float f) {
 }
 
 // C[i][j] (+)= sum_k A(i,k) B(k,j), A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj],
-// operands rounded to BF16, FP32 accumulation.  One wave per 16x16 tile of C.
+// operands rounded to BF16, FP32 accumulation.  One wave per 16x16 tile of C: the ranker's
+// products are small (training batches of 512 rows, widths up to 256: at most 34 M MACs each) and
+// bound by latency, and a tile per wave puts the most waves in flight; the operands come from
+// the L2 without LDS staging, each entry read by the 16 (or 4) tiles that share it.
 // Fragment layouts (16x16x16 bf16): A[i][k] in lane i + 16(k/4), element k%4;
 // B[k][j] in lane j + 16(k/4), element k%4; D[i][j] in lane j + 16(i/4), element i%4.
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, const float* __restrict__ A, int sai,
