@@ -17,6 +17,7 @@ projected to the run's counts), config_7x7_1f (the reference's CPU-runnable 7x7 
 beside the oracle), reference_config (the reference program's own default configuration:
 LOWRANK and PLAIN on the GPU),
 plain_fp64_gpu (the reference formulation on the GPU, same start), speedup_vs_plain,
+vs_oracle_run (the run against the oracle's recorded whole run of the default workload),
 best_improvement_variant, time_to_solution_s, per-kernel-category statistics, autotuner
 choice and the symmetric-accept decision.
 """
@@ -47,6 +48,23 @@ STRIP_COUNTED = dict(N=10590, freqs=9, variant=4, launches=HEADLINE_ACCEPTS, alg
                      bytes_per_launch=3507488159.6, mfma_util=0.7615, clock_ghz=2.223,
                      source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (copy-calibrated), MfmaUtil over the same "
                             "183 strip launches (profiles/colblk_counters_antenna21x21_9f.txt)")
+
+
+# the oracle's whole DBS run on the default workload (tests/golden/make_oracle_sequence.py: the C++
+# restatement of pngfopt.hpp to convergence on the host, start C^-1 by LAPACK)
+ORACLE_RUN = os.path.join(ROOT, "tests", "golden", "oracle_antenna21x21_9f.npz")
+
+
+def oracle_run_check(r):
+    """A run of the default workload against the oracle's recorded whole run: same accepted flips
+    at the same evaluations, and the relative difference of the final objectives."""
+    if not os.path.exists(ORACLE_RUN):
+        return None
+    z = np.load(ORACLE_RUN)
+    same = (int(z["flip_count"]) == r["flip_count"] and np.array_equal(z["acc_tiles"], r["acc_tiles"]) and
+            np.array_equal(z["acc_eval"], r["acc_eval"]))
+    return dict(same_accepts=bool(same), accepts=int(len(z["acc_tiles"])), flip_count=int(z["flip_count"]),
+                F_final=float(z["F_final"]), F_rel=abs(r["F_final"] - float(z["F_final"])) / float(z["F_final"]))
 
 
 def log(*a):
@@ -607,6 +625,8 @@ def main():
     if args.tiles or args.nfreq:
         wl_name = "antenna%dx%d_%df" % (wl["T"], wl["T"], len(wl["freqs"]))
     method = P.METHOD_LOWRANK if args.method == "lowrank" else P.METHOD_PLAIN
+    # the workload whose whole run the oracle recorded
+    default_run = wl_name == "antenna21x21_9f" and wl["max_evals"] == 0
 
     log("[bench] init %s method=%s" % (wl_name, args.method))
     opt, init_s = make(wl, method, args.max_deferred, P.SYM_AUTO if args.symmetry == "auto" else P.SYM_OFF,
@@ -717,6 +737,7 @@ def main():
                          flip_count=pr["flip_count"], accepts=pr["accepts"], F_final=pr["F_final"],
                          speculative_evals=pr["speculative_evals"], init_s=pinit,
                          same_accept_sequence=bool(np.array_equal(pr["acc_tiles"], last["acc_tiles"])),
+                         vs_oracle_run=oracle_run_check(pr) if default_run else None,
                          hbm_ceiling_s=ceil_s, hbm_ceiling_value=pr["flip_count"] / ceil_s,
                          kernels={k: dict(v, avg_ms=v["total_ms"] / v["launches"] if v["launches"] else 0.0,
                                           GBps=v["bytes"] / (v["total_ms"] * 1e-3) / 1e9 if v["total_ms"] else 0.0)
@@ -768,6 +789,7 @@ def main():
         "speedup_vs_plain_ceiling": (value / plain["hbm_ceiling_value"]) if plain else None,
         "accept_path_variants": variants,
         "F_final_vs_plain": (last["F_final"] / plain["F_final"]) if plain else None,
+        "vs_oracle_run": oracle_run_check(last) if default_run else None,
         "best_improvement_variant": best,
         "sweep": sweep,
         "time_to_solution_s": {"first_improvement": last["seconds"], "best_improvement": rb["seconds"],

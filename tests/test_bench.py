@@ -89,3 +89,19 @@ def test_roofline_of_an_hbm_bound_kernel():
 def test_no_kernel_timed():
     rec, allk = bench.roofline(FakeHandle(100, {}))
     assert rec is None and allk == {}
+
+
+def test_oracle_run_check():
+    """The bench line's comparison with the oracle's recorded whole run: the record itself passes
+    with no objective difference; a run with one accepted flip moved, or a different evaluation
+    count, does not."""
+    import numpy as np
+    z = np.load(bench.ORACLE_RUN)
+    r = dict(flip_count=int(z["flip_count"]), acc_tiles=z["acc_tiles"].copy(), acc_eval=z["acc_eval"].copy(),
+             F_final=float(z["F_final"]))
+    c = bench.oracle_run_check(r)
+    assert c["same_accepts"] and c["F_rel"] == 0.0 and c["accepts"] == len(z["acc_tiles"])
+    r2 = dict(r, acc_eval=r["acc_eval"] + np.eye(1, len(r["acc_eval"]), 5, dtype=r["acc_eval"].dtype)[0])
+    assert not bench.oracle_run_check(r2)["same_accepts"]
+    r3 = dict(r, flip_count=r["flip_count"] + 1)
+    assert not bench.oracle_run_check(r3)["same_accepts"]
