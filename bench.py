@@ -57,14 +57,25 @@ ORACLE_RUN = os.path.join(ROOT, "tests", "golden", "oracle_antenna21x21_9f.npz")
 
 def oracle_run_check(r):
     """A run of the default workload against the oracle's recorded whole run: same accepted flips
-    at the same evaluations, and the relative difference of the final objectives."""
+    at the same evaluations (else the index of the first accept where they part), and the
+    relative difference of the final objectives."""
     if not os.path.exists(ORACLE_RUN):
         return None
     z = np.load(ORACLE_RUN)
     same = (int(z["flip_count"]) == r["flip_count"] and np.array_equal(z["acc_tiles"], r["acc_tiles"]) and
             np.array_equal(z["acc_eval"], r["acc_eval"]))
-    return dict(same_accepts=bool(same), accepts=int(len(z["acc_tiles"])), flip_count=int(z["flip_count"]),
-                F_final=float(z["F_final"]), F_rel=abs(r["F_final"] - float(z["F_final"])) / float(z["F_final"]))
+    # the first accept (index into the oracle's list) where the two runs part, if they do
+    first = None
+    n = min(len(z["acc_tiles"]), len(r["acc_tiles"]))
+    for i in range(n):
+        if z["acc_tiles"][i] != r["acc_tiles"][i] or z["acc_eval"][i] != r["acc_eval"][i]:
+            first = i
+            break
+    if first is None and not same:
+        first = n
+    return dict(same_accepts=bool(same), first_difference=first, accepts=int(len(z["acc_tiles"])),
+                flip_count=int(z["flip_count"]), F_final=float(z["F_final"]),
+                F_rel=abs(r["F_final"] - float(z["F_final"])) / float(z["F_final"]))
 
 
 def log(*a):

@@ -100,8 +100,11 @@ def test_oracle_run_check():
     r = dict(flip_count=int(z["flip_count"]), acc_tiles=z["acc_tiles"].copy(), acc_eval=z["acc_eval"].copy(),
              F_final=float(z["F_final"]))
     c = bench.oracle_run_check(r)
-    assert c["same_accepts"] and c["F_rel"] == 0.0 and c["accepts"] == len(z["acc_tiles"])
+    assert c["same_accepts"] and c["first_difference"] is None and c["F_rel"] == 0.0
+    assert c["accepts"] == len(z["acc_tiles"])
     r2 = dict(r, acc_eval=r["acc_eval"] + np.eye(1, len(r["acc_eval"]), 5, dtype=r["acc_eval"].dtype)[0])
-    assert not bench.oracle_run_check(r2)["same_accepts"]
+    c2 = bench.oracle_run_check(r2)
+    assert not c2["same_accepts"] and c2["first_difference"] == 5
     r3 = dict(r, flip_count=r["flip_count"] + 1)
-    assert not bench.oracle_run_check(r3)["same_accepts"]
+    c3 = bench.oracle_run_check(r3)
+    assert not c3["same_accepts"] and c3["first_difference"] == len(z["acc_tiles"])
