@@ -49,6 +49,33 @@ def finish_init(opt, tm, T):
     opt.init_state()
 
 
+@pytest.mark.parametrize("layout,T,ne", [(P.LAYOUT_DISJOINT, 21, 1), (P.LAYOUT_SHARED, 11, 3)])
+def test_device_synth_g_matches_host(layout, T, ne):
+    """The synthetic G the device forms (pngf_synth_g, what the benchmark and the full-size tests
+    run on) against the host synthesiser's (pngf_synth_g_host, what the oracle's recorded runs
+    were computed from), at the default softening: the design block agrees to 4e-16 of its
+    largest entry."""
+    freqs = [26e9, 34e9]
+    N = P.layout_size(layout, T)
+    o = P.Optimizer(layout, T, num_freq=len(freqs), num_eobj=ne, method=P.METHOD_PLAIN)
+    try:
+        for f, hz in enumerate(freqs):
+            o.set_frequency(f, hz)
+            o.synth_g(f)
+        o.set_tile_map(initial_pattern(T, 0))
+        o.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+        o.init_state()
+        for f, hz in enumerate(freqs):
+            Gd = o.g(f)
+            Gh = P.synth_g_host(layout, T, ne, hz)[:N, :N]
+            d = rel(Gd, Gh)
+            print("[synth] layout %d T %d f %.0f GHz: device vs host %.3g, bitwise %s" % (
+                layout, T, hz / 1e9, d, np.array_equal(Gd, Gh)))
+            assert d < 4e-16
+    finally:
+        o.close()
+
+
 def test_load_g_files_match_set_g_and_oracle(tmp_path):
     """Shared layout, ne = 3 (two NF2FF rows), reference driver and objective: load_g from
     files equals set_g bitwise (state, every flip's score, the whole DBS run) and follows the
