@@ -539,6 +539,54 @@ def trace_summary(path):
         table(plain, "## PLAIN runs (two DBS runs: one for the rate, one event-timed)")
 
 
+def load_probe(nfreq=3):
+    """acmefdfd-layout G files (raw column-major complex128, M = N, 21x21 tiles: 1.79 GB each)
+    through pngf_load_g, the reader that replaces pngfopt.hpp:121-184: the files are written
+    from the host synthesiser to a temporary directory, then loaded into one handle, one
+    frequency after another, and the bytes per second of the loads are printed as one JSON line,
+    beside the rate of a plain host read of the same files into memory.  The files were just
+    written, so they come from the page cache: this is the loader's host-to-device rate, not the
+    disk's."""
+    import shutil
+    import tempfile
+    wl = WORKLOADS["antenna21x21_9f"]
+    T, layout, freqs = wl["T"], wl["layout"], wl["freqs"][:nfreq]
+    N = P.layout_size(layout, T)
+    tmp = tempfile.mkdtemp(prefix="pngf_g_")
+    try:
+        paths = []
+        for f, hz in enumerate(freqs):
+            G = P.synth_g_host(layout, T, 1, hz)
+            path = os.path.join(tmp, "Gmat_sub_%02d.bin" % (f + 1))
+            G.ravel(order="K").tofile(path)
+            paths.append(path)
+            del G
+        nbytes = sum(os.path.getsize(p) for p in paths)
+        t0 = time.perf_counter()
+        for p in paths:
+            with open(p, "rb") as fh:
+                np.fromfile(fh, dtype=np.complex128)
+        read_s = time.perf_counter() - t0
+        opt = P.Optimizer(layout, T, num_freq=len(freqs), method=P.METHOD_LOWRANK)
+        try:
+            for f, hz in enumerate(freqs):
+                opt.set_frequency(f, hz)
+            t0 = time.perf_counter()
+            for f, p in enumerate(paths):
+                opt.load_g(f, p)
+            load_s = time.perf_counter() - t0
+        finally:
+            opt.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {"metric": "G file load rate (acmefdfd layout, page-cached files, host to device)",
+           "value": nbytes / load_s / 1e9, "unit": "GB/s", "files": len(paths), "N": N, "bytes": nbytes,
+           "seconds": load_s, "host_read_GBps": nbytes / read_s / 1e9, "data": "synthetic"}
+    log("[load] %d files, %.2f GB: pngf_load_g %.3f s (%.1f GB/s); host read alone %.1f GB/s" % (
+        len(paths), nbytes / 1e9, load_s, out["value"], out["host_read_GBps"]))
+    print(json.dumps(out))
+
+
 def bench_multi(workers=4):
     """16 DBS instances of uneven size (11..25 tiles, 5 frequencies) on one GPU at once.
     Prints its own JSON line: total committed evaluations / wall seconds of the concurrent
@@ -604,6 +652,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="host worker threads of the multi16 workload")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="LOWRANK accept: row strip from the column strip when G is symmetric (auto)")
+    ap.add_argument("--load-probe", action="store_true",
+                    help="only time pngf_load_g on acmefdfd-layout G files (21x21 tiles, 3 frequencies)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the two-strip and non-reciprocal-G runs")
     args = ap.parse_args()
@@ -622,6 +672,9 @@ def main():
         return
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.load_probe:
+        load_probe()
         return
     if args.workload == "multi16":
         bench_multi(args.workers)
