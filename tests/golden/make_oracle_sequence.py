@@ -3,6 +3,9 @@
 * tests/golden/oracle_antenna21x21_9f.npz: the benchmark's workload (21x21 tiles, disjoint
   layout, N = 10590, 9 frequencies 26..34 GHz, default synthetic G, Bernoulli(0.5) start from
   mt19937(0), raster first improvement of mean |S11|^2 to convergence);
+* tests/golden/oracle_antenna21x21_9f_nonreciprocal.npz: the same workload on G perturbed
+  non-reciprocally at 1e-7 of its largest entry, the rounding level of acmefdfd's single-precision
+  Schur complement (apf_solver.hpp:188-198), so that W is not symmetric;
 * tests/golden/oracle_reference_default.npz: the reference program's own default configuration
   (main.cpp:116-117, 154-160, 181-241): 21x21 tiles on the shared Yee grid (N = 8064), 5
   frequencies 25..35 GHz, ne = 3, the directivity objective, MagicHat order with mirror-symmetric
@@ -20,9 +23,10 @@ the objective after each accept, the start and final objectives and the evaluati
 the default softening (cond(C) ~ 1e5-1e6) the accepted sequence depends on the rounding of the
 sums, so a fixture records what the reference-structured arithmetic gives.
 
-Run (8 host cores; the benchmark's workload takes about 45 minutes and 40 GB of host memory, the
+Run (8 host cores; each 21x21 x 9 workload takes about 45 minutes and 40 GB of host memory, the
 reference default about 10 minutes):
-    python tests/golden/make_oracle_sequence.py [antenna21x21_9f | reference_default]
+    python tests/golden/make_oracle_sequence.py [antenna21x21_9f | antenna21x21_9f_nonreciprocal |
+                                                 reference_default]
 """
 import gc
 import os
@@ -46,6 +50,12 @@ CASES = {
     "antenna21x21_9f": dict(T=21, layout=WORKLOADS["antenna21x21_9f"]["layout"],
                             freqs=WORKLOADS["antenna21x21_9f"]["freqs"], ne=1, max_flips=1,
                             objective=O.OBJ_S11, seed=0),
+    # the benchmark's workload on G perturbed non-reciprocally at 1e-7 of its largest entry
+    # (pngf_perturb_g / perturb_block, seed f + 1 at frequency f): the rounding level of acmefdfd's
+    # single-precision Schur complement, which makes W unsymmetric (two strips per accept)
+    "antenna21x21_9f_nonreciprocal": dict(T=21, layout=WORKLOADS["antenna21x21_9f"]["layout"],
+                                          freqs=WORKLOADS["antenna21x21_9f"]["freqs"], ne=1, max_flips=1,
+                                          objective=O.OBJ_S11, seed=0, perturb=1e-7),
     "reference_default": dict(T=21, layout=O.LAYOUT_SHARED, freqs=[25e9, 27.5e9, 30e9, 32.5e9, 35e9], ne=3,
                               max_flips=4, objective=O.OBJ_REF, seed=-514213645),
 }
@@ -61,6 +71,8 @@ def run(name):
     t0 = time.time()
     for f, hz in enumerate(c["freqs"]):
         G = P.synth_g_host(layout, T, ne, hz)
+        if c.get("perturb"):
+            G = P.perturb_block(G, O.layout_size(layout, T), c["perturb"], seed=f + 1)
         o = O.Optimizer(layout, T, G, ne=ne, max_flips=c["max_flips"])
         o.set_tile_map(tm)
         o.set_source_port(mid, mid, 1, 2)

@@ -89,36 +89,75 @@ def test_drift_at_benchmark_conditioning():
     assert lo["eobj"] <= 2.0 * pl["eobj"]
 
 
-GOLDEN_SEQ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_antenna21x21_9f.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# (fixture, G perturbation, method, symmetry mode, accepts that must match): None = the whole run.
+# At cond(C) ~ 1e5-1e6 which flip a run accepts can turn on the rounding of one sum; the counts
+# are below where each path was measured to part from the oracle's run on one MI355X.
+ORACLE_RUN_CASES = [
+    ("antenna21x21_9f", 0.0, P.METHOD_LOWRANK, P.SYM_AUTO, None),          # the headline path
+    ("antenna21x21_9f", 0.0, P.METHOD_LOWRANK, P.SYM_OFF, 90),             # both strips: parts at 106
+    ("antenna21x21_9f", 0.0, P.METHOD_PLAIN, P.SYM_AUTO, 150),             # parts at 160
+    ("antenna21x21_9f_nonreciprocal", 1e-7, P.METHOD_LOWRANK, P.SYM_AUTO, 110),  # both strips: parts at 124
+    ("antenna21x21_9f_nonreciprocal", 1e-7, P.METHOD_PLAIN, P.SYM_AUTO, None),
+]
 
 
 @pytest.mark.timeout(600)
-def test_benchmark_run_follows_oracle():
+@pytest.mark.parametrize("name,perturb,method,symmetry,nmatch", ORACLE_RUN_CASES)
+def test_benchmark_run_follows_oracle(name, perturb, method, symmetry, nmatch):
     """The benchmark's workload (21x21 tiles x 9 frequencies, default synthetic G, cond(C) ~
     1e5-1e6) against the oracle's whole run on it, recorded by tests/golden/make_oracle_sequence.py
     (the C++ restatement of pngfopt.hpp with C^-1 by LAPACK zgetrf + zgetri as linalg.hpp:34-85;
-    3527 evaluations, 183 accepts, 31 minutes on 8 host cores).  LOWRANK on the device-driven loop,
-    the headline's path, accepts the same flips at the same evaluations; the start objective agrees
-    to 1e-10 and every accepted objective to 1e-4 relative (measured: 2e-6 at the end)."""
-    gold = np.load(GOLDEN_SEQ)
+    3527 evaluations and 183 accepts, about 30 minutes on 8 host cores), and on G perturbed
+    non-reciprocally at 1e-7 of its largest entry (the rounding level of acmefdfd's
+    single-precision output; W is then not symmetric and LOWRANK forms both strips per accept,
+    the path a drop-in on acmefdfd's files takes; the oracle again takes 183 accepts).  The
+    headline path (LOWRANK, one strip) and PLAIN on the perturbed G accept the oracle's flips at
+    its evaluations over the whole run, every accepted objective to 1e-4 relative.  The other
+    paths follow the oracle's first nmatch accepts and converge to an objective within 1 % of
+    its.  In every case the start objective agrees to 1e-10."""
+    path = os.path.join(GOLDEN, "oracle_%s.npz" % name)
+    gold = np.load(path)
     wl = WORKLOADS["antenna21x21_9f"]
-    a = build(wl["T"], wl["freqs"], initial_pattern(wl["T"], 0))
+    T, freqs = wl["T"], wl["freqs"]
+    a = P.Optimizer(P.LAYOUT_DISJOINT, T, num_freq=len(freqs), method=method, symmetry=symmetry)
     try:
+        for f, hz in enumerate(freqs):
+            a.set_frequency(f, hz)
+            a.synth_g(f)
+            if perturb:
+                a.perturb_g(f, perturb, seed=f + 1)
+        a.set_tile_map(initial_pattern(T, 0))
+        a.set_source_port((T - 1) // 2, (T - 1) // 2, 1, 2)
+        a.init_state()
+        if method == P.METHOD_LOWRANK:
+            assert a.symmetry()[0] == (perturb == 0.0 and symmetry == P.SYM_AUTO)
         r = a.dbs_run()
     finally:
         a.close()
-    print("[oracle-seq] GPU %d evals %d accepts F %.10g; oracle %d evals %d accepts F %.10g" % (
-        r["flip_count"], r["accepts"], r["F_final"], int(gold["flip_count"]), len(gold["acc_tiles"]),
-        float(gold["F_final"])))
+    n = min(len(r["acc_tiles"]), len(gold["acc_tiles"]))
+    same = (r["acc_tiles"][:n] == gold["acc_tiles"][:n]) & (r["acc_eval"][:n] == gold["acc_eval"][:n])
+    first = int(np.argmin(same)) if not same.all() else n
+    print("[oracle-seq] %s method %d symmetry %d: GPU %d evals %d accepts F %.10g; oracle %d evals %d accepts "
+          "F %.10g; first %d accepts the oracle's" % (
+              name, method, symmetry, r["flip_count"], r["accepts"], r["F_final"], int(gold["flip_count"]),
+              len(gold["acc_tiles"]), float(gold["F_final"]), first))
     assert r["converged"]
     assert abs(r["F_start"] - float(gold["F_start"])) < 1e-10 * float(gold["F_start"])
-    assert r["flip_count"] == int(gold["flip_count"])
-    assert np.array_equal(r["acc_tiles"], gold["acc_tiles"])
-    assert np.array_equal(r["acc_eval"], gold["acc_eval"])
-    assert rel(r["acc_F"], gold["acc_F"]) < 1e-4
+    if nmatch is None:
+        assert r["flip_count"] == int(gold["flip_count"])
+        assert np.array_equal(r["acc_tiles"], gold["acc_tiles"])
+        assert np.array_equal(r["acc_eval"], gold["acc_eval"])
+        assert rel(r["acc_F"], gold["acc_F"]) < 1e-4
+    else:
+        assert first >= nmatch
+        assert rel(r["acc_F"][:nmatch], gold["acc_F"][:nmatch]) < 1e-4
+        assert abs(r["F_final"] - float(gold["F_final"])) < 1e-2 * float(gold["F_final"])
 
 
-GOLDEN_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_reference_default.npz")
+GOLDEN_REF = os.path.join(GOLDEN, "oracle_reference_default.npz")
 
 
 @pytest.mark.timeout(600)

@@ -236,9 +236,10 @@ def test_golden_fixture():
 
 
 @pytest.mark.parametrize("name,ntiles,raster", [("antenna21x21_9f", 21 * 21, True),
+                                                 ("antenna21x21_9f_nonreciprocal", 21 * 21, True),
                                                  ("reference_default", 21 * 21, False)])
 def test_whole_run_fixtures(name, ntiles, raster):
-    """The oracle's whole runs on the two full-size workloads (tests/golden/make_oracle_sequence.py)
+    """The oracle's whole runs on the full-size workloads (tests/golden/make_oracle_sequence.py)
     have the shape first-improvement DBS gives (main.cpp:338-459): accepts at strictly increasing
     evaluations, each strictly improving the objective (main.cpp:429).  The raster run ends after
     one full pass of the tiles without an improving flip; the MagicHat run after its last accept
