@@ -50,18 +50,20 @@ STRIP_COUNTED = dict(N=10590, freqs=9, variant=4, launches=HEADLINE_ACCEPTS, alg
                             "183 strip launches (profiles/colblk_counters_antenna21x21_9f.txt)")
 
 
-# the oracle's whole DBS run on the default workload (tests/golden/make_oracle_sequence.py: the C++
-# restatement of pngfopt.hpp to convergence on the host, start C^-1 by LAPACK)
+# the oracle's whole DBS runs on the default workload, and on its G perturbed non-reciprocally at
+# 1e-7 (tests/golden/make_oracle_sequence.py: the C++ restatement of pngfopt.hpp to convergence on
+# the host, start C^-1 by LAPACK)
 ORACLE_RUN = os.path.join(ROOT, "tests", "golden", "oracle_antenna21x21_9f.npz")
+ORACLE_RUN_NONRECIPROCAL = os.path.join(ROOT, "tests", "golden", "oracle_antenna21x21_9f_nonreciprocal.npz")
 
 
-def oracle_run_check(r):
+def oracle_run_check(r, path=ORACLE_RUN):
     """A run of the default workload against the oracle's recorded whole run: same accepted flips
     at the same evaluations (else the index of the first accept where they part), and the
     relative difference of the final objectives."""
-    if not os.path.exists(ORACLE_RUN):
+    if not os.path.exists(path):
         return None
-    z = np.load(ORACLE_RUN)
+    z = np.load(path)
     same = (int(z["flip_count"]) == r["flip_count"] and np.array_equal(z["acc_tiles"], r["acc_tiles"]) and
             np.array_equal(z["acc_eval"], r["acc_eval"]))
     # the first accept (index into the oracle's list) where the two runs part, if they do
@@ -152,7 +154,7 @@ def strip_stats(opt, peak=PEAK_FP64_TFLOPS):
     return out
 
 
-def variant(wl, name, symmetry, perturb, headline, precision=P.PREC_C128):
+def variant(wl, name, symmetry, perturb, headline, precision=P.PREC_C128, oracle_path=None):
     """The LOWRANK run from the same start on another accept path (or in complex64): one untimed
     warmup, one run for the rate, one profiled run for the strips' event times."""
     opt, init_s = make(wl, P.METHOD_LOWRANK, symmetry=symmetry, perturb=perturb, precision=precision)
@@ -173,7 +175,8 @@ def variant(wl, name, symmetry, perturb, headline, precision=P.PREC_C128):
     out = dict(value=r["flip_count"] / r["seconds"], seconds=r["seconds"], flip_count=r["flip_count"],
                accepts=r["accepts"], F_final=r["F_final"], symmetric_accept=used, rel_asym=asym,
                init_s=init_s, strips=strips, F_final_vs_headline=r["F_final"] / headline["F_final"],
-               same_accept_sequence=bool(np.array_equal(r["acc_tiles"], headline["acc_tiles"])))
+               same_accept_sequence=bool(np.array_equal(r["acc_tiles"], headline["acc_tiles"])),
+               vs_oracle_run=oracle_run_check(r, oracle_path) if oracle_path else None)
     if precision == P.PREC_C64:
         # the complex64 state's objective against a complex128 init_state of the design it reached
         fresh = P.Optimizer(wl["layout"], wl["T"], num_freq=len(wl["freqs"]), method=P.METHOD_LOWRANK)
@@ -766,8 +769,11 @@ def main():
     # taken by itself on a G perturbed at the single-precision level
     variants = None
     if not args.no_variants and method == P.METHOD_LOWRANK:
-        variants = {"symmetry_off": variant(wl, "symmetry off", P.SYM_OFF, 0.0, last),
-                    "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last),
+        orc = default_run
+        variants = {"symmetry_off": variant(wl, "symmetry off", P.SYM_OFF, 0.0, last,
+                                            oracle_path=ORACLE_RUN if orc else None),
+                    "nonreciprocal_1e-7": variant(wl, "G perturbed 1e-7", P.SYM_AUTO, 1e-7, last,
+                                                  oracle_path=ORACLE_RUN_NONRECIPROCAL if orc else None),
                     "nonreciprocal_1e-7_projected": variant(wl, "G perturbed 1e-7, projected", P.SYM_PROJECT,
                                                             1e-7, last),
                     "precision_c64": variant(wl, "complex64", P.SYM_AUTO, 0.0, last, P.PREC_C64)}
