@@ -6,6 +6,8 @@
 * tests/golden/oracle_antenna21x21_9f_nonreciprocal.npz: the same workload on G perturbed
   non-reciprocally at 1e-7 of its largest entry, the rounding level of acmefdfd's single-precision
   Schur complement (apf_solver.hpp:188-198), so that W is not symmetric;
+* tests/golden/oracle_multi16_T{15,21,25}_s0.npz: the seed-0 instances of the 16-instance
+  workload at 15x15, 21x21 and 25x25 tiles (disjoint layout, 5 frequencies 28..32 GHz);
 * tests/golden/oracle_reference_default.npz: the reference program's own default configuration
   (main.cpp:116-117, 154-160, 181-241): 21x21 tiles on the shared Yee grid (N = 8064), 5
   frequencies 25..35 GHz, ne = 3, the directivity objective, MagicHat order with mirror-symmetric
@@ -24,9 +26,8 @@ the default softening (cond(C) ~ 1e5-1e6) the accepted sequence depends on the r
 sums, so a fixture records what the reference-structured arithmetic gives.
 
 Run (8 host cores; each 21x21 x 9 workload takes about 45 minutes and 40 GB of host memory, the
-reference default about 10 minutes):
-    python tests/golden/make_oracle_sequence.py [antenna21x21_9f | antenna21x21_9f_nonreciprocal |
-                                                 reference_default]
+25x25 instance about 40 minutes and 45 GB, the reference default about 10 minutes):
+    python tests/golden/make_oracle_sequence.py [name ...]      (names: the keys of CASES)
 """
 import gc
 import os
@@ -42,7 +43,7 @@ sys.path.insert(0, ROOT)
 
 import oracle as O  # noqa: E402
 import pngf_amd as P  # noqa: E402
-from pngf_amd.problem import WORKLOADS, initial_pattern  # noqa: E402
+from pngf_amd.problem import MULTI16, WORKLOADS, initial_pattern  # noqa: E402
 
 DX, DY, DZ = 0.01 / 60, 0.01 / 60, 0.01 / 180
 
@@ -56,6 +57,14 @@ CASES = {
     "antenna21x21_9f_nonreciprocal": dict(T=21, layout=WORKLOADS["antenna21x21_9f"]["layout"],
                                           freqs=WORKLOADS["antenna21x21_9f"]["freqs"], ne=1, max_flips=1,
                                           objective=O.OBJ_S11, seed=0, perturb=1e-7),
+    # seed-0 instances of the 16-instance workload at 15x15, 21x21 and 25x25 tiles (5 frequencies
+    # 28..32 GHz; the 11x11 one the GPU test compares with a live oracle run)
+    "multi16_T15_s0": dict(T=15, layout=MULTI16["layout"], freqs=MULTI16["freqs"], ne=1, max_flips=1,
+                           objective=O.OBJ_S11, seed=0),
+    "multi16_T21_s0": dict(T=21, layout=MULTI16["layout"], freqs=MULTI16["freqs"], ne=1, max_flips=1,
+                           objective=O.OBJ_S11, seed=0),
+    "multi16_T25_s0": dict(T=25, layout=MULTI16["layout"], freqs=MULTI16["freqs"], ne=1, max_flips=1,
+                           objective=O.OBJ_S11, seed=0),
     "reference_default": dict(T=21, layout=O.LAYOUT_SHARED, freqs=[25e9, 27.5e9, 30e9, 32.5e9, 35e9], ne=3,
                               max_flips=4, objective=O.OBJ_REF, seed=-514213645),
 }

@@ -39,6 +39,13 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+def accepts_in_common(r, z):
+    """Leading accepts (tile and evaluation index) that a run shares with a recorded one."""
+    n = min(len(r["acc_tiles"]), len(z["acc_tiles"]))
+    same = (np.asarray(r["acc_tiles"][:n]) == z["acc_tiles"][:n]) & (np.asarray(r["acc_eval"][:n]) == z["acc_eval"][:n])
+    return int(np.argmin(same)) if not same.all() else n
+
+
 def build(T, freqs, tm, method=P.METHOD_LOWRANK, soft=0.0, max_deferred=0):
     o = P.Optimizer(P.LAYOUT_DISJOINT, T, num_freq=len(freqs), method=method, synth_soft=soft,
                     max_deferred=max_deferred)
@@ -137,9 +144,7 @@ def test_benchmark_run_follows_oracle(name, perturb, method, symmetry, nmatch):
         r = a.dbs_run()
     finally:
         a.close()
-    n = min(len(r["acc_tiles"]), len(gold["acc_tiles"]))
-    same = (r["acc_tiles"][:n] == gold["acc_tiles"][:n]) & (r["acc_eval"][:n] == gold["acc_eval"][:n])
-    first = int(np.argmin(same)) if not same.all() else n
+    first = accepts_in_common(r, gold)
     print("[oracle-seq] %s method %d symmetry %d: GPU %d evals %d accepts F %.10g; oracle %d evals %d accepts "
           "F %.10g; first %d accepts the oracle's" % (
               name, method, symmetry, r["flip_count"], r["accepts"], r["F_final"], int(gold["flip_count"]),
@@ -233,12 +238,22 @@ def _multi16_build():
     return opts
 
 
+# accepts the seed-0 instances of the 16-instance workload share with the oracle's whole runs
+# (fixtures): below where each was measured to part from it on one MI355X (default softening,
+# where a decision can turn on the rounding of one sum)
+MULTI16_ORACLE_PREFIX = {15: 75, 21: 80}
+
+
 @pytest.mark.timeout(1200)
 def test_multi16_concurrent_matches_alone_and_oracle():
     """The 16-instance workload through pngf_dbs_run_many (host workers over per-instance
     streams and graph-replayed device loops) gives every instance bitwise the run it has alone
     on a fresh handle; the 11x11 seed-0 instance's whole run (465 evaluations) is the oracle's:
-    same accepted flips at the same evaluations, objective to RTOL_ILL (default softening)."""
+    same accepted flips at the same evaluations, objective to RTOL_ILL (default softening).  The
+    seed-0 instances at 15x15 and 21x21 follow the oracle's recorded whole runs
+    (tests/golden/make_oracle_sequence.py) for their first accepts (measured: 85 of 99 and 93 of
+    147), with the objectives along them within 1e-3 (measured 1.4e-4); the start objectives
+    agree to 1e-10."""
     m = MULTI16
     # the oracle's run of the 11x11 seed-0 instance, on host threads meanwhile
     box = {}
@@ -285,6 +300,19 @@ def test_multi16_concurrent_matches_alone_and_oracle():
     assert np.array_equal(g["acc_tiles"], o["acc_tiles"])
     assert np.array_equal(g["acc_eval"], o["acc_eval"])
     assert rel(g["acc_F"], o["acc_F"]) < RTOL_ILL
+    # the seed-0 instances of the larger sizes against the oracle's recorded whole runs
+    for T, nmatch in MULTI16_ORACLE_PREFIX.items():
+        z = np.load(os.path.join(GOLDEN, "oracle_multi16_T%d_s0.npz" % T))
+        g = conc[keys.index((T, 0))]
+        first = accepts_in_common(g, z)
+        print("[multi16] %dx%d seed 0: %d evals %d accepts F %.10g; oracle %d evals %d accepts F %.10g; "
+              "first %d accepts the oracle's" % (T, T, g["flip_count"], g["accepts"], g["F_final"],
+                                                  int(z["flip_count"]), len(z["acc_tiles"]), float(z["F_final"]),
+                                                  first))
+        assert abs(g["F_start"] - float(z["F_start"])) < 1e-10 * float(z["F_start"])
+        assert first >= nmatch, (T, first)
+        # objectives along the shared accepts: the states drift apart by ~1e-4 at this conditioning
+        assert rel(g["acc_F"][:nmatch], z["acc_F"][:nmatch]) < 1e-3
 
 
 @pytest.mark.timeout(1200)

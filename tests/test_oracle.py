@@ -237,6 +237,8 @@ def test_golden_fixture():
 
 @pytest.mark.parametrize("name,ntiles,raster", [("antenna21x21_9f", 21 * 21, True),
                                                  ("antenna21x21_9f_nonreciprocal", 21 * 21, True),
+                                                 ("multi16_T15_s0", 15 * 15, True),
+                                                 ("multi16_T21_s0", 21 * 21, True),
                                                  ("reference_default", 21 * 21, False)])
 def test_whole_run_fixtures(name, ntiles, raster):
     """The oracle's whole runs on the full-size workloads (tests/golden/make_oracle_sequence.py)
