@@ -239,6 +239,7 @@ def test_golden_fixture():
                                                  ("antenna21x21_9f_nonreciprocal", 21 * 21, True),
                                                  ("multi16_T15_s0", 15 * 15, True),
                                                  ("multi16_T21_s0", 21 * 21, True),
+                                                 ("multi16_T25_s0", 25 * 25, True),
                                                  ("reference_default", 21 * 21, False)])
 def test_whole_run_fixtures(name, ntiles, raster):
     """The oracle's whole runs on the full-size workloads (tests/golden/make_oracle_sequence.py)
