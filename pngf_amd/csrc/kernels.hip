@@ -253,7 +253,8 @@ __device__ int lds_gj(Z* a, int lda, int k, int ncol, int* perm) {
 
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
 // gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
-// per-block slot of a global scratch buffer instead.
+// per-block slot of a global scratch buffer instead, except in the LOWRANK scoring kernel,
+// whose k x (k+1) system alone (149 KB at k = 96) stays in LDS where the device allows it.
 template <class Z>
 static size_t lds_matrix_bytes(int gmax, const Z* gscratch) {
     return gscratch ? 0 : (size_t)gmax * (2 * gmax + 1) * sizeof(Z);
@@ -340,7 +341,7 @@ __device__ void lr_eval_one(const Dims& d, const DevVecsT<Z>& v, const GroupTab&
     }
     __syncthreads();
     int info;
-    if (!gscratch) {
+    if (!gscratch && gmax <= kLdsGmax) {
         // Gauss-Jordan in LDS; y_c then goes to column k of storage row c (after a barrier, so
         // that no thread still reads the eliminated matrix)
         info = lds_gj(a, lda, k, k + 1, piv);
@@ -597,6 +598,16 @@ __global__ void __launch_bounds__(64) lr_eval_wave_kernel(Dims d, DevVecsT<Z> v,
     if (lane == 0 && out_info) out_info[b * d.nf + f] = info;
 }
 
+// Dynamic LDS a workgroup may take on the current device, less what kernels declare statically
+// besides it (a few scalars), from the device attribute; 0 if the query fails.
+static size_t lds_per_workgroup() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 0;
+    return v > 256 ? (size_t)v - 256 : 0;
+}
+
 template <class Z>
 void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, GroupTab gt, const int32_t* tile,
                     const int8_t* P, const int32_t* owners, const int32_t* batch, int nb, zd* out_eobj,
@@ -621,7 +632,21 @@ void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, GroupTa
         return;
     }
     int nth = d.gmax <= 32 ? 128 : 256;
-    size_t sm = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(Z) + lds_matrix_bytes(d.gmax, gscratch);
+    const size_t small = (size_t)4 * d.gmax * sizeof(int) + (size_t)6 * d.gmax * sizeof(Z);
+    size_t sm = small + lds_matrix_bytes(d.gmax, gscratch);
+    if (gscratch) {
+        // larger flips (k up to 96: the reference driver's 2- and 4-tile flips): the k x (k+1)
+        // system still goes to LDS, for the LU with partial pivoting, when the device's per-
+        // workgroup LDS holds it (96 x 97 complex128 entries: 149 KB), instead of the global slot
+        // whose every elimination step waits on the L2
+        const size_t need = small + (size_t)d.gmax * (d.gmax + 1) * sizeof(Z);
+        if (need <= lds_per_workgroup()) {
+            (void)hipFuncSetAttribute((const void*)lr_eval_kernel<Z>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)need);
+            gscratch = nullptr;
+            sm = need;
+        }
+    }
     hipLaunchKernelGGL(lr_eval_kernel<Z>, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
                        out_info, gscratch, nb_dev);
 }
