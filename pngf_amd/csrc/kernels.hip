@@ -1328,28 +1328,57 @@ __device__ __forceinline__ void skinny_mfma(Dims d, DevTabs tb, const int32_t* _
             cur ^= 1;
         }
     }
-#pragma unroll
-    for (int r = 0; r < RT; r++)
+    // Epilogue: out = W0 entry - acc.  The t entries of this lane's columns are read once, and
+    // the W0 entries of a group of EG column tiles are all loaded before any of their stores:
+    // the compiler cannot tell that a store to A or Bt leaves t and W0 alone, so in program
+    // order load-store-load-store every element waited for two dependent global loads.
+    // (unconditional loads of in-range entries, so that they issue together; the change set
+    // array always holds at least one entry)
+    int tcol[CT];
 #pragma unroll
     for (int c = 0; c < CT; c++) {
-        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
-        if (cc >= k) continue;
+        const int cc = max(0, min((c * 16 + lr) >> 1, k - 1));
+        tcol[c] = MODE != kSkinnyVC ? t[cc] : 0;
+    }
+    constexpr int EG = CT < 3 ? CT : 3;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = n0 + 16 * r + lq + 4 * v;
-            if (row >= d.N) continue;
-            const double a = acc[r][c][v];
-            // global-space pointers: the tables' entries would otherwise be flat accesses
-            if (MODE == kSkinnyVC) {
-                ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * k + cc) * 2 + q] = a;
-            } else if (!ROW) {
-                gdbl* w = (gdbl*)(W0 + (size_t)row * d.ldN + t[cc]);
-                ((gdbl_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
-            } else {
-                gdbl* w = (gdbl*)(W0 + (size_t)t[cc] * d.ldN + row);
-                ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
-            }
-        }
+    for (int c0 = 0; c0 < CT; c0 += EG) {
+        double wv[RT][EG][4];
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int e = 0; e < EG; e++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int c = c0 + e;
+                    const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+                    const int row = n0 + 16 * r + lq + 4 * v;
+                    wv[r][e][v] = 0.0;
+                    if (MODE != kSkinnyVC && cc < k && row < d.N) {
+                        // global-space pointers: the tables' entries would otherwise be flat accesses
+                        gdbl* w = ROW ? (gdbl*)(W0 + (size_t)tcol[c] * d.ldN + row)
+                                      : (gdbl*)(W0 + (size_t)row * d.ldN + tcol[c]);
+                        wv[r][e][v] = w[q];
+                    }
+                }
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int e = 0; e < EG; e++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int c = c0 + e;
+                    const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+                    const int row = n0 + 16 * r + lq + 4 * v;
+                    if (cc >= k || row >= d.N) continue;
+                    const double a = acc[r][c][v];
+                    if (MODE == kSkinnyVC)
+                        ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * k + cc) * 2 + q] = a;
+                    else if (!ROW)
+                        ((gdbl_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = wv[r][e][v] - a;
+                    else
+                        ((gdbl_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = wv[r][e][v] - a;
+                }
     }
 }
 
@@ -1950,25 +1979,46 @@ __global__ void __launch_bounds__(WAVES * 64) lr_skinny_mfma32_kernel(Dims d, De
             cur ^= 1;
         }
     }
+    // epilogue as in the FP64 kernel: t read once, a group's W0 entries loaded before its stores
+    int tcol[CT];
 #pragma unroll
-    for (int r = 0; r < RT; r++)
+    for (int c = 0; c < CT; c++) tcol[c] = t[max(0, min((c * 16 + lr) >> 1, k - 1))];
+    constexpr int EG = CT < 3 ? CT : 3;
 #pragma unroll
-    for (int c = 0; c < CT; c++) {
-        const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
-        if (cc >= k) continue;
+    for (int c0 = 0; c0 < CT; c0 += EG) {
+        float wv[RT][EG][4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = n0 + 16 * r + 4 * lq + v;
-            if (row >= d.N) continue;
-            const float a = acc[r][c][v];
-            if (!ROW) {
-                gflt* w = (gflt*)(W0 + (size_t)row * d.ldN + t[cc]);
-                ((gflt_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = w[q] - a;
-            } else {
-                gflt* w = (gflt*)(W0 + (size_t)t[cc] * d.ldN + row);
-                ((gflt_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = w[q] - a;
-            }
-        }
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int e = 0; e < EG; e++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int c = c0 + e;
+                    const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+                    const int row = n0 + 16 * r + 4 * lq + v;
+                    wv[r][e][v] = 0.f;
+                    if (cc < k && row < d.N) {
+                        gflt* w = ROW ? (gflt*)(W0 + (size_t)tcol[c] * d.ldN + row)
+                                      : (gflt*)(W0 + (size_t)row * d.ldN + tcol[c]);
+                        wv[r][e][v] = w[q];
+                    }
+                }
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int e = 0; e < EG; e++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int c = c0 + e;
+                    const int col = c * 16 + lr, cc = col >> 1, q = col & 1;
+                    const int row = n0 + 16 * r + 4 * lq + v;
+                    if (cc >= k || row >= d.N) continue;
+                    const float a = acc[r][c][v];
+                    if (!ROW)
+                        ((gflt_w*)A)[((size_t)row * d.Kmax + K + cc) * 2 + q] = wv[r][e][v] - a;
+                    else
+                        ((gflt_w*)Bt)[((size_t)f * bt_stride + (size_t)row * d.gmax + cc) * 2 + q] = wv[r][e][v] - a;
+                }
     }
 }
 
