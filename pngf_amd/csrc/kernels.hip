@@ -133,6 +133,73 @@ __device__ void lu_inverse_cols(const Z* a, int lda, int k, const int* piv, Z* X
     __syncthreads();
 }
 
+// The same inverse from LU factors held in LDS, for k <= 96, written to X (row-major, k x k):
+// every entry goes through the operations of lu_inverse_cols in the same order, but the rows of
+// column c are split between threads c and c + 96, which keep them in registers (48 each); at
+// each elimination step the owner of row j publishes its entry through LDS (rowbuf, two
+// alternating rows of 96, so one barrier per step) and every thread updates its own rows.
+// All threads of the block (256) call it.
+template <class Z>
+__device__ void lu_inverse_rows(const Z* a, int lda, int k, const int* piv, Z* X, Z* rowbuf) {
+    constexpr int RH = 48, NC = 96;
+    const int tid = threadIdx.x;
+    const int c = tid % NC, r0 = (tid / NC) * RH;
+    const bool on = tid < 2 * NC && c < k;
+    // the 1 of e_c after the row interchanges piv[0..k-1]
+    int pos = c;
+    for (int j = 0; j < k; j++) {
+        const int p = piv[j];
+        if (p != j) {
+            if (pos == j) pos = p;
+            else if (pos == p) pos = j;
+        }
+    }
+    Z xr[RH];
+#pragma unroll
+    for (int u = 0; u < RH; u++) xr[u] = zc<Z>(r0 + u == pos ? 1 : 0, 0);
+    int buf = 0;
+    // unit lower triangle, j increasing
+    for (int j = 0; j < k; j++) {
+        if (on)
+#pragma unroll
+            for (int u = 0; u < RH; u++)
+                if (r0 + u == j) rowbuf[buf * NC + c] = xr[u];
+        __syncthreads();
+        if (on) {
+            const Z xj = rowbuf[buf * NC + c];
+#pragma unroll
+            for (int u = 0; u < RH; u++) {
+                const int i = r0 + u;
+                if (i > j && i < k) xr[u] = zfms(a[i * lda + j], xj, xr[u]);
+            }
+        }
+        buf ^= 1;
+    }
+    // upper triangle, j decreasing
+    for (int j = k - 1; j >= 0; j--) {
+        if (on)
+#pragma unroll
+            for (int u = 0; u < RH; u++)
+                if (r0 + u == j) {
+                    xr[u] = zdiv(xr[u], a[j * lda + j]);
+                    rowbuf[buf * NC + c] = xr[u];
+                }
+        __syncthreads();
+        if (on) {
+            const Z xj = rowbuf[buf * NC + c];
+#pragma unroll
+            for (int u = 0; u < RH; u++)
+                if (r0 + u < j) xr[u] = zfms(a[(r0 + u) * lda + j], xj, xr[u]);
+        }
+        buf ^= 1;
+    }
+    if (on)
+#pragma unroll
+        for (int u = 0; u < RH; u++)
+            if (r0 + u < k) X[(r0 + u) * k + c] = xr[u];
+    __syncthreads();
+}
+
 // Entry (r, c) of the maintained diagonal block D (row-major, leading dimension gmax).  For
 // symmetric W the block is read as (D + D^T) / 2, which removes the rounding asymmetry that the
 // eager block updates accumulate (a symmetric-representation Woodbury update fed an asymmetric
@@ -253,8 +320,9 @@ __device__ int lds_gj(Z* a, int lda, int k, int ncol, int* perm) {
 
 // The k x (k+1) capacitance | rhs matrix and the k x k inverse live in LDS when
 // gmax <= kLdsGmax (33 KB); larger flip sets (2- and 4-tile flips, k <= 96) use a
-// per-block slot of a global scratch buffer instead, except in the LOWRANK scoring kernel,
-// whose k x (k+1) system alone (149 KB at k = 96) stays in LDS where the device allows it.
+// per-block slot of a global scratch buffer instead, except in the LOWRANK scoring and accept
+// kernels, whose k x (k+1) system alone (149 KB at k = 96) stays in LDS where the device allows
+// it (the accept then builds the inverse in registers and writes it to the global slot).
 template <class Z>
 static size_t lds_matrix_bytes(int gmax, const Z* gscratch) {
     return gscratch ? 0 : (size_t)gmax * (2 * gmax + 1) * sizeof(Z);
@@ -924,10 +992,10 @@ void launch_objective(hipStream_t st, const ObjParams& op, const FreqConst* fc, 
 //   [2k*k+2k, +ne)      newEobj
 //   [2k*k+2k+4, +ne*k)  Rt = R[:, t] (snapshot)
 template <class Z>
-__global__ void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __restrict__ tt,
+__global__ __launch_bounds__(256) void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __restrict__ tt,
                                const int32_t* __restrict__ ss, const int32_t* __restrict__ lpp, int k, Z* acc,
                                int acc_stride, int32_t* info_out, Z* gscratch, const int32_t* dyn,
-                               const int32_t* g_dev) {
+                               const int32_t* g_dev, int lds_big) {
     typedef typename zreal<Z>::type R;
     extern __shared__ char smem[];
     if (dyn) {
@@ -940,9 +1008,11 @@ __global__ void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __re
     int* piv = (int*)smem;
     Z* dpe = (Z*)(piv + 100);
     Z* tmp = dpe + 96;
-    // the matrix: LDS for k <= 32, else this frequency's global slot (the body is expanded for
-    // each, so that the LDS case compiles to LDS instructions)
-    auto rest = [&](Z* a) __attribute__((always_inline)) {
+    // the matrix: LDS for k <= 32, else this frequency's global slot, or, with lds_big (k <= 96
+    // where the workgroup's LDS holds k x (k+1) entries), LDS for the LU with the inverse built
+    // in registers and written to the global slot (the body is expanded for each, so that the
+    // LDS cases compile to LDS instructions)
+    auto rest = [&](Z* a, Z* Xg, Z* rowbuf) __attribute__((always_inline)) {
     // k <= 32 (LDS): Gauss-Jordan on [cap | rhs | I]; larger k: LU, back substitution, inverse
     const bool gj = gscratch == nullptr;
     const int lda = gj ? 2 * k + 1 : k + 1;
@@ -1017,8 +1087,11 @@ __global__ void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __re
         }
     }
     // cap^-1: X[r][c] = a[perm r][k+1+c] / d_r after Gauss-Jordan, else from the LU factors
-    Z* X = a + k * lda;
-    if (!gj) lu_inverse_cols(a, lda, k, piv, X, k);
+    Z* X = Xg ? Xg : a + k * lda;
+    if (!gj) {
+        if (rowbuf) lu_inverse_rows(a, lda, k, piv, X, rowbuf);
+        else lu_inverse_cols(a, lda, k, piv, X, k);
+    }
     auto xinv = [&](int r, int c) {
         return gj ? zdiv(a[piv[r] * lda + k + 1 + c], a[piv[r] * lda + r]) : X[r * k + c];
     };
@@ -1037,9 +1110,11 @@ __global__ void lr_prep_kernel(Dims d, DevVecsT<Z> v, int g, const int32_t* __re
     if (tid == 0 && info_out) info_out[f] = info;
     };
     if (!gscratch)
-        rest(tmp + 96);
+        rest(tmp + 96, nullptr, nullptr);
+    else if (lds_big)
+        rest(tmp + 96, gscratch + (size_t)f * gmax * (2 * gmax + 1), tmp + 96 + (size_t)gmax * (gmax + 1));
     else
-        rest(gscratch + (size_t)f * gmax * (2 * gmax + 1));
+        rest(gscratch + (size_t)f * gmax * (2 * gmax + 1), nullptr, nullptr);
 }
 
 template <class Z>
@@ -1047,8 +1122,19 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, 
                            const int32_t* s, const int32_t* lp, int k, Z* acc, int acc_stride,
                            int32_t* info, Z* gscratch, const int32_t* dyn, const int32_t* g_dev) {
     size_t sm = 100 * sizeof(int) + 192 * sizeof(Z) + lds_matrix_bytes(d.gmax, gscratch);
+    int lds_big = 0;
+    if (gscratch && d.gmax <= 96) {
+        // the k x (k+1) LU and two rows of the inverse in LDS when the workgroup may take them
+        const size_t need = sm + (size_t)d.gmax * (d.gmax + 1) * sizeof(Z) + 2 * 96 * sizeof(Z);
+        if (need <= lds_per_workgroup()) {
+            (void)hipFuncSetAttribute((const void*)lr_prep_kernel<Z>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)need);
+            sm = need;
+            lds_big = 1;
+        }
+    }
     hipLaunchKernelGGL(lr_prep_kernel<Z>, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
-                       acc_stride, info, gscratch, dyn, g_dev);
+                       acc_stride, info, gscratch, dyn, g_dev, lds_big);
 }
 
 // M[kk][n]: stored, or from Lam and A (see DevTabs); the sum runs in the order the accept's
