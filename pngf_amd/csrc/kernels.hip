@@ -16,6 +16,11 @@ namespace pngf {
 __device__ inline unsigned long long rbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline unsigned long long rbits(float x) { return (unsigned long long)__float_as_uint(x); }
 
+// Wave-wide maximum and minimum of the ROCm device library (the HIP headers declare the 64-bit
+// one only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES); every lane gets the result.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
+
 // ------------------------------------------------------------------ helpers --
 
 // Components of flip group g whose metal state changes, ascending, with sign
@@ -55,39 +60,67 @@ __device__ int lds_lu(Z* a, int lda, int k, int ncol, int* piv) {
     typedef typename zreal<Z>::type R;
     const int tid = threadIdx.x, nth = blockDim.x;
     __shared__ int s_info;
+    __shared__ R s_pv[6];  // pivot, old diagonal entry, reciprocal of the pivot
     if (tid == 0) s_info = 0;
     for (int j = 0; j < k; j++) {
         if (tid < 64) {
+            // first row of max |Re|+|Im|: each lane's first maximum over its rows, then the
+            // largest value (|.| >= 0 orders like its bit pattern; NaN and empty lanes: key 0)
+            // and the smallest row holding it, as two wave reductions
             R best = -1; int bi = 0x7fffffff;
             for (int i = j + tid; i < k; i += 64) {
                 R v = zabs1(a[i * lda + j]);
                 if (v > best) { best = v; bi = i; }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                R ov = __shfl_xor(best, off, 64);
-                int oi = __shfl_xor(bi, off, 64);
-                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            const unsigned long long key = best >= 0 ? rbits(best) + 1ull : 0ull;
+            const unsigned long long mk = __ockl_wfred_max_u64(key);
+            // (a column of NaN only: no lane holds a row, and row j is taken, as izamax does)
+            const unsigned mp = __ockl_wfred_min_u32(key == mk ? (unsigned)bi : 0xffffffffu);
+            const int p = mp < (unsigned)k ? (int)mp : j;
+            if (tid == 0) {
+                piv[j] = p;
+                const Z pv = a[p * lda + j], ajj = a[j * lda + j];
+                const Z r = (pv.x == 0 && pv.y == 0) ? pv : zrecip(pv);
+                s_pv[0] = pv.x; s_pv[1] = pv.y; s_pv[2] = ajj.x; s_pv[3] = ajj.y; s_pv[4] = r.x; s_pv[5] = r.y;
             }
-            if (tid == 0) piv[j] = bi;
         }
         __syncthreads();
+        // the row interchange and the multipliers in one phase: column j is left to the
+        // multiplier loop (row p's entry there is the old diagonal one, read before the barrier)
         const int p = piv[j];
+        const Z pv = zc<Z>(s_pv[0], s_pv[1]), ajj = zc<Z>(s_pv[2], s_pv[3]), r = zc<Z>(s_pv[4], s_pv[5]);
+        const bool zero = pv.x == 0 && pv.y == 0;
         if (p != j)
-            for (int c = tid; c < ncol; c += nth) { Z x = a[j * lda + c]; a[j * lda + c] = a[p * lda + c]; a[p * lda + c] = x; }
-        __syncthreads();
-        const Z pv = a[j * lda + j];
-        if (pv.x == 0.0 && pv.y == 0.0) {
+            for (int c = tid; c < ncol; c += nth)
+                if (c != j || zero) { Z x = a[j * lda + c]; a[j * lda + c] = a[p * lda + c]; a[p * lda + c] = x; }
+        if (zero) {
             if (tid == 0 && s_info == 0) s_info = j + 1;
             __syncthreads();
             continue;
         }
-        const Z r = zrecip(pv);
-        for (int i = j + 1 + tid; i < k; i += nth) a[i * lda + j] = zmul(a[i * lda + j], r);
+        for (int i = j + 1 + tid; i < k; i += nth) a[i * lda + j] = zmul(i == p ? ajj : a[i * lda + j], r);
+        if (tid == 0) a[j * lda + j] = pv;
         __syncthreads();
-        const int w = ncol - j - 1, h = k - j - 1;
-        for (int e = tid; e < w * h; e += nth) {
-            const int i = j + 1 + e / w, c = j + 1 + e % w;
-            a[i * lda + c] = zfms(a[i * lda + j], a[j * lda + c], a[i * lda + c]);
+        // trailing update: lanes take columns, waves take rows (every wave reads the same
+        // multiplier, a broadcast), four rows' loads issued before their stores
+        {
+            const int lane = tid & 63, nw = nth >> 6;
+            const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+            for (int c = j + 1 + lane; c < ncol; c += 64) {
+                const Z u = a[j * lda + c];
+                int i = j + 1 + wv;
+                for (; i + 3 * nw < k; i += 4 * nw) {
+                    Z l[4], x[4];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        l[m] = a[(i + m * nw) * lda + j];
+                        x[m] = a[(i + m * nw) * lda + c];
+                    }
+#pragma unroll
+                    for (int m = 0; m < 4; m++) a[(i + m * nw) * lda + c] = zfms(l[m], u, x[m]);
+                }
+                for (; i < k; i += nw) a[i * lda + c] = zfms(a[i * lda + j], u, a[i * lda + c]);
+            }
         }
         __syncthreads();
     }
@@ -134,69 +167,111 @@ __device__ void lu_inverse_cols(const Z* a, int lda, int k, const int* piv, Z* X
 }
 
 // The same inverse from LU factors held in LDS, for k <= 96, written to X (row-major, k x k):
-// every entry goes through the operations of lu_inverse_cols in the same order, but the rows of
-// column c are split between threads c and c + 96, which keep them in registers (48 each); at
-// each elimination step the owner of row j publishes its entry through LDS (rowbuf, two
-// alternating rows of 96, so one barrier per step) and every thread updates its own rows.
-// All threads of the block (256) call it.
+// every entry goes through the operations of lu_inverse_cols in the same order.  The 256
+// threads form a 16 x 16 grid; thread (tr, tc) keeps the 6 x 6 entries of rows tr + 16 m and
+// columns tc + 16 n in registers (rows interleaved, so that every thread has work while the
+// active rows shrink).  At each elimination step the 16 owners of row j publish it through LDS
+// (rowbuf, two alternating rows of 96, so one barrier per step); every thread then reads its 6
+// multipliers and 6 entries of row j and updates its tile: 12 LDS reads per 36 updates.
 template <class Z>
 __device__ void lu_inverse_rows(const Z* a, int lda, int k, const int* piv, Z* X, Z* rowbuf) {
-    constexpr int RH = 48, NC = 96;
-    const int tid = threadIdx.x;
-    const int c = tid % NC, r0 = (tid / NC) * RH;
-    const bool on = tid < 2 * NC && c < k;
-    // the 1 of e_c after the row interchanges piv[0..k-1]
-    int pos = c;
+    constexpr int T = 6, G = 16, NC = 96;
+    const int tc = threadIdx.x & (G - 1), tr = threadIdx.x / G;
+    // the 1 of e_c, for each of this thread's columns, after the row interchanges piv[0..k-1]
+    int pos[T];
+#pragma unroll
+    for (int n = 0; n < T; n++) pos[n] = tc + G * n;
     for (int j = 0; j < k; j++) {
         const int p = piv[j];
         if (p != j) {
-            if (pos == j) pos = p;
-            else if (pos == p) pos = j;
+#pragma unroll
+            for (int n = 0; n < T; n++) {
+                if (pos[n] == j) pos[n] = p;
+                else if (pos[n] == p) pos[n] = j;
+            }
         }
     }
-    Z xr[RH];
+    Z x[T][T];
 #pragma unroll
-    for (int u = 0; u < RH; u++) xr[u] = zc<Z>(r0 + u == pos ? 1 : 0, 0);
+    for (int m = 0; m < T; m++)
+#pragma unroll
+        for (int n = 0; n < T; n++) x[m][n] = zc<Z>(tr + G * m == pos[n] ? 1 : 0, 0);
     int buf = 0;
-    // unit lower triangle, j increasing
+    // unit lower triangle, j increasing: rows i > j
     for (int j = 0; j < k; j++) {
-        if (on)
+        const int mj = j / G;
+        if (tr == j - G * mj)
 #pragma unroll
-            for (int u = 0; u < RH; u++)
-                if (r0 + u == j) rowbuf[buf * NC + c] = xr[u];
+            for (int m = 0; m < T; m++)
+                if (m == mj)
+#pragma unroll
+                    for (int n = 0; n < T; n++) rowbuf[buf * NC + tc + G * n] = x[m][n];
         __syncthreads();
-        if (on) {
-            const Z xj = rowbuf[buf * NC + c];
+        Z xj[T], l[T];
 #pragma unroll
-            for (int u = 0; u < RH; u++) {
-                const int i = r0 + u;
-                if (i > j && i < k) xr[u] = zfms(a[i * lda + j], xj, xr[u]);
+        for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
+#pragma unroll
+        for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
+        // slots below row j's are updated whole (rows past k hold values that are never
+        // stored); row j's slot only below j
+#pragma unroll
+        for (int m = 0; m < T; m++) {
+            if (m > mj && G * m < k) {
+#pragma unroll
+                for (int n = 0; n < T; n++) x[m][n] = zfms(l[m], xj[n], x[m][n]);
+            } else if (m == mj) {
+                const bool below = tr + G * m > j;
+#pragma unroll
+                for (int n = 0; n < T; n++) {
+                    const Z nv = zfms(l[m], xj[n], x[m][n]);
+                    if (below) x[m][n] = nv;
+                }
             }
         }
         buf ^= 1;
     }
-    // upper triangle, j decreasing
+    // upper triangle, j decreasing: x_j /= u_jj, then rows i < j
     for (int j = k - 1; j >= 0; j--) {
-        if (on)
+        const int mj = j / G;
+        if (tr == j - G * mj) {
+            const Z d = a[j * lda + j];
 #pragma unroll
-            for (int u = 0; u < RH; u++)
-                if (r0 + u == j) {
-                    xr[u] = zdiv(xr[u], a[j * lda + j]);
-                    rowbuf[buf * NC + c] = xr[u];
-                }
+            for (int m = 0; m < T; m++)
+                if (m == mj)
+#pragma unroll
+                    for (int n = 0; n < T; n++) {
+                        x[m][n] = zdiv(x[m][n], d);
+                        rowbuf[buf * NC + tc + G * n] = x[m][n];
+                    }
+        }
         __syncthreads();
-        if (on) {
-            const Z xj = rowbuf[buf * NC + c];
+        Z xj[T], l[T];
 #pragma unroll
-            for (int u = 0; u < RH; u++)
-                if (r0 + u < j) xr[u] = zfms(a[(r0 + u) * lda + j], xj, xr[u]);
+        for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
+#pragma unroll
+        for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
+        // slots above row j's are updated whole, row j's slot only above j
+#pragma unroll
+        for (int m = 0; m < T; m++) {
+            if (m < mj) {
+#pragma unroll
+                for (int n = 0; n < T; n++) x[m][n] = zfms(l[m], xj[n], x[m][n]);
+            } else if (m == mj) {
+                const bool above = tr + G * m < j;
+#pragma unroll
+                for (int n = 0; n < T; n++) {
+                    const Z nv = zfms(l[m], xj[n], x[m][n]);
+                    if (above) x[m][n] = nv;
+                }
+            }
         }
         buf ^= 1;
     }
-    if (on)
 #pragma unroll
-        for (int u = 0; u < RH; u++)
-            if (r0 + u < k) X[(r0 + u) * k + c] = xr[u];
+    for (int m = 0; m < T; m++)
+#pragma unroll
+        for (int n = 0; n < T; n++)
+            if (tr + G * m < k && tc + G * n < k) X[(tr + G * m) * k + tc + G * n] = x[m][n];
     __syncthreads();
 }
 
@@ -252,11 +327,6 @@ __device__ int change_set_wave(const GroupTab& gt, int g, const int32_t* tile, c
     }
     return k;
 }
-
-// Wave-wide maximum and minimum of the ROCm device library (the HIP headers declare the 64-bit
-// one only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES); every lane gets the result.
-extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
-extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
 
 // Gauss-Jordan elimination with partial pivoting of the k x k leading block (k <= 32) of the
 // LDS matrix a (leading dimension lda), applying the row operations to columns k..ncol-1.
@@ -713,6 +783,9 @@ void launch_lr_eval(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, GroupTa
                                       (int)need);
             gscratch = nullptr;
             sm = need;
+            // one workgroup per CU (the LDS it takes): 1024 threads, four waves per SIMD to hide
+            // the latencies of the elimination steps (each thread's sums run as with 256)
+            nth = 1024;
         }
     }
     hipLaunchKernelGGL(lr_eval_kernel<Z>, grid, dim3(nth), sm, st, d, v, gt, tile, P, owners, batch, out_eobj,
