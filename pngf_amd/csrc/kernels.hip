@@ -2344,13 +2344,72 @@ __global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabsT<Z> tb, 
     }
 }
 
+// The same update for groups of up to 96 components: thread (ta, tb) of a 16 x 16 grid keeps
+// the 6 x 6 entries of rows ta + 16 i and columns tb + 16 j in registers, so that each step c
+// reads 6 entries of A_new and 6 of M_new from LDS for 36 updates (the kernel above reads two per
+// update).  Every entry's sum runs over c in the same order, from zero, as above.
+template <class Z>
+__global__ void __launch_bounds__(256) lr_blocks_tile_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt,
+                                                             int k, int K) {
+    constexpr int T = 6, G = 16;
+    __shared__ Z Ac[16][97];  // Ac[c][a] = A_new[comp a][c0 + c]
+    __shared__ Z Mc[16][97];  // Mc[c][b] = M_new[c0 + c][comp b]
+    const int h = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int nc = gt.ncomp[h];
+    if (nc == 0) return;
+    const int ta = tid / G, tbb = tid % G;
+    const int32_t* cl = gt.comps + (size_t)h * gt.gmax;
+    const Z* A = tb.Aall[f];
+    const Z* M = tb.Mall[f];
+    Z* D = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
+    Z acc[T][T];
+#pragma unroll
+    for (int i = 0; i < T; i++)
+#pragma unroll
+        for (int j = 0; j < T; j++) acc[i][j] = zc<Z>(0, 0);
+    for (int c0 = 0; c0 < k; c0 += 16) {
+        const int cl_ = min(16, k - c0);
+        for (int e = tid; e < nc * 16; e += 256) {
+            const int a = e / 16, c = e - a * 16;
+            Ac[c][a] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zc<Z>(0, 0);
+        }
+        for (int e = tid; e < 16 * nc; e += 256) {
+            const int c = e / nc, bb = e - c * nc;
+            Mc[c][bb] = c < cl_ ? M[(size_t)(K + c0 + c) * d.ldN + cl[bb]] : zc<Z>(0, 0);
+        }
+        __syncthreads();
+        for (int c = 0; c < cl_; c++) {
+            Z av[T], mv[T];
+#pragma unroll
+            for (int i = 0; i < T; i++) av[i] = Ac[c][ta + G * i];
+#pragma unroll
+            for (int j = 0; j < T; j++) mv[j] = Mc[c][tbb + G * j];
+#pragma unroll
+            for (int i = 0; i < T; i++)
+#pragma unroll
+                for (int j = 0; j < T; j++) acc[i][j] = zfma(av[i], mv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < T; i++)
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            const int a = ta + G * i, bb = tbb + G * j;
+            if (a < nc && bb < nc) {
+                Z* p = &D[a * gt.gmax + bb];
+                *p = zsub(*p, acc[i][j]);
+            }
+        }
+}
+
 template <class Z>
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt, int k,
                       int K) {
     if (gt.gmax <= 32)
         hipLaunchKernelGGL((lr_blocks_kernel<Z, 4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
     else
-        hipLaunchKernelGGL((lr_blocks_kernel<Z, 36>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+        hipLaunchKernelGGL(lr_blocks_tile_kernel<Z>, dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
 }
 
 // LOWRANK accept epilogue for k <= 32 in one launch.  A block takes a list of up to 32
