@@ -167,7 +167,8 @@ __device__ void lu_inverse_cols(const Z* a, int lda, int k, const int* piv, Z* X
 }
 
 // The same inverse from LU factors held in LDS, for k <= 96, written to X (row-major, k x k):
-// every entry goes through the operations of lu_inverse_cols in the same order.  The 256
+// every entry goes through the operations of lu_inverse_cols in the same order (a step also
+// gives the rows it does not touch x - 0 x_j, which leaves x as it is for a finite x_j).  The 256
 // threads form a 16 x 16 grid; thread (tr, tc) keeps the 6 x 6 entries of rows tr + 16 m and
 // columns tc + 16 n in registers (rows interleaved, so that every thread has work while the
 // active rows shrink).  At each elimination step the 16 owners of row j publish it through LDS
@@ -212,20 +213,14 @@ __device__ void lu_inverse_rows(const Z* a, int lda, int k, const int* piv, Z* X
         for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
 #pragma unroll
         for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
-        // slots below row j's are updated whole (rows past k hold values that are never
-        // stored); row j's slot only below j
+        // every slot holding a row below j is updated, a row not below j with a zero multiplier
+        // (x - 0 x_j is x: no per-row branches, whose merges would copy the tile)
 #pragma unroll
         for (int m = 0; m < T; m++) {
-            if (m > mj && G * m < k) {
+            if (G * m + G - 1 > j) {
+                const Z lm = tr + G * m > j ? l[m] : zc<Z>(0, 0);
 #pragma unroll
-                for (int n = 0; n < T; n++) x[m][n] = zfms(l[m], xj[n], x[m][n]);
-            } else if (m == mj) {
-                const bool below = tr + G * m > j;
-#pragma unroll
-                for (int n = 0; n < T; n++) {
-                    const Z nv = zfms(l[m], xj[n], x[m][n]);
-                    if (below) x[m][n] = nv;
-                }
+                for (int n = 0; n < T; n++) x[m][n] = zfms(lm, xj[n], x[m][n]);
             }
         }
         buf ^= 1;
@@ -250,19 +245,13 @@ __device__ void lu_inverse_rows(const Z* a, int lda, int k, const int* piv, Z* X
         for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
 #pragma unroll
         for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
-        // slots above row j's are updated whole, row j's slot only above j
+        // every slot holding a row above j, a row not above j with a zero multiplier
 #pragma unroll
         for (int m = 0; m < T; m++) {
-            if (m < mj) {
+            if (G * m < j) {
+                const Z lm = tr + G * m < j ? l[m] : zc<Z>(0, 0);
 #pragma unroll
-                for (int n = 0; n < T; n++) x[m][n] = zfms(l[m], xj[n], x[m][n]);
-            } else if (m == mj) {
-                const bool above = tr + G * m < j;
-#pragma unroll
-                for (int n = 0; n < T; n++) {
-                    const Z nv = zfms(l[m], xj[n], x[m][n]);
-                    if (above) x[m][n] = nv;
-                }
+                for (int n = 0; n < T; n++) x[m][n] = zfms(lm, xj[n], x[m][n]);
             }
         }
         buf ^= 1;
