@@ -659,6 +659,9 @@ def main():
                     help="only time pngf_load_g on acmefdfd-layout G files (21x21 tiles, 3 frequencies)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the two-strip and non-reciprocal-G runs")
+    ap.add_argument("--reference-config", action="store_true",
+                    help="run only the reference program's default configuration (LOWRANK and PLAIN) and "
+                         "print its result as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path (use --gpus 1)",
@@ -678,6 +681,9 @@ def main():
         return
     if args.load_probe:
         load_probe()
+        return
+    if args.reference_config:
+        print(json.dumps(reference_config()), flush=True)
         return
     if args.workload == "multi16":
         bench_multi(args.workers)
