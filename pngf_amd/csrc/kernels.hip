@@ -2283,60 +2283,12 @@ void launch_lr_small(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const
     hipLaunchKernelGGL(lr_small_kernel<Z>, grid, dim3(256), 0, st, d, tb, v, acc, acc_stride, k, K);
 }
 
-// D_h -= A_new[comp_h, :] M_new[:, comp_h] for every group h.
-template <class Z, int PER>
-__global__ void __launch_bounds__(256) lr_blocks_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt, int k,
-                                                        int K) {
-    __shared__ Z Ac[96][17];
-    __shared__ Z Mc[16][97];
-    const int h = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
-    const int nc = gt.ncomp[h];
-    if (nc == 0) return;
-    const int32_t* cl = gt.comps + (size_t)h * gt.gmax;
-    const Z* A = tb.Aall[f];
-    const Z* M = tb.Mall[f];
-    Z* D = v.Dblk + ((size_t)f * d.ngroups + h) * gt.gmax * gt.gmax;
-    Z acc[PER];
-#pragma unroll
-    for (int u = 0; u < PER; u++) acc[u] = zc<Z>(0, 0);
-    for (int c0 = 0; c0 < k; c0 += 16) {
-        const int cl_ = min(16, k - c0);
-        for (int e = tid; e < nc * 16; e += 256) {
-            const int a = e / 16, c = e - a * 16;
-            Ac[a][c] = c < cl_ ? A[(size_t)cl[a] * d.Kmax + K + c0 + c] : zc<Z>(0, 0);
-        }
-        for (int e = tid; e < 16 * nc; e += 256) {
-            const int c = e / nc, bb = e - c * nc;
-            Mc[c][bb] = c < cl_ ? M[(size_t)(K + c0 + c) * d.ldN + cl[bb]] : zc<Z>(0, 0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < PER; u++) {
-            const int e = tid + 256 * u;
-            if (e < nc * nc) {
-                const int a = e / nc, bb = e - a * nc;
-                Z s = acc[u];
-                for (int c = 0; c < cl_; c++) s = zfma(Ac[a][c], Mc[c][bb], s);
-                acc[u] = s;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < PER; u++) {
-        const int e = tid + 256 * u;
-        if (e < nc * nc) {
-            const int a = e / nc, bb = e - a * nc;
-            Z* p = &D[a * gt.gmax + bb];
-            *p = zsub(*p, acc[u]);
-        }
-    }
-}
-
-// The same update for groups of up to 96 components: thread (ta, tb) of a 16 x 16 grid keeps
-// the 6 x 6 entries of rows ta + 16 i and columns tb + 16 j in registers, so that each step c
-// reads 6 entries of A_new and 6 of M_new from LDS for 36 updates (the kernel above reads two per
-// update).  Every entry's sum runs over c in the same order, from zero, as above.
+// D_h -= A_new[comp_h, :] M_new[:, comp_h] for every group h (groups of up to 96 components: the
+// accept path of flips with k > 32; smaller ones fold this into lr_finish_kernel).  Thread (ta, tb)
+// of a 16 x 16 grid keeps the 6 x 6 entries of rows ta + 16 i and columns tb + 16 j in registers,
+// so that each step c reads 6 entries of A_new and 6 of M_new from LDS for 36 updates (one entry
+// per thread would read two per update).  Every entry's sum runs over c in ascending order from
+// zero, 16 columns of A_new and rows of M_new staged at a time.
 template <class Z>
 __global__ void __launch_bounds__(256) lr_blocks_tile_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, GroupTab gt,
                                                              int k, int K) {
@@ -2395,10 +2347,7 @@ __global__ void __launch_bounds__(256) lr_blocks_tile_kernel(Dims d, DevTabsT<Z>
 template <class Z>
 void launch_lr_blocks(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, GroupTab gt, int k,
                       int K) {
-    if (gt.gmax <= 32)
-        hipLaunchKernelGGL((lr_blocks_kernel<Z, 4>), dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
-    else
-        hipLaunchKernelGGL(lr_blocks_tile_kernel<Z>, dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
+    hipLaunchKernelGGL(lr_blocks_tile_kernel<Z>, dim3(d.ngroups, d.nf), dim3(256), 0, st, d, tb, v, gt, k, K);
 }
 
 // LOWRANK accept epilogue for k <= 32 in one launch.  A block takes a list of up to 32
