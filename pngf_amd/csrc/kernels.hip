@@ -166,104 +166,6 @@ __device__ void lu_inverse_cols(const Z* a, int lda, int k, const int* piv, Z* X
     __syncthreads();
 }
 
-// The same inverse from LU factors held in LDS, for k <= 96, written to X (row-major, k x k):
-// every entry goes through the operations of lu_inverse_cols in the same order (a step also
-// gives the rows it does not touch x - 0 x_j, which leaves x as it is for a finite x_j).  The 256
-// threads form a 16 x 16 grid; thread (tr, tc) keeps the 6 x 6 entries of rows tr + 16 m and
-// columns tc + 16 n in registers (rows interleaved, so that every thread has work while the
-// active rows shrink).  At each elimination step the 16 owners of row j publish it through LDS
-// (rowbuf, two alternating rows of 96, so one barrier per step); every thread then reads its 6
-// multipliers and 6 entries of row j and updates its tile: 12 LDS reads per 36 updates.
-template <class Z>
-__device__ void lu_inverse_rows(const Z* a, int lda, int k, const int* piv, Z* X, Z* rowbuf) {
-    constexpr int T = 6, G = 16, NC = 96;
-    const int tc = threadIdx.x & (G - 1), tr = threadIdx.x / G;
-    // the 1 of e_c, for each of this thread's columns, after the row interchanges piv[0..k-1]
-    int pos[T];
-#pragma unroll
-    for (int n = 0; n < T; n++) pos[n] = tc + G * n;
-    for (int j = 0; j < k; j++) {
-        const int p = piv[j];
-        if (p != j) {
-#pragma unroll
-            for (int n = 0; n < T; n++) {
-                if (pos[n] == j) pos[n] = p;
-                else if (pos[n] == p) pos[n] = j;
-            }
-        }
-    }
-    Z x[T][T];
-#pragma unroll
-    for (int m = 0; m < T; m++)
-#pragma unroll
-        for (int n = 0; n < T; n++) x[m][n] = zc<Z>(tr + G * m == pos[n] ? 1 : 0, 0);
-    int buf = 0;
-    // unit lower triangle, j increasing: rows i > j
-    for (int j = 0; j < k; j++) {
-        const int mj = j / G;
-        if (tr == j - G * mj)
-#pragma unroll
-            for (int m = 0; m < T; m++)
-                if (m == mj)
-#pragma unroll
-                    for (int n = 0; n < T; n++) rowbuf[buf * NC + tc + G * n] = x[m][n];
-        __syncthreads();
-        Z xj[T], l[T];
-#pragma unroll
-        for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
-#pragma unroll
-        for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
-        // every slot holding a row below j is updated, a row not below j with a zero multiplier
-        // (x - 0 x_j is x: no per-row branches, whose merges would copy the tile)
-#pragma unroll
-        for (int m = 0; m < T; m++) {
-            if (G * m + G - 1 > j) {
-                const Z lm = tr + G * m > j ? l[m] : zc<Z>(0, 0);
-#pragma unroll
-                for (int n = 0; n < T; n++) x[m][n] = zfms(lm, xj[n], x[m][n]);
-            }
-        }
-        buf ^= 1;
-    }
-    // upper triangle, j decreasing: x_j /= u_jj, then rows i < j
-    for (int j = k - 1; j >= 0; j--) {
-        const int mj = j / G;
-        if (tr == j - G * mj) {
-            const Z d = a[j * lda + j];
-#pragma unroll
-            for (int m = 0; m < T; m++)
-                if (m == mj)
-#pragma unroll
-                    for (int n = 0; n < T; n++) {
-                        x[m][n] = zdiv(x[m][n], d);
-                        rowbuf[buf * NC + tc + G * n] = x[m][n];
-                    }
-        }
-        __syncthreads();
-        Z xj[T], l[T];
-#pragma unroll
-        for (int n = 0; n < T; n++) xj[n] = rowbuf[buf * NC + tc + G * n];
-#pragma unroll
-        for (int m = 0; m < T; m++) l[m] = a[min(tr + G * m, k - 1) * lda + j];
-        // every slot holding a row above j, a row not above j with a zero multiplier
-#pragma unroll
-        for (int m = 0; m < T; m++) {
-            if (G * m < j) {
-                const Z lm = tr + G * m < j ? l[m] : zc<Z>(0, 0);
-#pragma unroll
-                for (int n = 0; n < T; n++) x[m][n] = zfms(lm, xj[n], x[m][n]);
-            }
-        }
-        buf ^= 1;
-    }
-#pragma unroll
-    for (int m = 0; m < T; m++)
-#pragma unroll
-        for (int n = 0; n < T; n++)
-            if (tr + G * m < k && tc + G * n < k) X[(tr + G * m) * k + tc + G * n] = x[m][n];
-    __syncthreads();
-}
-
 // Entry (r, c) of the maintained diagonal block D (row-major, leading dimension gmax).  For
 // symmetric W the block is read as (D + D^T) / 2, which removes the rounding asymmetry that the
 // eager block updates accumulate (a symmetric-representation Woodbury update fed an asymmetric
@@ -1071,10 +973,10 @@ __global__ __launch_bounds__(256) void lr_prep_kernel(Dims d, DevVecsT<Z> v, int
     Z* dpe = (Z*)(piv + 100);
     Z* tmp = dpe + 96;
     // the matrix: LDS for k <= 32, else this frequency's global slot, or, with lds_big (k <= 96
-    // where the workgroup's LDS holds k x (k+1) entries), LDS for the LU with the inverse built
-    // in registers and written to the global slot (the body is expanded for each, so that the
-    // LDS cases compile to LDS instructions)
-    auto rest = [&](Z* a, Z* Xg, Z* rowbuf) __attribute__((always_inline)) {
+    // where the workgroup's LDS holds k x (k+1) entries), LDS for the LU, whose factors then go to
+    // the global slot (fac) for lr_prep_inv_kernel and lr_prep_lam_kernel (the body is expanded
+    // for each, so that the LDS cases compile to LDS instructions)
+    auto rest = [&](Z* a, Z* fac) __attribute__((always_inline)) {
     // k <= 32 (LDS): Gauss-Jordan on [cap | rhs | I]; larger k: LU, back substitution, inverse
     const bool gj = gscratch == nullptr;
     const int lda = gj ? 2 * k + 1 : k + 1;
@@ -1148,12 +1050,21 @@ __global__ __launch_bounds__(256) void lr_prep_kernel(Dims d, DevVecsT<Z> v, int
             v.Eobj[f * ne + r] = e;
         }
     }
-    // cap^-1: X[r][c] = a[perm r][k+1+c] / d_r after Gauss-Jordan, else from the LU factors
-    Z* X = Xg ? Xg : a + k * lda;
-    if (!gj) {
-        if (rowbuf) lu_inverse_rows(a, lda, k, piv, X, rowbuf);
-        else lu_inverse_cols(a, lda, k, piv, X, k);
+    if (fac) {
+        // the factors (k x k, leading dimension k) and the pivots (as ints in the scratch-LU
+        // place of the acc layout) for the inverse, which more workgroups share
+        for (int e = tid; e < k * k; e += nth) {
+            const int r = e / k, c = e - r * k;
+            fac[e] = a[r * lda + c];
+        }
+        int* pv = (int*)(st + k * k);
+        for (int c = tid; c < k; c += nth) pv[c] = piv[c];
+        if (tid == 0 && info_out) info_out[f] = info;
+        return;
     }
+    // cap^-1: X[r][c] = a[perm r][k+1+c] / d_r after Gauss-Jordan, else from the LU factors
+    Z* X = a + k * lda;
+    if (!gj) lu_inverse_cols(a, lda, k, piv, X, k);
     auto xinv = [&](int r, int c) {
         return gj ? zdiv(a[piv[r] * lda + k + 1 + c], a[piv[r] * lda + r]) : X[r * k + c];
     };
@@ -1172,11 +1083,129 @@ __global__ __launch_bounds__(256) void lr_prep_kernel(Dims d, DevVecsT<Z> v, int
     if (tid == 0 && info_out) info_out[f] = info;
     };
     if (!gscratch)
-        rest(tmp + 96, nullptr, nullptr);
+        rest(tmp + 96, nullptr);
     else if (lds_big)
-        rest(tmp + 96, gscratch + (size_t)f * gmax * (2 * gmax + 1), tmp + 96 + (size_t)gmax * (gmax + 1));
+        rest(tmp + 96, gscratch + (size_t)f * gmax * (2 * gmax + 1));
     else
-        rest(gscratch + (size_t)f * gmax * (2 * gmax + 1), nullptr, nullptr);
+        rest(gscratch + (size_t)f * gmax * (2 * gmax + 1), nullptr);
+}
+
+// cap^-1 of the large flips from the factors lr_prep_kernel left in the global slot: one
+// workgroup per (frequency, 16 columns), the factors staged in LDS.  Thread (tr, tc) keeps rows
+// tr + 16 m (m < 6) of column 16 blockIdx.y + tc in registers; every entry goes through the
+// operations of lu_inverse_cols in the same order (a step also gives the rows it does not touch
+// x - 0 x_j, which leaves x as it is for a finite x_j).  At each step the owners of row j publish
+// it through LDS (two alternating rows of 16, one barrier per step).  X goes to the slot after
+// the factors.
+template <class Z>
+__global__ __launch_bounds__(256) void lr_prep_inv_kernel(Dims d, int k, const Z* __restrict__ acc, int acc_stride,
+                                                          Z* gscratch, const int32_t* dyn) {
+    constexpr int T = 6, G = 16;
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+    }
+    const int f = blockIdx.x, c0 = G * blockIdx.y;
+    if (c0 >= k) return;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    Z* L = (Z*)smraw;                 // [k][k] factors
+    Z* rowbuf = L + (size_t)k * k;    // [2][16]
+    int* piv = (int*)(rowbuf + 2 * G);
+    const int tid = threadIdx.x, tc = tid % G, tr = tid / G;
+    const Z* fac = gscratch + (size_t)f * d.gmax * (2 * d.gmax + 1);
+    const int* pg = (const int*)(acc + (size_t)f * acc_stride + (size_t)k * k);
+    for (int e = tid; e < k * k; e += 256) L[e] = fac[e];
+    for (int c = tid; c < k; c += 256) piv[c] = pg[c];
+    __syncthreads();
+    const int c = c0 + tc;
+    // the 1 of e_c after the row interchanges piv[0..k-1]
+    int pos = c;
+    for (int j = 0; j < k; j++) {
+        const int p = piv[j];
+        if (p != j) {
+            if (pos == j) pos = p;
+            else if (pos == p) pos = j;
+        }
+    }
+    Z x[T];
+#pragma unroll
+    for (int m = 0; m < T; m++) x[m] = zc<Z>(tr + G * m == pos ? 1 : 0, 0);
+    int buf = 0;
+    // unit lower triangle, j increasing: rows i > j
+    for (int j = 0; j < k; j++) {
+        const int mj = j / G;
+        if (tr == j - G * mj)
+#pragma unroll
+            for (int m = 0; m < T; m++)
+                if (m == mj) rowbuf[buf * G + tc] = x[m];
+        __syncthreads();
+        const Z xj = rowbuf[buf * G + tc];
+        Z l[T];
+#pragma unroll
+        for (int m = 0; m < T; m++) l[m] = L[min(tr + G * m, k - 1) * k + j];
+#pragma unroll
+        for (int m = 0; m < T; m++)
+            if (G * m + G - 1 > j) {
+                const Z lm = tr + G * m > j ? l[m] : zc<Z>(0, 0);
+                x[m] = zfms(lm, xj, x[m]);
+            }
+        buf ^= 1;
+    }
+    // upper triangle, j decreasing: x_j /= u_jj, then rows i < j
+    for (int j = k - 1; j >= 0; j--) {
+        const int mj = j / G;
+        if (tr == j - G * mj) {
+            const Z dj = L[j * k + j];
+#pragma unroll
+            for (int m = 0; m < T; m++)
+                if (m == mj) {
+                    x[m] = zdiv(x[m], dj);
+                    rowbuf[buf * G + tc] = x[m];
+                }
+        }
+        __syncthreads();
+        const Z xj = rowbuf[buf * G + tc];
+        Z l[T];
+#pragma unroll
+        for (int m = 0; m < T; m++) l[m] = L[min(tr + G * m, k - 1) * k + j];
+#pragma unroll
+        for (int m = 0; m < T; m++)
+            if (G * m < j) {
+                const Z lm = tr + G * m < j ? l[m] : zc<Z>(0, 0);
+                x[m] = zfms(lm, xj, x[m]);
+            }
+        buf ^= 1;
+    }
+    Z* X = gscratch + (size_t)f * d.gmax * (2 * d.gmax + 1) + (size_t)k * k;
+    if (c < k)
+#pragma unroll
+        for (int m = 0; m < T; m++)
+            if (tr + G * m < k) X[(tr + G * m) * k + c] = x[m];
+}
+
+// Lambda of the large flips from lr_prep_inv_kernel's cap^-1, as lr_prep_kernel forms it for the
+// smaller ones: diag(s) cap^-1, stored exactly symmetric for symmetric W.
+template <class Z>
+__global__ __launch_bounds__(256) void lr_prep_lam_kernel(Dims d, int k, const int32_t* __restrict__ ss, Z* acc,
+                                                          int acc_stride, const Z* gscratch, const int32_t* dyn) {
+    typedef typename zreal<Z>::type R;
+    if (dyn) {
+        if (!dyn[kDynActive]) return;
+        k = dyn[kDynK];
+    }
+    const int f = blockIdx.x;
+    const Z* X = gscratch + (size_t)f * d.gmax * (2 * d.gmax + 1) + (size_t)k * k;
+    Z* st = acc + (size_t)f * acc_stride;
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int r = e / k, c = e - r * k;
+        Z x = X[r * k + c];
+        if (d.sym) {
+            Z u = X[c * k + r];
+            if (ss[r] != ss[c]) u = zc<Z>(-u.x, -u.y);
+            x = zc<Z>(R(0.5) * (x.x + u.x), R(0.5) * (x.y + u.y));
+        }
+        st[e] = x;
+    }
 }
 
 template <class Z>
@@ -1185,18 +1214,29 @@ void launch_lr_accept_prep(hipStream_t st, const Dims& d, const DevVecsT<Z>& v, 
                            int32_t* info, Z* gscratch, const int32_t* dyn, const int32_t* g_dev) {
     size_t sm = 100 * sizeof(int) + 192 * sizeof(Z) + lds_matrix_bytes(d.gmax, gscratch);
     int lds_big = 0;
+    size_t sm_inv = 0;
     if (gscratch && d.gmax <= 96) {
-        // the k x (k+1) LU and two rows of the inverse in LDS when the workgroup may take them
-        const size_t need = sm + (size_t)d.gmax * (d.gmax + 1) * sizeof(Z) + 2 * 96 * sizeof(Z);
-        if (need <= lds_per_workgroup()) {
+        // the k x (k+1) LU in LDS, then the inverse on (frequency, 16-column) workgroups with the
+        // k x k factors in LDS, when the device's per-workgroup LDS takes them
+        const size_t need = sm + (size_t)d.gmax * (d.gmax + 1) * sizeof(Z);
+        sm_inv = (size_t)d.gmax * d.gmax * sizeof(Z) + 32 * sizeof(Z) + 96 * sizeof(int);
+        if (need <= lds_per_workgroup() && sm_inv <= lds_per_workgroup()) {
             (void)hipFuncSetAttribute((const void*)lr_prep_kernel<Z>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)need);
+            (void)hipFuncSetAttribute((const void*)lr_prep_inv_kernel<Z>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm_inv);
             sm = need;
             lds_big = 1;
         }
     }
     hipLaunchKernelGGL(lr_prep_kernel<Z>, dim3(d.nf), dim3(256), sm, st, d, v, g, t, s, lp, k, acc,
                        acc_stride, info, gscratch, dyn, g_dev, lds_big);
+    if (lds_big) {
+        hipLaunchKernelGGL(lr_prep_inv_kernel<Z>, dim3(d.nf, (d.gmax + 15) / 16), dim3(256), sm_inv, st, d, k,
+                           (const Z*)acc, acc_stride, gscratch, dyn);
+        hipLaunchKernelGGL(lr_prep_lam_kernel<Z>, dim3(d.nf), dim3(256), 0, st, d, k, s, acc, acc_stride,
+                           (const Z*)gscratch, dyn);
+    }
 }
 
 // M[kk][n]: stored, or from Lam and A (see DevTabs); the sum runs in the order the accept's
