@@ -5,7 +5,7 @@
 // the k components t (ascending) with signs s = +-1.
 //   PLAIN   (reference formulation, pngfopt.hpp:379-539): cap = I + (G-I)[t,:] C^-1[:,t] diag(s)
 //   LOWRANK (W = (G-I) C^-1, q = W PEinc): cap = I + W[t,t] diag(s), rhs = q_t + W[t,t] dPE
-// Both give the same newEobj in exact arithmetic; see DESIGN notes in the runtime.
+// Both give the same newEobj in exact arithmetic (the accept paths are in runtime.hip).
 #include "kernels.h"
 
 #include <type_traits>
