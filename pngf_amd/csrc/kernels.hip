@@ -2255,12 +2255,20 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, con
 // PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
 // LOWRANK: M_new = diag(s) cap^-1 W[t,:]  (W[t,:]^T = W[:,t], the new A_all columns, when W
 //          is symmetric)
+// A workgroup takes 32 rows n (B rows staged in LDS) and every c; thread (tn, tcg) keeps the 4 x 3
+// outputs n = tn + 8 a, c = tcg + 32 b in registers, reading capinv 16 columns at a time from an
+// LDS copy, so that each l costs 7 LDS reads for 12 updates.  Every output's sum runs over l in
+// ascending order from zero.  k <= 96.
 template <class Z>
-__global__ void apply_capinv_kernel(Dims d, const Z* __restrict__ acc, int acc_stride, const Z* __restrict__ B,
-                                    size_t b_stride, const Z* const* b_tab, int b_col0, int ldb, int k,
-                                    const int32_t* __restrict__ s, Z* const* out_tab, Z* out_single,
-                                    size_t out_stride, int out_row0, int ld_out) {
+__global__ __launch_bounds__(256) void apply_capinv_kernel(Dims d, const Z* __restrict__ acc, int acc_stride,
+                                                           const Z* __restrict__ B, size_t b_stride,
+                                                           const Z* const* b_tab, int b_col0, int ldb, int k,
+                                                           const int32_t* __restrict__ s, Z* const* out_tab,
+                                                           Z* out_single, size_t out_stride, int out_row0,
+                                                           int ld_out) {
+    constexpr int NA = 4, NB = 3, LC = 16;
     __shared__ Z Bs[32][97];
+    __shared__ Z Cs[96][LC + 1];
     const int f = blockIdx.y, tid = threadIdx.x;
     const int n0 = blockIdx.x * 32;
     const Z* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
@@ -2270,17 +2278,44 @@ __global__ void apply_capinv_kernel(Dims d, const Z* __restrict__ acc, int acc_s
         const int n = n0 + nn;
         Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zc<Z>(0, 0);
     }
-    __syncthreads();
-    Z* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
-    for (int e = tid; e < 32 * k; e += 256) {
-        const int c = e / 32, nn = e - c * 32;
-        const int n = n0 + nn;
-        if (n >= d.N) continue;
-        Z sacc = zc<Z>(0, 0);
-        for (int l = 0; l < k; l++) sacc = zfma(ci[c * k + l], Bs[nn][l], sacc);
-        if (s && s[c] < 0) sacc = zc<Z>(-sacc.x, -sacc.y);
-        out[(size_t)(out_row0 + c) * ld_out + n] = sacc;
+    const int tn = tid % 8, tcg = tid / 8;
+    Z sacc[NA][NB];
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) sacc[a][b] = zc<Z>(0, 0);
+    for (int l0 = 0; l0 < k; l0 += LC) {
+        const int lc = min(LC, k - l0);
+        __syncthreads();
+        for (int e = tid; e < k * LC; e += 256) {
+            const int c = e / LC, l = e - c * LC;
+            Cs[c][l] = l < lc ? ci[c * k + l0 + l] : zc<Z>(0, 0);
+        }
+        __syncthreads();
+        for (int l = 0; l < lc; l++) {
+            Z bv[NA], cv[NB];
+#pragma unroll
+            for (int a = 0; a < NA; a++) bv[a] = Bs[tn + 8 * a][l0 + l];
+#pragma unroll
+            for (int b = 0; b < NB; b++) cv[b] = Cs[min(tcg + 32 * b, k - 1)][l];
+#pragma unroll
+            for (int a = 0; a < NA; a++)
+#pragma unroll
+                for (int b = 0; b < NB; b++) sacc[a][b] = zfma(cv[b], bv[a], sacc[a][b]);
+        }
     }
+    Z* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int n = n0 + tn + 8 * a, c = tcg + 32 * b;
+            if (n < d.N && c < k) {
+                Z v = sacc[a][b];
+                if (s && s[c] < 0) v = zc<Z>(-v.x, -v.y);
+                out[(size_t)(out_row0 + c) * ld_out + n] = v;
+            }
+        }
 }
 
 template <class Z>
