@@ -2328,34 +2328,53 @@ void launch_apply_capinv(hipStream_t st, const Dims& d, const Z* acc, int acc_st
                        b_col0, ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
 }
 
-// q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept)
+// q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept).  A workgroup of 128
+// takes 32 components n: their rows of A_new (k entries each, 169 KB apart in A_all) are staged
+// in LDS with consecutive lanes on consecutive entries, then thread n < 32 forms q[n] and threads
+// 32.. the R[r][n] (M_new rows read across consecutive n).  Each sum runs over c in ascending
+// order from the old value.
 template <class Z>
-__global__ void lr_small_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v, const Z* __restrict__ acc, int acc_stride,
-                                int k, int K) {
-    const int f = blockIdx.y;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= d.N) return;
+__global__ __launch_bounds__(128) void lr_small_kernel(Dims d, DevTabsT<Z> tb, DevVecsT<Z> v,
+                                                       const Z* __restrict__ acc, int acc_stride, int k, int K) {
+    constexpr int RW = 32;
+    __shared__ Z As[RW][97];
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const int n0 = blockIdx.x * RW;
     const Z* st = acc + (size_t)f * acc_stride;
     const Z* w = st + 2 * k * k + k;
     const Z* Rt = st + 2 * k * k + 2 * k + 4;
-    const Z* A = tb.Aall[f] + (size_t)n * d.Kmax + K;
-    const Z* M = tb.Mall[f] + (size_t)K * d.ldN + n;
-    Z qa = v.q[(size_t)f * d.ldN + n];
-    for (int c = 0; c < k; c++) qa = zfma(A[c], w[c], qa);
-    v.q[(size_t)f * d.ldN + n] = qa;
-    for (int r = 0; r < d.ne; r++) {
-        Z* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
-        Z ra = R[n];
-        for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], M[(size_t)c * d.ldN], ra);
-        R[n] = ra;
+    const Z* A = tb.Aall[f];
+    for (int e = tid; e < RW * k; e += 128) {
+        const int r = e / k, c = e - r * k;
+        const int n = n0 + r;
+        As[r][c] = n < d.N ? A[(size_t)n * d.Kmax + K + c] : zc<Z>(0, 0);
+    }
+    __syncthreads();
+    if (tid < RW) {
+        const int n = n0 + tid;
+        if (n < d.N) {
+            Z qa = v.q[(size_t)f * d.ldN + n];
+            for (int c = 0; c < k; c++) qa = zfma(As[tid][c], w[c], qa);
+            v.q[(size_t)f * d.ldN + n] = qa;
+        }
+    } else {
+        const Z* M = tb.Mall[f] + (size_t)K * d.ldN;
+        for (int e = tid - RW; e < d.ne * RW; e += 128 - RW) {
+            const int r = e / RW, n = n0 + (e - r * RW);
+            if (n >= d.N) continue;
+            Z* R = v.R + ((size_t)f * d.ne + r) * d.ldN;
+            Z ra = R[n];
+            for (int c = 0; c < k; c++) ra = zfms(Rt[r * k + c], M[(size_t)c * d.ldN + n], ra);
+            R[n] = ra;
+        }
     }
 }
 
 template <class Z>
 void launch_lr_small(hipStream_t st, const Dims& d, const DevTabsT<Z>& tb, const DevVecsT<Z>& v, const Z* acc,
                      int acc_stride, int k, int K) {
-    dim3 grid((d.N + 255) / 256, d.nf);
-    hipLaunchKernelGGL(lr_small_kernel<Z>, grid, dim3(256), 0, st, d, tb, v, acc, acc_stride, k, K);
+    dim3 grid((d.N + 31) / 32, d.nf);
+    hipLaunchKernelGGL(lr_small_kernel<Z>, grid, dim3(128), 0, st, d, tb, v, acc, acc_stride, k, K);
 }
 
 // D_h -= A_new[comp_h, :] M_new[:, comp_h] for every group h (groups of up to 96 components: the
