@@ -21,6 +21,21 @@ __device__ inline unsigned long long rbits(float x) { return (unsigned long long
 extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
 extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
 
+// x of lane `src` in every lane (src uniform)
+__device__ inline zd readlane_z(zd x, int src) {
+    const unsigned long long rx = (unsigned long long)__double_as_longlong(x.x);
+    const unsigned long long ry = (unsigned long long)__double_as_longlong(x.y);
+    const unsigned long long ox = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)rx, src) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(rx >> 32), src) << 32);
+    const unsigned long long oy = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)ry, src) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(ry >> 32), src) << 32);
+    return zmk(__longlong_as_double((long long)ox), __longlong_as_double((long long)oy));
+}
+__device__ inline zf readlane_z(zf x, int src) {
+    return zmkf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.x), src)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.y), src)));
+}
+
 // ------------------------------------------------------------------ helpers --
 
 // Components of flip group g whose metal state changes, ascending, with sign
@@ -54,37 +69,59 @@ __device__ int change_set(const GroupTab& gt, int g, const int32_t* tile, const 
 // columns k..ncol-1 as well.  Pivot = first row of max |Re|+|Im| (LAPACK
 // izamax / zgetf2); rows are swapped across all ncol columns and the multipliers
 // l = a_ij * (1/a_jj) are stored below the diagonal.  piv[j] = pivot row.
-// All threads of the block call it.  Returns LAPACK-style info (0 or j+1).
+// All threads of the block call it (blockDim.x a multiple of 64); k <= 128.  Returns
+// LAPACK-style info (0 or j+1).
+//
+// Look-ahead: in the trailing update of column j, wave 0 updates column j + 1 first (lane i its
+// rows j+1+i and j+65+i) and searches it for the next pivot from those registers, while the other
+// waves update the columns after it; so a column costs two barriers and no search phase of its
+// own.  Every entry goes through the same operations as in a column-at-a-time elimination.
 template <class Z>
-__device__ int lds_lu(Z* a, int lda, int k, int ncol, int* piv) {
+__device__ __forceinline__ int lds_lu(Z* a, int lda, int k, int ncol, int* piv) {
     typedef typename zreal<Z>::type R;
-    const int tid = threadIdx.x, nth = blockDim.x;
+    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, nw = nth >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     __shared__ int s_info;
     __shared__ R s_pv[6];  // pivot, old diagonal entry, reciprocal of the pivot
-    if (tid == 0) s_info = 0;
-    for (int j = 0; j < k; j++) {
-        if (tid < 64) {
-            // first row of max |Re|+|Im|: each lane's first maximum over its rows, then the
-            // largest value (|.| >= 0 orders like its bit pattern; NaN and empty lanes: key 0)
-            // and the smallest row holding it, as two wave reductions
-            R best = -1; int bi = 0x7fffffff;
-            for (int i = j + tid; i < k; i += 64) {
-                R v = zabs1(a[i * lda + j]);
-                if (v > best) { best = v; bi = i; }
-            }
-            const unsigned long long key = best >= 0 ? rbits(best) + 1ull : 0ull;
-            const unsigned long long mk = __ockl_wfred_max_u64(key);
-            // (a column of NaN only: no lane holds a row, and row j is taken, as izamax does)
-            const unsigned mp = __ockl_wfred_min_u32(key == mk ? (unsigned)bi : 0xffffffffu);
-            const int p = mp < (unsigned)k ? (int)mp : j;
-            if (tid == 0) {
-                piv[j] = p;
-                const Z pv = a[p * lda + j], ajj = a[j * lda + j];
-                const Z r = (pv.x == 0 && pv.y == 0) ? pv : zrecip(pv);
-                s_pv[0] = pv.x; s_pv[1] = pv.y; s_pv[2] = ajj.x; s_pv[3] = ajj.y; s_pv[4] = r.x; s_pv[5] = r.y;
-            }
+    // the pivot of column j from this wave's values of it (rows j+lane and j+lane+64; wave 0
+    // only): the first row of max |Re|+|Im| as two wave reductions (|.| >= 0 orders like its bit
+    // pattern; NaN and empty lanes: key 0; a column of NaN only takes row j, as izamax does)
+    auto pick = [&](int j, Z y0, Z y1) __attribute__((always_inline)) {
+        R best = -1;
+        int bi = 0x7fffffff;
+        if (j + lane < k) {
+            const R v = zabs1(y0);
+            if (v > best) { best = v; bi = j + lane; }
         }
-        __syncthreads();
+        if (j + lane + 64 < k) {
+            const R v = zabs1(y1);
+            if (v > best) { best = v; bi = j + lane + 64; }
+        }
+        const unsigned long long key = best >= 0 ? rbits(best) + 1ull : 0ull;
+        const unsigned long long mk = __ockl_wfred_max_u64(key);
+        const unsigned mp = __ockl_wfred_min_u32(key == mk ? (unsigned)bi : 0xffffffffu);
+        const int p = mp < (unsigned)k ? (int)mp : j;
+        // the pivot entry and the diagonal one, from the lanes holding them
+        const Z pv = p - j < 64 ? readlane_z(y0, p - j) : readlane_z(y1, p - j - 64);
+        const Z ajj = readlane_z(y0, 0);
+        if (lane == 0) {
+            piv[j] = p;
+            const Z r = (pv.x == 0 && pv.y == 0) ? pv : zrecip(pv);
+            s_pv[0] = pv.x; s_pv[1] = pv.y; s_pv[2] = ajj.x; s_pv[3] = ajj.y; s_pv[4] = r.x; s_pv[5] = r.y;
+        }
+    };
+    // the search of column j read from LDS (the first column, and the one after a zero pivot)
+    auto search = [&](int j) __attribute__((always_inline)) {
+        if (wv == 0) {
+            const Z y0 = j + lane < k ? a[(j + lane) * lda + j] : zc<Z>(0, 0);
+            const Z y1 = j + lane + 64 < k ? a[(j + lane + 64) * lda + j] : zc<Z>(0, 0);
+            pick(j, y0, y1);
+        }
+    };
+    if (tid == 0) s_info = 0;
+    search(0);
+    __syncthreads();
+    for (int j = 0; j < k; j++) {
         // the row interchange and the multipliers in one phase: column j is left to the
         // multiplier loop (row p's entry there is the old diagonal one, read before the barrier)
         const int p = piv[j];
@@ -96,30 +133,48 @@ __device__ int lds_lu(Z* a, int lda, int k, int ncol, int* piv) {
         if (zero) {
             if (tid == 0 && s_info == 0) s_info = j + 1;
             __syncthreads();
+            if (j + 1 < k) search(j + 1);
+            __syncthreads();
             continue;
         }
         for (int i = j + 1 + tid; i < k; i += nth) a[i * lda + j] = zmul(i == p ? ajj : a[i * lda + j], r);
         if (tid == 0) a[j * lda + j] = pv;
         __syncthreads();
-        // trailing update: lanes take columns, waves take rows (every wave reads the same
-        // multiplier, a broadcast), four rows' loads issued before their stores
-        {
-            const int lane = tid & 63, nw = nth >> 6;
-            const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-            for (int c = j + 1 + lane; c < ncol; c += 64) {
+        // wave 0: column j + 1 (rows j + 1 .. k - 1), then the next pivot from it
+        if (wv == 0) {
+            const int c = j + 1;
+            const Z u = a[j * lda + c];
+            const int i0 = j + 1 + lane, i1 = i0 + 64;
+            Z y0 = zc<Z>(0, 0), y1 = zc<Z>(0, 0);
+            if (i0 < k) {
+                y0 = zfms(a[i0 * lda + j], u, a[i0 * lda + c]);
+                a[i0 * lda + c] = y0;
+            }
+            if (i1 < k) {
+                y1 = zfms(a[i1 * lda + j], u, a[i1 * lda + c]);
+                a[i1 * lda + c] = y1;
+            }
+            if (j + 1 < k) pick(j + 1, y0, y1);
+        }
+        // the columns after it: lanes take columns, the other waves take rows (every wave reads
+        // the same multiplier, a broadcast), four rows' loads issued before their stores; with one
+        // wave, wave 0 does them as well
+        if (wv > 0 || nw == 1) {
+            const int rw = nw > 1 ? wv - 1 : 0, nr = nw > 1 ? nw - 1 : 1;
+            for (int c = j + 2 + lane; c < ncol; c += 64) {
                 const Z u = a[j * lda + c];
-                int i = j + 1 + wv;
-                for (; i + 3 * nw < k; i += 4 * nw) {
+                int i = j + 1 + rw;
+                for (; i + 3 * nr < k; i += 4 * nr) {
                     Z l[4], x[4];
 #pragma unroll
                     for (int m = 0; m < 4; m++) {
-                        l[m] = a[(i + m * nw) * lda + j];
-                        x[m] = a[(i + m * nw) * lda + c];
+                        l[m] = a[(i + m * nr) * lda + j];
+                        x[m] = a[(i + m * nr) * lda + c];
                     }
 #pragma unroll
-                    for (int m = 0; m < 4; m++) a[(i + m * nw) * lda + c] = zfms(l[m], u, x[m]);
+                    for (int m = 0; m < 4; m++) a[(i + m * nr) * lda + c] = zfms(l[m], u, x[m]);
                 }
-                for (; i < k; i += nw) a[i * lda + c] = zfms(a[i * lda + j], u, a[i * lda + c]);
+                for (; i < k; i += nr) a[i * lda + c] = zfms(a[i * lda + j], u, a[i * lda + c]);
             }
         }
         __syncthreads();
@@ -486,20 +541,6 @@ void launch_loop_control(hipStream_t st, const Dims& d, GroupTab gt, const int32
                        iterlog);
 }
 
-// x of lane `src` in every lane (src uniform)
-__device__ inline zd readlane_z(zd x, int src) {
-    const unsigned long long rx = (unsigned long long)__double_as_longlong(x.x);
-    const unsigned long long ry = (unsigned long long)__double_as_longlong(x.y);
-    const unsigned long long ox = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)rx, src) |
-                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(rx >> 32), src) << 32);
-    const unsigned long long oy = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)ry, src) |
-                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(ry >> 32), src) << 32);
-    return zmk(__longlong_as_double((long long)ox), __longlong_as_double((long long)oy));
-}
-__device__ inline zf readlane_z(zf x, int src) {
-    return zmkf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.x), src)),
-                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.y), src)));
-}
 
 // LOWRANK scoring of one (candidate b, frequency f) per wave, for flips of at most G <= 32
 // components: lane r holds row r of [cap | rhs] in registers (cap = I + D[t,t] diag(s), D the
