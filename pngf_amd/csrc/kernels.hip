@@ -2296,7 +2296,39 @@ void launch_lr_rowblk(hipStream_t st, const Dims& d, const DevTabsT<zf>& tb, con
 // PLAIN:   M = cap^-1 VCinv           (pngfopt.hpp:276-333; no sign)
 // LOWRANK: M_new = diag(s) cap^-1 W[t,:]  (W[t,:]^T = W[:,t], the new A_all columns, when W
 //          is symmetric)
-// A workgroup takes 32 rows n (B rows staged in LDS) and every c; thread (tn, tcg) keeps the 4 x 3
+// k <= 32: thread e of the workgroup's 32 x k outputs (c = e / 32, n = e % 32) sums over l with
+// capinv read from global memory (its rows are shared by the workgroup's threads).
+template <class Z>
+__global__ __launch_bounds__(256) void apply_capinv_small_kernel(Dims d, const Z* __restrict__ acc, int acc_stride,
+                                                                 const Z* __restrict__ B, size_t b_stride,
+                                                                 const Z* const* b_tab, int b_col0, int ldb, int k,
+                                                                 const int32_t* __restrict__ s, Z* const* out_tab,
+                                                                 Z* out_single, size_t out_stride, int out_row0,
+                                                                 int ld_out) {
+    __shared__ Z Bs[32][33];
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const int n0 = blockIdx.x * 32;
+    const Z* Bf = b_tab ? b_tab[f] + b_col0 : B + (size_t)f * b_stride;
+    const Z* ci = acc + (size_t)f * acc_stride;
+    for (int e = tid; e < 32 * k; e += 256) {
+        const int nn = e / k, l = e - nn * k;
+        const int n = n0 + nn;
+        Bs[nn][l] = n < d.N ? Bf[(size_t)n * ldb + l] : zc<Z>(0, 0);
+    }
+    __syncthreads();
+    Z* out = out_tab ? out_tab[f] : out_single + (size_t)f * out_stride;
+    for (int e = tid; e < 32 * k; e += 256) {
+        const int c = e / 32, nn = e - c * 32;
+        const int n = n0 + nn;
+        if (n >= d.N) continue;
+        Z sacc = zc<Z>(0, 0);
+        for (int l = 0; l < k; l++) sacc = zfma(ci[c * k + l], Bs[nn][l], sacc);
+        if (s && s[c] < 0) sacc = zc<Z>(-sacc.x, -sacc.y);
+        out[(size_t)(out_row0 + c) * ld_out + n] = sacc;
+    }
+}
+
+// k > 32: a workgroup takes 32 rows n (B rows staged in LDS) and every c; thread (tn, tcg) keeps the 4 x 3
 // outputs n = tn + 8 a, c = tcg + 32 b in registers, reading capinv 16 columns at a time from an
 // LDS copy, so that each l costs 7 LDS reads for 12 updates.  Every output's sum runs over l in
 // ascending order from zero.  k <= 96.
@@ -2365,8 +2397,12 @@ void launch_apply_capinv(hipStream_t st, const Dims& d, const Z* acc, int acc_st
                          Z* out_single, size_t out_stride, int out_row0, int ld_out, const Z* const* b_tab,
                          int b_col0) {
     dim3 grid((d.N + 31) / 32, d.nf);
-    hipLaunchKernelGGL(apply_capinv_kernel<Z>, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, b_tab,
-                       b_col0, ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
+    if (k <= 32)
+        hipLaunchKernelGGL(apply_capinv_small_kernel<Z>, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride,
+                           b_tab, b_col0, ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
+    else
+        hipLaunchKernelGGL(apply_capinv_kernel<Z>, grid, dim3(256), 0, st, d, acc, acc_stride, Bt, bt_stride, b_tab,
+                           b_col0, ldb, k, s, out_tab, out_single, out_stride, out_row0, ld_out);
 }
 
 // q += A_new w ;  R[r] -= Rt[r] M_new   (O(kN) parts of the LOWRANK accept).  A workgroup of 128
